@@ -1,0 +1,301 @@
+"""ctypes view of the C oracle (oracle/_ref/liboracle.so) and a runner for the
+reference dump harness (oracle/_ref/ref_harness).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker, never by the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(HERE, "_ref")
+LIB_PATH = os.path.join(REF_DIR, "liboracle.so")
+HARNESS = os.path.join(REF_DIR, "ref_harness")
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE, "lib"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        dp, ip, vp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p
+        L.orc_cluster_new.restype = vp
+        L.orc_cluster_new.argtypes = [C.c_int, dp, dp, vp, C.c_double, C.c_double, C.c_int]
+        L.orc_cluster_free.argtypes = [vp]
+        L.orc_cluster_neqns.argtypes = [vp]
+        L.orc_cluster_orders.argtypes = [vp, ip, ip]
+        L.orc_cluster_mie.argtypes = [vp, vp]
+        L.orc_mie_mult.argtypes = [vp, C.c_int, C.c_int, vp, vp]
+        L.orc_external_to_external.argtypes = [vp, C.c_int, ip, vp, vp]
+        L.orc_sphereinteraction.argtypes = [vp, C.c_int, ip, ip, C.c_int, vp, vp]
+        L.orc_planewave.argtypes = [vp, C.c_double, C.c_double, C.c_int, vp]
+        L.orc_cbicg.argtypes = [vp, C.c_int, C.c_int, C.c_double, vp, vp, dp]
+        L.orc_qeff.argtypes = [vp, vp, vp, dp]
+        L.orc_qtot.argtypes = [vp, dp, dp]
+        L.orc_fixedorsoln.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, vp, dp, dp,
+                                      ip]
+        L.orc_node_selection.argtypes = [vp, C.c_double, C.c_int, C.c_int, ip, dp, ip]
+        L.orc_fft_external_to_external.argtypes = [vp, C.c_int, ip, vp, vp]
+        L.orc_local_pairs.argtypes = [vp, ip]
+        L.orc_rotcoef.argtypes = [C.c_double, C.c_int, C.c_int, dp]
+        L.orc_normalizedlegendre.argtypes = [C.c_double, C.c_int, C.c_int, dp]
+        L.orc_atcdim.argtypes = [C.c_int, C.c_int]
+        L.orc_moffset.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.orc_axialtrancoef.argtypes = [C.c_int, C.c_double, vp, C.c_int, C.c_int, vp]
+        L.orc_gentranmatrix.argtypes = [C.c_int, C.c_int, dp, vp, C.c_int, vp]
+        L.orc_riccati_psi.argtypes = [C.c_int, C.c_double, dp]
+        L.orc_riccati_xi.argtypes = [C.c_int, C.c_double, vp]
+        L.orc_logderiv.argtypes = [C.c_int, C.c_double, C.c_double, vp]
+        L.orc_pair_translate.argtypes = [C.c_int, dp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]
+        L.orc_mie_order.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_mie_coef.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, vp, dp, dp]
+        L.orc_cg.restype = C.c_double
+        L.orc_cg.argtypes = [C.c_int] * 5
+        L.orc_bcof.restype = C.c_double
+        L.orc_bcof.argtypes = [C.c_int, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _z2(z):
+    return (C.c_double * 2)(z.real, z.imag)
+
+
+class OracleCluster:
+    """Cluster state of the C oracle (spheredata + mie arrays)."""
+
+    def __init__(self, pos, radius, ri, ri_medium=1.0, mie_eps=1e-6, switch_order=3):
+        L = lib()
+        self.pos = np.ascontiguousarray(pos, dtype=np.float64)
+        self.radius = np.ascontiguousarray(radius, dtype=np.float64)
+        self.ri = np.ascontiguousarray(ri, dtype=np.complex128)
+        self.n = len(self.radius)
+        self.h = L.orc_cluster_new(self.n, _d(self.pos), _d(self.radius), self.ri.ctypes.data, float(complex(ri_medium).real),
+                                   mie_eps, switch_order)
+        self.neqns = L.orc_cluster_neqns(self.h)
+        self.order = np.zeros(self.n, np.int32)
+        self.offset = np.zeros(self.n + 1, np.int32)
+        L.orc_cluster_orders(self.h, _i(self.order), _i(self.offset))
+        self.fft = None
+
+    def __del__(self):
+        try:
+            lib().orc_cluster_free(self.h)
+        except Exception:
+            pass
+
+    def mie(self):
+        out = np.zeros(4 * int(self.order.sum()), np.complex128)
+        lib().orc_cluster_mie(self.h, out.ctypes.data)
+        return out
+
+    def mie_mult(self, a, idir=1, inverse=False):
+        a = np.ascontiguousarray(a, np.complex128)
+        out = np.zeros_like(a)
+        lib().orc_mie_mult(self.h, idir, int(inverse), a.ctypes.data, out.ctypes.data)
+        return out
+
+    def external_to_external(self, ain, contran):
+        ain = np.ascontiguousarray(ain, np.complex128)
+        nrhs = ain.shape[0]
+        out = np.zeros_like(ain)
+        ct = np.array(contran, np.int32)
+        lib().orc_external_to_external(self.h, nrhs, _i(ct), ain.ctypes.data, out.ctypes.data)
+        return out
+
+    def sphereinteraction(self, ain, contran, miemult, use_fft=False):
+        """ain: (nrhs, neqns).  Returns aout (nrhs, neqns)."""
+        ain = np.ascontiguousarray(ain, np.complex128)
+        nrhs = ain.shape[0]
+        out = np.zeros_like(ain)
+        ct = np.array(contran, np.int32)
+        mm = np.array(miemult, np.int32)
+        lib().orc_sphereinteraction(self.h, nrhs, _i(ct), _i(mm), int(use_fft), ain.ctypes.data, out.ctypes.data)
+        return out
+
+    def planewave(self, alpha=0.0, sinc=0.0, direction=1):
+        out = np.zeros((2, self.neqns), np.complex128)
+        lib().orc_planewave(self.h, alpha, sinc, direction, out.ctypes.data)
+        return out
+
+    def cbicg(self, pnp, anp0, niter=10000, eps=1e-6, use_fft=False):
+        anp = np.ascontiguousarray(anp0, np.complex128).copy()
+        pnp = np.ascontiguousarray(pnp, np.complex128)
+        err = np.zeros(1)
+        it = lib().orc_cbicg(self.h, int(use_fft), niter, eps, pnp.ctypes.data, anp.ctypes.data, _d(err))
+        return anp, it, float(err[0])
+
+    def fixedorsoln(self, alpha=0.0, sinc=0.0, eps=1e-6, niter=10000, use_fft=False):
+        amnp = np.zeros((2, self.neqns), np.complex128)
+        qeff = np.zeros((self.n, 3, 3))
+        maxerr = np.zeros(1)
+        maxiter = np.zeros(1, np.int32)
+        istat = lib().orc_fixedorsoln(self.h, int(use_fft), alpha, sinc, 1, eps, niter, amnp.ctypes.data, _d(qeff),
+                                      _d(maxerr), _i(maxiter))
+        return dict(amnp=amnp, qeff=qeff, maxerr=float(maxerr[0]), iterations=int(maxiter[0]), istat=istat)
+
+    def qeff(self, amnp, pmnp):
+        q = np.zeros((self.n, 3, 3))
+        lib().orc_qeff(self.h, np.ascontiguousarray(amnp).ctypes.data, np.ascontiguousarray(pmnp).ctypes.data, _d(q))
+        return q
+
+    def qtot(self, qeff):
+        q = np.zeros((3, 3))
+        lib().orc_qtot(self.h, _d(np.ascontiguousarray(qeff)), _d(q))
+        return q
+
+    def node_selection(self, fva=0.0, node_order=-1, neighbor_model=2):
+        cd = np.zeros(3, np.int32)
+        d = np.zeros(1)
+        no = np.zeros(1, np.int32)
+        lib().orc_node_selection(self.h, fva, node_order, neighbor_model, _i(cd), _d(d), _i(no))
+        self.fft = dict(cell_dim=cd.copy(), d_cell=float(d[0]), node_order=int(no[0]))
+        return self.fft
+
+    def local_pairs(self):
+        n = lib().orc_local_pairs(self.h, None)
+        p = np.zeros((n, 2), np.int32)
+        lib().orc_local_pairs(self.h, _i(p))
+        return p
+
+
+def rotcoef(cbe, kmax, nmax):
+    out = np.zeros(((nmax + 1) ** 2, 2 * kmax + 1))
+    lib().orc_rotcoef(cbe, kmax, nmax, _d(out))
+    return out  # out[n(n+1)+k, m+kmax] = d^n_{m k}
+
+
+def axialtrancoef(itype, r, nmax, lmax, ri=1.0):
+    L = lib()
+    ri2 = np.array([ri, ri], np.complex128)
+    out = np.zeros(L.orc_atcdim(nmax, lmax), np.complex128)
+    L.orc_axialtrancoef(itype, r, ri2.ctypes.data, nmax, lmax, out.ctypes.data)
+    return out
+
+
+def gentranmatrix(nodr_s, nodr_t, xp, itype=3, ri=1.0):
+    ri2 = np.array([ri, ri], np.complex128)
+    xp = np.ascontiguousarray(xp, np.float64)
+    out = np.zeros((2, nodr_s * (nodr_s + 2), nodr_t * (nodr_t + 2)), np.complex128)
+    lib().orc_gentranmatrix(nodr_s, nodr_t, _d(xp), ri2.ctypes.data, itype, out.ctypes.data)
+    return out  # out[p, source, target]
+
+
+def pair_translate(itype, rtran, noa, nog, sop, top, ain, force_rot=False, ri=1.0):
+    ri2 = np.array([ri, ri], np.complex128)
+    g = np.zeros(2 * nog * (nog + 2), np.complex128)
+    a = np.ascontiguousarray(ain, np.complex128)
+    rt = np.ascontiguousarray(rtran, np.float64)
+    lib().orc_pair_translate(itype, _d(rt), ri2.ctypes.data, noa, nog, int(sop), int(top), int(force_rot),
+                             a.ctypes.data, g.ctypes.data)
+    return g
+
+
+def riccati_psi(n, x):
+    out = np.zeros(n + 1)
+    lib().orc_riccati_psi(n, x, _d(out))
+    return out
+
+
+def riccati_xi(n, x):
+    out = np.zeros(n + 1, np.complex128)
+    lib().orc_riccati_xi(n, x, out.ctypes.data)
+    return out
+
+
+def logderiv(n, z):
+    out = np.zeros(n + 1, np.complex128)
+    z = complex(z)
+    lib().orc_logderiv(n, z.real, z.imag, out.ctypes.data)
+    return out
+
+
+def mie_order(x, m, mmed=1.0, qeps=1e-6):
+    m = complex(m)
+    return lib().orc_mie_order(x, m.real, m.imag, float(complex(mmed).real), qeps)
+
+
+def mie_coef(x, m, nodr, mmed=1.0):
+    out = np.zeros(4 * nodr, np.complex128)
+    qe = np.zeros(1)
+    qs = np.zeros(1)
+    m = complex(m)
+    lib().orc_mie_coef(x, m.real, m.imag, float(complex(mmed).real), nodr, out.ctypes.data, _d(qe), _d(qs))
+    return out.reshape(nodr, 2, 2).transpose(0, 2, 1), float(qe[0]), float(qs[0])  # [n-1, p, q]
+
+
+# ---------------------------------------------------------------- reference
+def have_reference_harness() -> bool:
+    return os.path.exists(HARNESS)
+
+
+def run_reference(pos, radius, ri, mode, ain=None, outdir=None, timeout=None, **nml):
+    """Run the reference dump harness on a cluster; returns dict of outputs."""
+    if not have_reference_harness():
+        raise FileNotFoundError(HARNESS)
+    own = outdir is None
+    outdir = outdir or tempfile.mkdtemp(prefix="mstmref_")
+    posfile = os.path.join(outdir, "cluster.pos")
+    with open(posfile, "w") as f:
+        for p, r, m in zip(pos, radius, ri):
+            f.write(f"{p[0]:.17e} {p[1]:.17e} {p[2]:.17e} {r:.17e} {m.real:.17e} {m.imag:.17e}\n")
+    prefix = os.path.join(outdir, "out")
+    items = dict(mode=f"'{mode}'", posfile=f"'{posfile}'", nsphere=len(radius), outprefix=f"'{prefix}'")
+    if ain is not None:
+        infile = os.path.join(outdir, "ain.bin")
+        np.ascontiguousarray(ain, np.complex128).tofile(infile)
+        items["infile"] = f"'{infile}'"
+    for k, v in nml.items():
+        if isinstance(v, bool):
+            items[k] = ".true." if v else ".false."
+        elif isinstance(v, float):
+            items[k] = repr(v).replace("e", "d") if "e" in repr(v) else repr(v) + "d0"
+        else:
+            items[k] = str(v)
+    jobfile = os.path.join(outdir, "job.nml")
+    with open(jobfile, "w") as f:
+        f.write("&job\n" + ",\n".join(f" {k}={v}" for k, v in items.items()) + "\n/\n")
+    subprocess.run([HARNESS, jobfile], check=True, cwd=outdir, timeout=timeout,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    res = {}
+    with open(prefix + "_info.txt") as f:
+        for line in f:
+            v = line.split()
+            key, vals = v[0], v[1:]
+            nums = [float(x.replace("D", "e")) for x in vals]
+            res[key] = nums[0] if len(nums) == 1 else nums
+    for name in os.listdir(outdir):
+        if name.startswith("out_") and name.endswith(".bin"):
+            key = name[4:-4]
+            raw = open(os.path.join(outdir, name), "rb").read()
+            if key in ("order", "offset"):
+                res[key] = np.frombuffer(raw, np.int32).copy()
+            elif key in ("qeff", "qtot", "rotcoef"):
+                res[key] = np.frombuffer(raw, np.float64).copy()
+            else:
+                res[key] = np.frombuffer(raw, np.complex128).copy()
+    if own:
+        import shutil
+        shutil.rmtree(outdir, ignore_errors=True)
+    return res

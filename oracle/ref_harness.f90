@@ -1,0 +1,220 @@
+!
+! ref_harness.f90 -- full-precision dump harness around the reference MSTM
+! modules (december2023/: intrinsics, mpidefs-serial, modules, fft_translation,
+! scatprops, solver), which oracle/Makefile compiles unmodified from where they
+! lie.  TEST INFRASTRUCTURE ONLY: it writes golden vectors and times the
+! reference for the CPU baseline.  It replays the setup order of
+! main_calling_program (mstm-input-37.f90:1060-1262) for a cluster read from
+! a position file, then runs one job:
+!   mode='setup'    orders, offsets, Mie coefficients
+!   mode='matvec'   sphereinteraction on two RHS read from infile
+!                   (con_tran = F,T; mie_mult = T,T), plus the bare
+!                   external_to_external_expansion of the same RHS
+!   mode='solve'    fixedorsoln (BiCG) + qtotcalc
+!   mode='special'  rotcoef / axialtrancoefrecurrence / gentranmatrix samples
+! All outputs are raw little-endian stream files <outprefix>_<name>.bin plus
+! a small text summary <outprefix>_info.txt.
+!
+      program ref_harness
+      use mpidefs
+      use intrinsics
+      use numconstants
+      use specialfuncs
+      use surface_subroutines
+      use spheredata
+      use mie
+      use translation
+      use fft_translation
+      use scatprops
+      use solver
+      implicit none
+      character(len=16) :: mode
+      character(len=256) :: posfile,infile,outprefix,jobfile
+      integer :: nsphere,niter,node_order_in,nbr_model,i,n,ios,istat,maxiter,ncol,reps,r
+      real(8) :: mie_eps,eps,cell_vf,alpha,sinc,vr,maxerr,t0,t1,rmed
+      real(8) :: line(6),tmin(3),tmax(3),xg
+      logical :: use_fft,store_tables
+      complex(8), allocatable :: ain(:,:),aout(:,:),gout(:,:),amnp(:,:)
+      real(8), allocatable :: qeff(:,:,:),qvabs(:,:)
+      real(8) :: qtot(3,3)
+      namelist /job/ mode,posfile,nsphere,mie_eps,use_fft,cell_vf,node_order_in,nbr_model, &
+         eps,niter,alpha,sinc,infile,outprefix,reps,store_tables,rmed
+
+      mode='setup'
+      posfile='cluster.pos'
+      infile='ain.bin'
+      outprefix='out'
+      nsphere=0
+      mie_eps=1.d-6
+      use_fft=.false.
+      cell_vf=0.d0
+      node_order_in=-1
+      nbr_model=2
+      eps=1.d-6
+      niter=10000
+      alpha=0.d0
+      sinc=0.d0
+      reps=1
+      store_tables=.false.
+      rmed=1.d0
+      call get_command_argument(1,jobfile)
+      open(10,file=trim(jobfile),status='old')
+      read(10,nml=job)
+      close(10)
+
+      call mstm_mpi(mpi_command='init')
+      number_spheres=nsphere
+      allocate(sphere_radius(nsphere),sphere_position(3,nsphere),sphere_ref_index(2,0:nsphere), &
+         host_sphere(nsphere),number_field_expansions(nsphere))
+      open(11,file=trim(posfile),status='old')
+      do i=1,nsphere
+         read(11,*) line(1:6)
+         sphere_position(:,i)=line(1:3)
+         sphere_radius(i)=line(4)
+         sphere_ref_index(1,i)=dcmplx(line(5),line(6))
+         sphere_ref_index(2,i)=sphere_ref_index(1,i)
+      enddo
+      close(11)
+      layer_ref_index(0)=dcmplx(rmed,0.d0)
+      sphere_ref_index(:,0)=layer_ref_index(0)
+      store_translation_matrix=store_tables
+      plane_surface_present=.false.
+      call plane_boundary_initialization()
+      call findhostspheres()
+      call sphere_layer_initialization()
+      call miecoefcalc(mie_eps)
+      call init(max_mie_order)
+      call incident_field_initialization(alpha,sinc,1)
+      fft_translation_option=use_fft
+      if(use_fft) then
+         cell_volume_fraction=cell_vf
+         tmin=sphere_min_position
+         tmax=sphere_max_position
+         call clear_fft_matrix(clear_h=.true.)
+         neighbor_node_model=nbr_model
+         call node_selection(cell_volume_fraction,target_min=tmin,target_max=tmax, &
+            d_specified=.false.,local_host=0)
+         if(node_order_in.le.0) then
+            node_order=-node_order_in+ceiling(d_cell)
+         else
+            node_order=node_order_in
+         endif
+      endif
+
+      open(20,file=trim(outprefix)//'_info.txt')
+      write(20,'(a,i10)') 'nsphere ',nsphere
+      write(20,'(a,i10)') 'neqns ',number_eqns
+      write(20,'(a,i10)') 'max_mie_order ',max_mie_order
+      if(use_fft) then
+         write(20,'(a,3i8)') 'cell_dim ',cell_dim
+         write(20,'(a,es25.17)') 'd_cell ',d_cell
+         write(20,'(a,i8)') 'node_order ',node_order
+         write(20,'(a,es25.17)') 'cell_volume_fraction ',cell_volume_fraction
+      endif
+      call dump_i('order',sphere_order,nsphere)
+      call dump_i('offset',sphere_offset,nsphere+1)
+      call dump_z('anmie',an_mie,size(an_mie))
+
+      if(trim(mode).eq.'matvec') then
+         allocate(ain(number_eqns,2),aout(number_eqns,2),gout(number_eqns,2))
+         open(12,file=trim(infile),access='stream',form='unformatted',status='old')
+         read(12) ain
+         close(12)
+         t0=mstm_mpi_wtime()
+         do r=1,reps
+            call sphereinteraction(number_eqns,2,ain,aout,initial_run=(r.eq.1), &
+               con_tran=(/.false.,.true./),mie_mult=(/.true.,.true./))
+         enddo
+         t1=mstm_mpi_wtime()
+         write(20,'(a,es25.17)') 'matvec_seconds ',(t1-t0)/dble(reps)
+         call dump_z('aout',aout,2*number_eqns)
+         if(.not.use_fft) then
+            call external_to_external_expansion(number_eqns,2,ain,gout, &
+               con_tran=(/.false.,.true./))
+            call dump_z('hout',gout,2*number_eqns)
+         endif
+      elseif(trim(mode).eq.'solve') then
+         allocate(amnp(number_eqns,2),qeff(3,3,nsphere),qvabs(3,nsphere))
+         vr=0.d0
+         do i=1,nsphere
+            vr=vr+sphere_radius(i)**3
+         enddo
+         xg=vr**.333333
+         t0=mstm_mpi_wtime()
+         call fixedorsoln(alpha,sinc,1,eps,niter,amnp,qeff,3,maxerr,maxiter,0,istat, &
+            solution_method='i',initialize_solver=.true.)
+         t1=mstm_mpi_wtime()
+         call qtotcalc(nsphere,3,xg,qeff,qvabs,qtot)
+         write(20,'(a,i10)') 'iterations ',maxiter
+         write(20,'(a,es25.17)') 'maxerr ',maxerr
+         write(20,'(a,i10)') 'istat ',istat
+         write(20,'(a,es25.17)') 'solve_seconds ',t1-t0
+         write(20,'(a,es25.17)') 'cross_section_radius ',xg
+         write(20,'(a,9es25.17)') 'qtot ',qtot
+         call dump_z('amnp',amnp,2*number_eqns)
+         call dump_d('qeff',qeff,9*nsphere)
+         call dump_d('qtot',qtot,9)
+      elseif(trim(mode).eq.'special') then
+         call special_dumps()
+      endif
+      close(20)
+
+      contains
+
+         subroutine dump_z(name,z,nz)
+         character(*) :: name
+         integer :: nz
+         complex(8) :: z(nz)
+         open(30,file=trim(outprefix)//'_'//name//'.bin',access='stream',form='unformatted', &
+            status='replace')
+         write(30) z
+         close(30)
+         end subroutine dump_z
+
+         subroutine dump_d(name,d,nd)
+         character(*) :: name
+         integer :: nd
+         real(8) :: d(nd)
+         open(30,file=trim(outprefix)//'_'//name//'.bin',access='stream',form='unformatted', &
+            status='replace')
+         write(30) d
+         close(30)
+         end subroutine dump_d
+
+         subroutine dump_i(name,iv,ni)
+         character(*) :: name
+         integer :: ni,iv(ni)
+         open(30,file=trim(outprefix)//'_'//name//'.bin',access='stream',form='unformatted', &
+            status='replace')
+         write(30) iv
+         close(30)
+         end subroutine dump_i
+!
+! special functions at fixed sample points: Wigner d, axial coefficients
+! (both types, unequal orders) and the dense translation matrix.
+!
+         subroutine special_dumps()
+         integer :: nt,ns,td
+         real(8) :: dc(-4:4,0:48),xp(3)
+         complex(8) :: ri(2),ac(2000),gm(48,35,2)
+         call rotcoef(0.3d0,4,6,dc)
+         call dump_d('rotcoef',dc,size(dc))
+         ri=(1.d0,0.d0)
+         nt=6
+         ns=4
+         td=atcdim(nt,ns)
+         write(20,'(a,i8)') 'atcdim64 ',td
+         call axialtrancoefrecurrence(3,7.25d0,ri,nt,ns,td,ac)
+         call dump_z('axial3',ac,td)
+         call axialtrancoefrecurrence(1,2.75d0,ri,nt,ns,td,ac)
+         call dump_z('axial1',ac,td)
+         xp=(/1.5d0,-2.25d0,3.0d0/)
+         call gentranmatrix(5,6,translation_vector=xp,refractive_index=ri,ac_matrix=gm, &
+            vswf_type=3,mode_s=2,mode_t=2)
+         call dump_z('gentran3',gm,size(gm))
+         call gentranmatrix(5,6,translation_vector=xp,refractive_index=ri,ac_matrix=gm, &
+            vswf_type=1,mode_s=2,mode_t=2)
+         call dump_z('gentran1',gm,size(gm))
+         end subroutine special_dumps
+
+      end program ref_harness
