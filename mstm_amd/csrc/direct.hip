@@ -1,0 +1,371 @@
+// direct.hip -- direct (pair-by-pair) sphere interaction H.a on MI355X.
+//
+// Replaces the pair loop of external_to_external_expansion
+// (mstm-modules-33.f90:6390-6525) and its per-pair coefficient_translation
+// (rot path, :6697-6749).  Work decomposition:
+//   * spheres are grouped in blocks of S = MSTM_SLOTS; a tile is an ordered
+//     block pair (bI <= bJ) and is one 128-lane workgroup;
+//   * wave 0 computes I <- J (forward, sop = .false.), wave 1 J <- I
+//     (reverse, sop = .true., which adds the m-flip of shiftcoefficient and
+//     uses the axial block transposed) from the same tables;
+//   * lane = slot k (16) x column (rhs, mode) (4).  In step s lane k of
+//     wave 0 targets I[k] and sources J[(k+s)%S]; lane k of wave 1 targets
+//     J[k] and sources I[(k-s)%S].  A lane's target never changes inside a
+//     tile, so it accumulates in registers and writes one partial sum per
+//     (target, partner block): no atomics, deterministic order;
+//   * pair tables are laid out in HBM in exactly this consumption order and
+//     interleaved over the 16 slots, so each table load is 16 x 16 B.
+// The reduce kernel sums the partials and applies the Mie/con_tran post-op
+// of sphereinteraction (mstm-scatprops-26.f90:158-168).
+#include <utility>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+template <typename T, int VW> struct vec_of;
+template <> struct vec_of<float, 4> {
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+template <> struct vec_of<double, 2> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+
+template <typename F, int... Is>
+__device__ __forceinline__ void sfor_impl(F &&f, std::integer_sequence<int, Is...>)
+{
+    (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, typename F> __device__ __forceinline__ void static_for(F &&f)
+{
+    sfor_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// Load LEN elements of a slot section starting at element E0 (multiple of
+// VW) into r[]; p0 = group base + slot*VW (+ a runtime chunk offset).
+template <typename T, int E0, int LEN>
+__device__ __forceinline__ void load_sec(const T *__restrict__ p0, T *r)
+{
+    constexpr int VW = 16 / sizeof(T);
+    typedef typename vec_of<T, VW>::type V;
+#pragma unroll
+    for (int c = 0; c < (LEN + VW - 1) / VW; c++) {
+        const V *p = reinterpret_cast<const V *>(p0 + (int64_t)(E0 / VW + c) * MSTM_SLOTS * VW);
+        V v = __builtin_nontemporal_load(p);
+#pragma unroll
+        for (int q = 0; q < VW; q++)
+            if (c * VW + q < LEN) r[c * VW + q] = v[q];
+    }
+}
+
+template <typename T, int L>
+__global__ __launch_bounds__(128) void direct_tile_kernel(const T *__restrict__ tab, const int2 *__restrict__ tiles,
+                                                         const cplx<T> *__restrict__ xin, cplx<T> *__restrict__ part,
+                                                         int npad)
+{
+    constexpr int VW = 16 / sizeof(T);
+    constexpr int NB = L * (L + 2);
+    constexpr int SL = slot_len(L, L, VW);
+    constexpr int S = MSTM_SLOTS;
+    const int lane = threadIdx.x & 63;
+    const int dir = threadIdx.x >> 6;  // wave-uniform
+    const int kslot = lane >> 2, rhs = (lane >> 1) & 1, mode = lane & 1;
+    const int tile = blockIdx.x;
+    const int2 bt = tiles[tile];
+    const bool diag = bt.x == bt.y;
+    if (diag && dir) return;
+    const int tgt = (dir ? bt.y : bt.x) * S + kslot;
+
+    cplx<T> W[NB];
+#pragma unroll
+    for (int q = 0; q < NB; q++) W[q] = cmk((T)0, (T)0);
+
+    for (int s = 0; s < S; s++) {
+        if (diag && s == 0) continue;
+        int slot, src;
+        if (!dir) {
+            slot = kslot;
+            src = bt.y * S + ((kslot + s) & (S - 1));
+        } else {
+            slot = (kslot - s + S) & (S - 1);
+            src = bt.x * S + slot;
+        }
+        const T *p0 = tab + ((int64_t)tile * S + s) * (int64_t)SL * S + slot * VW;
+        // exp(i sigma k phi), sigma = +1 forward, -1 reverse (im of :6690)
+        T pv[2];
+        load_sec<T, phi_off(L, L, VW), 2>(p0, pv);
+        cplx<T> e1 = cmk(pv[0], dir ? -pv[1] : pv[1]);
+        cplx<T> ph[2 * L + 1];
+        ph[L] = cmk((T)1, (T)0);
+#pragma unroll
+        for (int k = 1; k <= L; k++) {
+            ph[L + k] = cmul(ph[L + k - 1], e1);
+            ph[L - k] = cconj(ph[L + k]);
+        }
+        // source coefficients: m-flip for the reverse direction, then phase
+        const cplx<T> *xs = xin + ((int64_t)(rhs * npad + src) * 2 + mode) * NB;
+        cplx<T> xp[NB];
+#pragma unroll
+        for (int n = 1; n <= L; n++)
+#pragma unroll
+            for (int k = -n; k <= n; k++) {
+                cplx<T> v;
+                if (!dir) v = xs[amn(k, n, L)];
+                else {
+                    v = xs[amn(-k, n, L)];
+                    if (k & 1) v = cmk(-v.re, -v.im);
+                }
+                xp[nk(n, k)] = cmul(ph[L + k], v);
+            }
+        static_for<2 * L + 1>([&](auto bc) {
+            constexpr int b = decltype(bc)::value;
+            constexpr int m = mblock_m(b);
+            constexpr int m1 = imx(1, iab(m));
+            constexpr int E0 = sec_off(b, L, L, VW);
+            constexpr int RL = rot_len(m, L, L);
+            constexpr int AL = ax_len(m, L, L);
+            constexpr int AP = pad_to(AL, VW);
+            constexpr int ND = L - m1 + 1;
+            T dr[RL];
+            load_sec<T, E0, RL>(p0, dr);
+            // forward rotation: y(n) = sum_k d^n(m,k) xp(n,k)
+            cplx<T> y[ND];
+#pragma unroll
+            for (int n = m1; n <= L; n++) {
+                int ro = 0;
+#pragma unroll
+                for (int q = m1; q < n; q++) ro += 2 * q + 1;
+                cplx<T> acc = cmk((T)0, (T)0);
+#pragma unroll
+                for (int k = -n; k <= n; k++) crfma(acc, dr[ro + k + n], xp[nk(n, k)]);
+                y[n - m1] = acc;
+            }
+            // axial translation of this m block (mode-specific matrix)
+            T da[AL];
+            load_sec<T, E0 + pad_to(RL, VW), AL>(p0 + (int64_t)mode * (AP / VW) * S * VW, da);
+            cplx<T> z[ND];
+            if (!dir) {
+#pragma unroll
+                for (int t = 0; t < ND; t++) {
+                    cplx<T> acc = cmk((T)0, (T)0);
+#pragma unroll
+                    for (int q = 0; q < ND; q++) cfma(acc, cmk(da[2 * (t + ND * q)], da[2 * (t + ND * q) + 1]), y[q]);
+                    z[t] = acc;
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < ND; t++) {
+                    cplx<T> acc = cmk((T)0, (T)0);
+#pragma unroll
+                    for (int q = 0; q < ND; q++) cfma(acc, cmk(da[2 * (q + ND * t)], da[2 * (q + ND * t) + 1]), y[q]);
+                    z[t] = acc;
+                }
+            }
+            // back rotation with the output phase folded in:
+            // W(t,k) += d^t(m,k) exp(-i sigma k phi) z(t)
+#pragma unroll
+            for (int t = m1; t <= L; t++) {
+                int ro = 0;
+#pragma unroll
+                for (int q = m1; q < t; q++) ro += 2 * q + 1;
+#pragma unroll
+                for (int k = -t; k <= t; k++) {
+                    cplx<T> f = cscale(ph[L - k], dr[ro + k + t]);
+                    cfma(W[nk(t, k)], f, z[t - m1]);
+                }
+            }
+        });
+    }
+    // partial for (target, partner block); m-flip back for the reverse wave
+    const int partner = dir ? bt.x : bt.y;
+    cplx<T> *out = part + (((int64_t)partner * 2 + rhs) * npad + tgt) * 2 * NB + mode * NB;
+#pragma unroll
+    for (int n = 1; n <= L; n++)
+#pragma unroll
+        for (int k = -n; k <= n; k++) {
+            cplx<T> v;
+            if (!dir) v = W[nk(n, k)];
+            else {
+                v = W[nk(n, -k)];
+                if (k & 1) v = cmk(-v.re, -v.im);
+            }
+            out[amn(k, n, L)] = v;
+        }
+}
+
+template <typename T>
+hipError_t launch_direct_tiles(const T *tab, const int2 *tiles, int ntiles, int L, const T *xin, T *partial, int npad,
+                               hipStream_t st)
+{
+    const cplx<T> *x = reinterpret_cast<const cplx<T> *>(xin);
+    cplx<T> *p = reinterpret_cast<cplx<T> *>(partial);
+    dim3 g(ntiles), b(128);
+    switch (L) {
+#define CASE(LL)                                                                                                       \
+    case LL: hipLaunchKernelGGL((direct_tile_kernel<T, LL>), g, b, 0, st, tab, tiles, x, p, npad); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+template hipError_t launch_direct_tiles<float>(const float *, const int2 *, int, int, const float *, float *, int,
+                                               hipStream_t);
+template hipError_t launch_direct_tiles<double>(const double *, const int2 *, int, int, const double *, double *, int,
+                                                hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Input preparation (sphereinteraction :88-97): for a con_tran column
+// x = S T^T conj(a) (S = m-flip with (-1)^m, T^T the transposed 2x2 Mie
+// matrix when mie_mult); otherwise x = a.  The tile kernels work on a
+// uniform-order copy (order Lp = max order, npad spheres); coefficients of
+// degree n > order(i) and padding spheres are zero, which is exact because
+// a translation table of order Lp contains those of lower orders.
+// flags[r] (host array) bit0 = con_tran, bit1 = mie_mult.  One thread per
+// (r, sphere, e).
+// an: Mie matrices, 4 complex per degree, at (sphere*Lp + n-1)*4.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void decode_mn(int e, int L, int &m, int &n)
+{
+    int row = e % (L + 2), colm = e / (L + 2) + 1;
+    if (row <= colm) {
+        m = row;
+        n = colm;
+    } else {
+        m = -colm;
+        n = row - 1;
+    }
+}
+
+template <typename T>
+__global__ void prep_kernel(const cplx<double> *__restrict__ ain, int ldain, cplx<T> *__restrict__ xin, int nrhs,
+                            int flag0, int flag1, int nsph, int npad, int Lp, const int *__restrict__ order,
+                            const int *__restrict__ offset, const cplx<double> *__restrict__ an)
+{
+    const int NBP = Lp * (Lp + 2);
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)2 * npad * NBP) return;
+    int e = (int)(q % NBP);
+    int64_t t = q / NBP;
+    int sph = (int)(t % npad);
+    int r = (int)(t / npad);
+    cplx<T> *o = xin + ((int64_t)(r * npad + sph) * 2) * NBP;
+    int m, n;
+    decode_mn(e, Lp, m, n);
+    if (r >= nrhs || sph >= nsph || n > order[sph]) {
+        o[e] = cmk((T)0, (T)0);
+        o[e + NBP] = cmk((T)0, (T)0);
+        return;
+    }
+    const int L = order[sph], NB = L * (L + 2);
+    const int f = r == 0 ? flag0 : flag1;
+    const cplx<double> *a = ain + (int64_t)r * ldain + offset[sph];
+    if (!(f & 1)) {
+        int es = amn(m, n, L);
+        cplx<double> v0 = a[es], v1 = a[es + NB];
+        o[e] = cmk((T)v0.re, (T)v0.im);
+        o[e + NBP] = cmk((T)v1.re, (T)v1.im);
+        return;
+    }
+    // flipped value at (m,n) is (-1)^m (T^T conj a)(-m,n)
+    int es = amn(-m, n, L);
+    cplx<double> v0 = cconj(a[es]), v1 = cconj(a[es + NB]);
+    if (f & 2) {
+        const cplx<double> *t4 = an + ((int64_t)sph * Lp + (n - 1)) * 4;  // t4[p + 2q] = T(p,q)
+        cplx<double> w0 = cadd(cmul(t4[0], v0), cmul(t4[1], v1));
+        cplx<double> w1 = cadd(cmul(t4[2], v0), cmul(t4[3], v1));
+        v0 = w0;
+        v1 = w1;
+    }
+    double sg = (m & 1) ? -1.0 : 1.0;
+    o[e] = cmk((T)(sg * v0.re), (T)(sg * v0.im));
+    o[e + NBP] = cmk((T)(sg * v1.re), (T)(sg * v1.im));
+}
+
+template <typename T>
+hipError_t launch_prep(const double *ain, int ldain, T *xin, int nrhs, const int *flags, int nsph, int npad, int Lp,
+                       const int *order, const int *offset, const double *an, hipStream_t st)
+{
+    int f0 = flags[0], f1 = nrhs > 1 ? flags[1] : 0;
+    int NBP = Lp * (Lp + 2);
+    int64_t n = (int64_t)2 * npad * NBP;
+    int bs = 256;
+    hipLaunchKernelGGL(prep_kernel<T>, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, st,
+                       reinterpret_cast<const cplx<double> *>(ain), ldain / 2, reinterpret_cast<cplx<T> *>(xin), nrhs,
+                       f0, f1, nsph, npad, Lp, order, offset, reinterpret_cast<const cplx<double> *>(an));
+    return hipGetLastError();
+}
+template hipError_t launch_prep<float>(const double *, int, float *, int, const int *, int, int, int, const int *,
+                                       const int *, const double *, hipStream_t);
+template hipError_t launch_prep<double>(const double *, int, double *, int, const int *, int, int, int, const int *,
+                                        const int *, const double *, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Sum of the per-block partials + post-op (sphereinteraction :158-168):
+// normal column  out = T h  (mie_mult) or h;  con_tran column out = conj(S h).
+// Written back in the reference (unpadded) layout.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void reduce_post_kernel(const cplx<T> *__restrict__ part, int nparts, cplx<double> *__restrict__ aout,
+                                   int ldaout, int nrhs, int flag0, int flag1, int nsph, int npad, int Lp,
+                                   const int *__restrict__ order, const int *__restrict__ offset,
+                                   const cplx<double> *__restrict__ an)
+{
+    const int NBP = Lp * (Lp + 2);
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nrhs * nsph * NBP) return;
+    int e = (int)(q % NBP);
+    int64_t t = q / NBP;
+    int sph = (int)(t % nsph);
+    int r = (int)(t / nsph);
+    int m, n;
+    decode_mn(e, Lp, m, n);
+    const int L = order[sph];
+    if (n > L) return;
+    const int NB = L * (L + 2);
+    const int f = r == 0 ? flag0 : flag1;
+    int es = (f & 1) ? amn(-m, n, Lp) : e;
+    cplx<double> h0 = cmk(0.0, 0.0), h1 = cmk(0.0, 0.0);
+    const int64_t stride = (int64_t)2 * npad * 2 * NBP;
+    const cplx<T> *p = part + ((int64_t)r * npad + sph) * 2 * NBP;
+    for (int b = 0; b < nparts; b++) {
+        cplx<T> a0 = p[(int64_t)b * stride + es], a1 = p[(int64_t)b * stride + es + NBP];
+        h0.re += (double)a0.re;
+        h0.im += (double)a0.im;
+        h1.re += (double)a1.re;
+        h1.im += (double)a1.im;
+    }
+    cplx<double> *o = aout + (int64_t)r * ldaout + offset[sph];
+    int eo = amn(m, n, L);
+    if (f & 1) {
+        double sg = (m & 1) ? -1.0 : 1.0;
+        o[eo] = cmk(sg * h0.re, -sg * h0.im);
+        o[eo + NB] = cmk(sg * h1.re, -sg * h1.im);
+    } else if (f & 2) {
+        const cplx<double> *t4 = an + ((int64_t)sph * Lp + (n - 1)) * 4;
+        o[eo] = cadd(cmul(t4[0], h0), cmul(t4[2], h1));
+        o[eo + NB] = cadd(cmul(t4[1], h0), cmul(t4[3], h1));
+    } else {
+        o[eo] = h0;
+        o[eo + NB] = h1;
+    }
+}
+
+template <typename T>
+hipError_t launch_reduce_post(const T *partial, int nparts, double *aout, int ldaout, int nrhs, const int *flags,
+                              int nsph, int npad, int Lp, const int *order, const int *offset, const double *an,
+                              hipStream_t st)
+{
+    int f0 = flags[0], f1 = nrhs > 1 ? flags[1] : 0;
+    int NBP = Lp * (Lp + 2);
+    int64_t n = (int64_t)nrhs * nsph * NBP;
+    int bs = 256;
+    hipLaunchKernelGGL(reduce_post_kernel<T>, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, st,
+                       reinterpret_cast<const cplx<T> *>(partial), nparts, reinterpret_cast<cplx<double> *>(aout),
+                       ldaout / 2, nrhs, f0, f1, nsph, npad, Lp, order, offset,
+                       reinterpret_cast<const cplx<double> *>(an));
+    return hipGetLastError();
+}
+template hipError_t launch_reduce_post<float>(const float *, int, double *, int, int, const int *, int, int, int,
+                                              const int *, const int *, const double *, hipStream_t);
+template hipError_t launch_reduce_post<double>(const double *, int, double *, int, int, const int *, int, int, int,
+                                               const int *, const int *, const double *, hipStream_t);

@@ -1,0 +1,27 @@
+// fft.hpp -- FFT-accelerated sphere interaction (node grid) interface.
+#pragma once
+#include <hip/hip_runtime.h>
+
+struct FftOp;
+struct FftGeometry {
+    int nsph, Lmax, neqns;
+    const double *pos;     // host, 3*nsph
+    const double *radius;  // host
+    const int *order, *offset;
+    double rimed;
+    // device arrays owned by the context
+    const double *d_pos;
+    const int *d_order, *d_offset;
+    const double *d_an;
+    const double *d_vcc;
+    int vcc_LM;
+};
+FftOp *fft_create(const FftGeometry &g, double fva, int node_order, int neighbor_model, int *cell_dim, double *d_cell,
+                  int *node_order_out, int *err);
+int fft_setup(FftOp *f, int prec, hipStream_t st);
+int fft_matvec(FftOp *f, int nrhs, const double *ain, double *aout, const int *flags, int prec, hipStream_t st);
+void fft_clear(FftOp *f);
+void fft_destroy(FftOp *f);
+const char *fft_last_error(const FftOp *f);
+double fft_stat(const FftOp *f, int which);
+int fft_required_vcc(const FftOp *f);

@@ -1,0 +1,48 @@
+// kernels.hpp -- launcher declarations shared by the .hip translation units.
+#pragma once
+#include "common.hpp"
+
+// tables.hip
+template <typename T>
+hipError_t launch_direct_tables(T *tab, const int2 *tiles, int ntiles, int L, int nsph, const double *pos, double rimed,
+                                const double *vcc, int LM, hipStream_t st);
+template <typename T>
+hipError_t launch_list_tables(T *tab, const double *rtran, int nslots, int LT, int LS, int itype, double rimed,
+                              const double *vcc, int LM, hipStream_t st);
+
+// direct.hip
+template <typename T>
+hipError_t launch_direct_tiles(const T *tab, const int2 *tiles, int ntiles, int L, const T *xin, T *partial, int npad,
+                               hipStream_t st);
+template <typename T>
+hipError_t launch_prep(const double *ain, int ldain, T *xin, int nrhs, const int *flags, int nsph, int npad, int Lp,
+                       const int *order, const int *offset, const double *an, hipStream_t st);
+template <typename T>
+hipError_t launch_reduce_post(const T *partial, int nparts, double *aout, int ldaout, int nrhs, const int *flags,
+                              int nsph, int npad, int Lp, const int *order, const int *offset, const double *an,
+                              hipStream_t st);
+
+// mie.hip
+hipError_t launch_mie(const double *x, const double *mre, const double *mim, double mmed, int nsph, double qeps,
+                      int *order, double *an, int ldan, hipStream_t st);
+hipError_t launch_mie_mult(const double *ain, double *aout, int nsph, const int *order, const int *offset,
+                           const int *mieoff, const double *an, int idir, int inverse, int Lmax, hipStream_t st);
+hipError_t launch_planewave(double *pmnp, int ld, int nsph, const double *pos, const int *order, const int *offset,
+                            double alpha, double sinc, double rimed, hipStream_t st);
+hipError_t launch_qeff(const double *amnp, const double *pmnp, int ld, int nsph, const double *radius, const int *order,
+                       const int *offset, const int *mieoff, const double *an, double rimed, double *qeff,
+                       hipStream_t st);
+
+// krylov.hip (all vectors complex FP64, lengths in complex elements)
+int krylov_grid();
+hipError_t launch_zdot(const double *a, const double *b, int n, int conj_a, double *partials, double *result,
+                       hipStream_t st);
+hipError_t launch_bicg_a(double *cap, double *caw, const double *cp, const double *cw, int n, double *partials,
+                         hipStream_t st);
+hipError_t launch_bicg_b(double *anp, double *cr, double *cq, const double *cp, const double *cap, const double *caw,
+                         double cakr, double caki, int n, double *partials, hipStream_t st);
+hipError_t launch_bicg_c(double *cp, double *cw, const double *cr, const double *cq, double cbkr, double cbki, int n,
+                         hipStream_t st);
+hipError_t launch_bicg_init(double *cr, double *cq, double *cw, double *cp, const double *pnp, const double *anp, int n,
+                            double *partials, hipStream_t st);
+hipError_t launch_zaxpby(double ar, double ai, const double *x, double br, double bi, double *y, int n, hipStream_t st);

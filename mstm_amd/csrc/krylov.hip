@@ -1,0 +1,175 @@
+// krylov.hip -- FP64 vector kernels of the complex biconjugate-gradient
+// iteration (cbicg, mstm-solver-8.f90:654-949) and of the mixed-precision
+// refinement.  Inner products are written as per-block partial sums that the
+// host adds in a fixed order (deterministic results).
+#include "common.hpp"
+#include "kernels.hpp"
+
+#define KB 256  // threads per block
+#define KGRID 512
+
+__device__ __forceinline__ void block_sum3(cplx<double> *v, int nv)
+{
+    __shared__ cplx<double> sh[3][KB];
+    for (int q = 0; q < nv; q++) sh[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int s = KB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int q = 0; q < nv; q++) sh[q][threadIdx.x] = cadd(sh[q][threadIdx.x], sh[q][threadIdx.x + s]);
+        __syncthreads();
+    }
+    for (int q = 0; q < nv; q++) v[q] = sh[q][0];
+}
+
+// partial[b] = sum (conj_a ? conj(a) : a) * b over the block's grid-stride share
+__global__ void zdot_kernel(const cplx<double> *__restrict__ a, const cplx<double> *__restrict__ b, int n, int conj_a,
+                            cplx<double> *__restrict__ partials)
+{
+    cplx<double> s = cmk(0.0, 0.0);
+    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) {
+        cplx<double> x = a[i];
+        if (conj_a) x = cconj(x);
+        cfma(s, x, b[i]);
+    }
+    block_sum3(&s, 1);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+hipError_t launch_zdot(const double *a, const double *b, int n, int conj_a, double *partials, double *, hipStream_t st)
+{
+    hipLaunchKernelGGL(zdot_kernel, dim3(KGRID), dim3(KB), 0, st, reinterpret_cast<const cplx<double> *>(a),
+                       reinterpret_cast<const cplx<double> *>(b), n, conj_a, reinterpret_cast<cplx<double> *>(partials));
+    return hipGetLastError();
+}
+
+// cap = cp - cap ; caw = cw - caw ; partial <cw, cap>   (:897-903)
+__global__ void bicg_a_kernel(cplx<double> *__restrict__ cap, cplx<double> *__restrict__ caw,
+                              const cplx<double> *__restrict__ cp, const cplx<double> *__restrict__ cw, int n,
+                              cplx<double> *__restrict__ partials)
+{
+    cplx<double> s = cmk(0.0, 0.0);
+    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) {
+        cplx<double> a = csub(cp[i], cap[i]);
+        cap[i] = a;
+        caw[i] = csub(cw[i], caw[i]);
+        cfma(s, cconj(cw[i]), a);
+    }
+    block_sum3(&s, 1);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// anp += cak cp ; cr -= cak cap ; cq -= conj(cak) caw ;
+// partials <cq, cr>, <cr, cr>   (:912-920)
+__global__ void bicg_b_kernel(cplx<double> *__restrict__ anp, cplx<double> *__restrict__ cr,
+                              cplx<double> *__restrict__ cq, const cplx<double> *__restrict__ cp,
+                              const cplx<double> *__restrict__ cap, const cplx<double> *__restrict__ caw,
+                              cplx<double> cak, int n, cplx<double> *__restrict__ partials)
+{
+    cplx<double> s[2] = {cmk(0.0, 0.0), cmk(0.0, 0.0)};
+    cplx<double> cakc = cconj(cak);
+    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) {
+        cplx<double> a = anp[i];
+        cfma(a, cak, cp[i]);
+        anp[i] = a;
+        cplx<double> r = cr[i];
+        cfma(r, cscale(cak, -1.0), cap[i]);
+        cr[i] = r;
+        cplx<double> q = cq[i];
+        cfma(q, cscale(cakc, -1.0), caw[i]);
+        cq[i] = q;
+        cfma(s[0], cconj(q), r);
+        cfma(s[1], cconj(r), r);
+    }
+    block_sum3(s, 2);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = s[0];
+        partials[2 * blockIdx.x + 1] = s[1];
+    }
+}
+
+// cp = cr + cbk cp ; cw = cq + conj(cbk) cw   (:928-929)
+__global__ void bicg_c_kernel(cplx<double> *__restrict__ cp, cplx<double> *__restrict__ cw,
+                              const cplx<double> *__restrict__ cr, const cplx<double> *__restrict__ cq,
+                              cplx<double> cbk, int n)
+{
+    cplx<double> cbkc = cconj(cbk);
+    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) {
+        cplx<double> p = cr[i];
+        cfma(p, cbk, cp[i]);
+        cp[i] = p;
+        cplx<double> w = cq[i];
+        cfma(w, cbkc, cw[i]);
+        cw[i] = w;
+    }
+}
+
+// cr = pnp - anp + cr ; cq = cw = conj(cr) ; cp = cr ; partial sum cr*cr
+// (bilinear, :828-834)
+__global__ void bicg_init_kernel(cplx<double> *__restrict__ cr, cplx<double> *__restrict__ cq,
+                                 cplx<double> *__restrict__ cw, cplx<double> *__restrict__ cp,
+                                 const cplx<double> *__restrict__ pnp, const cplx<double> *__restrict__ anp, int n,
+                                 cplx<double> *__restrict__ partials)
+{
+    cplx<double> s = cmk(0.0, 0.0);
+    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) {
+        cplx<double> r = cadd(csub(pnp[i], anp[i]), cr[i]);
+        cr[i] = r;
+        cq[i] = cconj(r);
+        cw[i] = cconj(r);
+        cp[i] = r;
+        cfma(s, r, r);
+    }
+    block_sum3(&s, 1);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// y = a*x + b*y (complex scalars a, b)
+__global__ void zaxpby_kernel(cplx<double> a, const cplx<double> *__restrict__ x, cplx<double> b,
+                              cplx<double> *__restrict__ y, int n)
+{
+    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) {
+        cplx<double> v = cmul(b, y[i]);
+        cfma(v, a, x[i]);
+        y[i] = v;
+    }
+}
+
+int krylov_grid() { return KGRID; }
+hipError_t launch_bicg_a(double *cap, double *caw, const double *cp, const double *cw, int n, double *partials,
+                         hipStream_t st)
+{
+    typedef cplx<double> Z;
+    hipLaunchKernelGGL(bicg_a_kernel, dim3(KGRID), dim3(KB), 0, st, (Z *)cap, (Z *)caw, (const Z *)cp, (const Z *)cw, n,
+                       (Z *)partials);
+    return hipGetLastError();
+}
+hipError_t launch_bicg_b(double *anp, double *cr, double *cq, const double *cp, const double *cap, const double *caw,
+                         double cakr, double caki, int n, double *partials, hipStream_t st)
+{
+    typedef cplx<double> Z;
+    hipLaunchKernelGGL(bicg_b_kernel, dim3(KGRID), dim3(KB), 0, st, (Z *)anp, (Z *)cr, (Z *)cq, (const Z *)cp,
+                       (const Z *)cap, (const Z *)caw, cmk(cakr, caki), n, (Z *)partials);
+    return hipGetLastError();
+}
+hipError_t launch_bicg_c(double *cp, double *cw, const double *cr, const double *cq, double cbkr, double cbki, int n,
+                         hipStream_t st)
+{
+    typedef cplx<double> Z;
+    hipLaunchKernelGGL(bicg_c_kernel, dim3(KGRID), dim3(KB), 0, st, (Z *)cp, (Z *)cw, (const Z *)cr, (const Z *)cq,
+                       cmk(cbkr, cbki), n);
+    return hipGetLastError();
+}
+hipError_t launch_bicg_init(double *cr, double *cq, double *cw, double *cp, const double *pnp, const double *anp, int n,
+                            double *partials, hipStream_t st)
+{
+    typedef cplx<double> Z;
+    hipLaunchKernelGGL(bicg_init_kernel, dim3(KGRID), dim3(KB), 0, st, (Z *)cr, (Z *)cq, (Z *)cw, (Z *)cp,
+                       (const Z *)pnp, (const Z *)anp, n, (Z *)partials);
+    return hipGetLastError();
+}
+hipError_t launch_zaxpby(double ar, double ai, const double *x, double br, double bi, double *y, int n, hipStream_t st)
+{
+    typedef cplx<double> Z;
+    hipLaunchKernelGGL(zaxpby_kernel, dim3(KGRID), dim3(KB), 0, st, cmk(ar, ai), (const Z *)x, cmk(br, bi), (Z *)y, n);
+    return hipGetLastError();
+}

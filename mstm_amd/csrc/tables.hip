@@ -1,0 +1,234 @@
+// tables.hip -- FP64 generation of rotate/axial/phase translation tables.
+//
+// One thread builds the table of one pair slot: Wigner-d rows (the
+// recurrence of rotcoef, mstm-modules-33.f90:1098-1175), axial translation
+// coefficients (seed sums + diamond recurrence of axialtrancoefrecurrence,
+// :1566-1676) and exp(i phi) (cartosphere :1986-2004).  Arithmetic is FP64;
+// the store type T is double (plain path) or float (mixed path).
+#include "common.hpp"
+#include "specfun.hpp"
+#include "kernels.hpp"
+
+// Table of one directed translation (target order LT <- source order LS)
+// along rtran = r_target - r_source, written interleaved for slot k.
+template <typename T>
+__device__ void build_slot(T *__restrict__ grp, int k, int LT, int LS, int itype, const double *rt, double rimed,
+                           const double *__restrict__ vcc, int LM)
+{
+    const int VW = 16 / (int)sizeof(T);
+    const int nmax = imx(LT, LS), nmin = imn(LT, LS);
+    double r = sqrt(rt[0] * rt[0] + rt[1] * rt[1] + rt[2] * rt[2]);
+    double ct = rt[2] / r;
+    double rho = sqrt(rt[0] * rt[0] + rt[1] * rt[1]);
+    double cph = 1.0, sph = 0.0;
+    if (!(rt[0] == 0.0 && rt[1] == 0.0)) {
+        cph = rt[0] / rho;
+        sph = rt[1] / rho;
+    }
+    // ---- Wigner d rows, degree by degree --------------------------------
+    double sbe = (fabs(ct) >= 1.0) ? 0.0 : sqrt(fabs((1.0 + ct) * (1.0 - ct)));
+    double cbe2 = 0.5 * (1.0 + ct), sbe2 = 0.5 * (1.0 - ct);
+    double dk0[2 * MSTM_MAX_L + 3], dk01[2 * MSTM_MAX_L + 3];
+    double rowp[2 * MSTM_MAX_L + 1], rowc[2 * MSTM_MAX_L + 1];
+    const int o0 = MSTM_MAX_L + 1;
+    for (int q = 0; q < 2 * MSTM_MAX_L + 3; q++) dk0[q] = dk01[q] = 0.0;
+    dk0[o0] = 1.0;
+    int sgn = 1;
+    double sben = 1.0, bc = 1.0;  // bc = bcof(n,n) = sqrt((2n)!/(n!)^2)
+    for (int n = 1; n <= nmax; n++) {
+        sgn = -sgn;
+        sben = sben * sbe / 2.0;
+        if (sben < 1e-30) sben = 0.0;
+        bc = bc * sqrt((double)(2 * n) * (double)(2 * n - 1)) / (double)n;
+        dk0[o0 + n] = sgn * sben * bc;
+        dk0[o0 - n] = sgn * dk0[o0 + n];
+        dk01[o0 + n] = 0.0;
+        dk01[o0 - n] = 0.0;
+        for (int kk = -n + 1; kk <= n - 1; kk++) {
+            double dkt = dk01[o0 + kk];
+            dk01[o0 + kk] = dk0[o0 + kk];
+            dk0[o0 + kk] = (ct * (double)(2 * n - 1) * dk01[o0 + kk] -
+                            sqrt((double)(n - kk - 1)) * sqrt((double)(n + kk - 1)) * dkt) /
+                           (sqrt((double)(n + kk)) * sqrt((double)(n - kk)));
+        }
+        int kmx = imn(n, nmin);
+        for (int kk = -n; kk <= n; kk++) rowp[kk + n] = dk0[o0 + kk];
+        // write row m = 0
+        auto put_row = [&](int m, const double *row, double sgnm) {
+            int b = (m == 0) ? 0 : (m < 0 ? 2 * (-m) - 1 : 2 * m);
+            int m1 = imx(1, iab(m));
+            int e0 = sec_off(b, LT, LS, VW);
+            for (int nn = m1; nn < n; nn++) e0 += 2 * nn + 1;
+            for (int kk = -n; kk <= n; kk++) {
+                double v = (sgnm == 0.0) ? row[kk + n] : row[-kk + n] * sgnm * ((kk & 1) ? -1.0 : 1.0);
+                grp[ilv(e0 + kk + n, k, VW)] = (T)v;
+            }
+        };
+        put_row(0, rowp, 0.0);
+        int im = 1;
+        for (int m = 1; m <= kmx; m++) {
+            im = -im;
+            double fmn = 1.0 / sqrt((double)(n - m + 1)) / sqrt((double)(n + m));
+            double dkm0 = 0.0, dkm1;
+            for (int kk = -n; kk <= n; kk++) {
+                dkm1 = dkm0;
+                dkm0 = rowp[kk + n];
+                double dkn1 = (kk == n) ? 0.0 : rowp[kk + 1 + n];
+                rowc[kk + n] = (sqrt((double)(n + kk)) * sqrt((double)(n - kk + 1)) * cbe2 * dkm1 -
+                                sqrt((double)(n - kk)) * sqrt((double)(n + kk + 1)) * sbe2 * dkn1 -
+                                (double)kk * sbe * rowp[kk + n]) *
+                               fmn;
+            }
+            put_row(m, rowc, 0.0);
+            // d(-m, -k) = d(m, k) (-1)^(k+m)
+            put_row(-m, rowc, (double)im);
+            for (int kk = -n; kk <= n; kk++) rowp[kk + n] = rowc[kk + n];
+        }
+    }
+    // ---- axial coefficients ---------------------------------------------
+    cplx<double> hw[2 * MSTM_MAX_L + 1];
+    const int nw = LT + LS;
+    double z = r * rimed;
+    if (itype == 3) {
+        sph_hankel_up(nw, z, hw);
+    } else {
+        double jw[2 * MSTM_MAX_L + 1];
+        sph_bessel_j(nw, z, jw);
+        for (int w = 0; w <= nw; w++) hw[w] = cmk(jw[w], 0.0);
+    }
+    cplx<double> act[(MSTM_MAX_L + 1) * (MSTM_MAX_L + 1)][2];
+#define ACT(n, l) act[(n) * (MSTM_MAX_L + 1) + (l)]
+    for (int b = 0; b < 2 * nmin + 1; b++) {
+        int m = mblock_m(b);
+        int m1 = imx(1, iab(m));
+        // seed: row n = m1 over all l, column l = LS over n > m1
+        for (int pass = 0; pass < 2; pass++) {
+            int cnt = pass == 0 ? (LS - m1 + 1) : (LT - m1);
+            for (int q = 0; q < cnt; q++) {
+                int n = pass == 0 ? m1 : m1 + 1 + q;
+                int l = pass == 0 ? m1 + q : LS;
+                int wmin = iab(n - l), wmax = n + l;
+                cplx<double> se = cmk(0.0, 0.0), so = cmk(0.0, 0.0);
+                for (int w = wmin; w <= wmax; w += 2) crfma(se, vcc[vcc_index(n, l, m, w, LM)], hw[w]);
+                for (int w = wmin + 1; w <= wmax - 1; w += 2) crfma(so, vcc[vcc_index(n, l, m, w, LM)], hw[w]);
+                // mode 1: se + i so ; mode 2: se - i so
+                ACT(n, l)[0] = cmk(se.re - so.im, se.im + so.re);
+                ACT(n, l)[1] = cmk(se.re + so.im, se.im - so.re);
+            }
+        }
+        if (m1 < nmin) {
+            for (int n = m1; n <= LT - 1; n++) {
+                double fpn = ax_fnp1(m, n), fnn = ax_fn(m, n), fmn = ax_fnm1(m, n);
+                for (int p = 0; p < 2; p++) {
+                    double sp = (p == 0) ? 1.0 : -1.0;
+                    for (int l = m1; l <= LS - 1; l++) {
+                        cplx<double> v = cscale(ACT(n, l + 1)[p], -ax_fnp1(m, l));
+                        double f = sp * (ax_fn(m, l) - fnn);
+                        cplx<double> a = ACT(n, l)[p];
+                        v = cadd(v, cmk(-f * a.im, f * a.re));  // f * i * a
+                        if (l >= m1 + 1) v = cadd(v, cscale(ACT(n, l - 1)[p], ax_fnm1(m, l)));
+                        if (n > m1) v = cadd(v, cscale(ACT(n - 1, l)[p], fmn));
+                        ACT(n + 1, l)[p] = cscale(v, 1.0 / fpn);
+                    }
+                }
+            }
+        }
+        int e0 = sec_off(b, LT, LS, VW) + pad_to(rot_len(m, LT, LS), VW);
+        int ap = pad_to(ax_len(m, LT, LS), VW);
+        for (int p = 0; p < 2; p++)
+            for (int l = m1; l <= LS; l++)
+                for (int n = m1; n <= LT; n++) {
+                    int e = e0 + p * ap + 2 * ((n - m1) + (LT - m1 + 1) * (l - m1));
+                    grp[ilv(e, k, VW)] = (T)ACT(n, l)[p].re;
+                    grp[ilv(e + 1, k, VW)] = (T)ACT(n, l)[p].im;
+                }
+    }
+#undef ACT
+    int ep = phi_off(LT, LS, VW);
+    grp[ilv(ep, k, VW)] = (T)cph;
+    grp[ilv(ep + 1, k, VW)] = (T)sph;
+}
+
+// Direct-path tables: slot q = (tile*S + step)*S + k (S = MSTM_SLOTS) holds
+// the pair target = blockI*S + k, source = blockJ*S + (k+step)%S (diagonal
+// tiles have blockJ = blockI); self pairs and padding spheres get zeros.
+template <typename T>
+__global__ void direct_tables_kernel(T *__restrict__ tab, const int2 *__restrict__ tiles, int ntiles, int L, int nsph,
+                                     const double *__restrict__ pos, double rimed, const double *__restrict__ vcc,
+                                     int LM, int64_t slot_elems)
+{
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)ntiles * MSTM_SLOTS * MSTM_SLOTS) return;
+    int k = (int)(q % MSTM_SLOTS);
+    int64_t g = q / MSTM_SLOTS;
+    int step = (int)(g % MSTM_SLOTS);
+    int tile = (int)(g / MSTM_SLOTS);
+    int2 bt = tiles[tile];
+    int it = bt.x * MSTM_SLOTS + k;
+    int is = bt.y * MSTM_SLOTS + (k + step) % MSTM_SLOTS;
+    T *grp = tab + g * slot_elems * MSTM_SLOTS;
+    const int VW = 16 / (int)sizeof(T);
+    if (it >= nsph || is >= nsph || it == is) {
+        for (int e = 0; e < slot_elems; e++) grp[ilv(e, k, VW)] = (T)0;
+        return;
+    }
+    double rt[3] = {pos[3 * it] - pos[3 * is], pos[3 * it + 1] - pos[3 * is + 1], pos[3 * it + 2] - pos[3 * is + 2]};
+    build_slot<T>(grp, k, L, L, 3, rt, rimed, vcc, LM);
+}
+
+template <typename T>
+hipError_t launch_direct_tables(T *tab, const int2 *tiles, int ntiles, int L, int nsph, const double *pos, double rimed,
+                                const double *vcc, int LM, hipStream_t st)
+{
+    const int VW = 16 / (int)sizeof(T);
+    int64_t se = slot_len(L, L, VW);
+    int64_t nq = (int64_t)ntiles * MSTM_SLOTS * MSTM_SLOTS;
+    int bs = 128;
+    int64_t nb = (nq + bs - 1) / bs;
+    hipLaunchKernelGGL(direct_tables_kernel<T>, dim3((unsigned)nb), dim3(bs), 0, st, tab, tiles, ntiles, L, nsph, pos,
+                       rimed, vcc, LM, se);
+    return hipGetLastError();
+}
+template hipError_t launch_direct_tables<float>(float *, const int2 *, int, int, int, const double *, double,
+                                                const double *, int, hipStream_t);
+template hipError_t launch_direct_tables<double>(double *, const int2 *, int, int, int, const double *, double,
+                                                 const double *, int, hipStream_t);
+
+// General slot list (FFT path: sphere<->node J tables and near-field H
+// tables).  For slot q: target/source orders from (LT,LS), rtran given.
+template <typename T>
+__global__ void list_tables_kernel(T *__restrict__ tab, const double *__restrict__ rtran, int nslots, int LT, int LS,
+                                   int itype, double rimed, const double *__restrict__ vcc, int LM, int64_t slot_elems)
+{
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nslots) return;
+    int k = q % MSTM_SLOTS;
+    int64_t g = q / MSTM_SLOTS;
+    T *grp = tab + g * slot_elems * MSTM_SLOTS;
+    const int VW = 16 / (int)sizeof(T);
+    const double *rt = rtran + 3 * (int64_t)q;
+    double r2 = rt[0] * rt[0] + rt[1] * rt[1] + rt[2] * rt[2];
+    if (r2 < 1e-12) {  // zero translation handled by the caller (identity / none)
+        for (int e = 0; e < slot_elems; e++) grp[ilv(e, k, VW)] = (T)0;
+        return;
+    }
+    build_slot<T>(grp, k, LT, LS, itype, rt, rimed, vcc, LM);
+}
+
+template <typename T>
+hipError_t launch_list_tables(T *tab, const double *rtran, int nslots, int LT, int LS, int itype, double rimed,
+                              const double *vcc, int LM, hipStream_t st)
+{
+    const int VW = 16 / (int)sizeof(T);
+    int64_t se = slot_len(LT, LS, VW);
+    int bs = 128;
+    int nb = (nslots + bs - 1) / bs;
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(list_tables_kernel<T>, dim3(nb), dim3(bs), 0, st, tab, rtran, nslots, LT, LS, itype, rimed, vcc,
+                       LM, se);
+    return hipGetLastError();
+}
+template hipError_t launch_list_tables<float>(float *, const double *, int, int, int, int, double, const double *, int,
+                                              hipStream_t);
+template hipError_t launch_list_tables<double>(double *, const double *, int, int, int, int, double, const double *,
+                                               int, hipStream_t);

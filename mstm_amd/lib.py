@@ -1,0 +1,71 @@
+"""ctypes binding of libmstm_hip.so (the C-ABI of include/mstm_hip.h).
+
+The library is loaded from the package directory only; if it is missing the
+import of any solver entry point raises: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmstm_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "mstm_hip.h")
+
+FP64, FP32 = 0, 1
+SOLVE_PLAIN, SOLVE_MIXED, SOLVE_LEARNED = 0, 1, 2
+
+
+class MstmHipError(RuntimeError):
+    pass
+
+
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+_SIGNATURES = {
+    "mstm_hip_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    "mstm_hip_destroy": (C.c_int, [_vp]),
+    "mstm_hip_last_error": (C.c_char_p, [_vp]),
+    "mstm_hip_set_spheres": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, C.c_double, C.c_double]),
+    "mstm_hip_set_mie": (C.c_int, [_vp, _ip, _dp]),
+    "mstm_hip_get_orders": (C.c_int, [_vp, _ip, _ip]),
+    "mstm_hip_number_eqns": (C.c_int, [_vp]),
+    "mstm_hip_get_mie": (C.c_int, [_vp, _dp]),
+    "mstm_hip_setup_direct": (C.c_int, [_vp, C.c_int]),
+    "mstm_hip_node_selection": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int, _ip, _dp, _ip]),
+    "mstm_hip_setup_fft": (C.c_int, [_vp, C.c_int]),
+    "mstm_hip_sphereinteraction": (C.c_int, [_vp, C.c_int, _dp, _dp, _ip, _ip, C.c_int, C.c_int]),
+    "mstm_hip_cbicg": (C.c_int, [_vp, C.c_int, C.c_double, _dp, _dp, C.c_int, _ip, _dp]),
+    "mstm_hip_fixedorsoln": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
+                                       _dp, _dp, _dp, _ip, _ip]),
+    "mstm_hip_clear": (C.c_int, [_vp, C.c_int]),
+    "mstm_hip_stats": (C.c_int, [_vp, _dp, C.c_int]),
+    "mstm_hip_stats_names": (C.c_char_p, []),
+}
+
+
+def header_symbols() -> list[str]:
+    """Every function declared in include/mstm_hip.h."""
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*|void)\s*(mstm_hip_\w+)\s*\(", txt, re.M)))
+
+
+def load_library(path: str | None = None):
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise MstmHipError(f"{path} is missing: build it with `python -m mstm_amd.build` (no CPU fallback exists)")
+    L = C.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L

@@ -1,0 +1,112 @@
+"""GPU parity of the direct operator, BiCG and efficiencies (through the
+C-ABI) against the reference's golden outputs and the C oracle.
+
+Tolerances (stated per the precision of the form under test):
+  FP64 plain form: coefficients 1e-10 relative (max-norm), Q 1e-10
+  FP32 forms:      matvec 2e-5 relative, coefficients 1e-5, Q 1e-6
+"""
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+FP32_TOL = 2e-5
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(np.asarray(b)).max())
+
+
+def load_cluster(ctx, g):
+    ctx.set_spheres(g["pos"], g["radius"], g["ri"], 1.0, 1e-6)
+
+
+@pytest.mark.parametrize("case", ["cfg1_direct", "dense_l2", "frac_agg"])
+def test_mie_orders_and_coefficients(gpu_ctx, case):
+    g = golden(case)
+    load_cluster(gpu_ctx, g)
+    order, offset = gpu_ctx.orders()
+    np.testing.assert_array_equal(order, g["order"])
+    np.testing.assert_array_equal(offset[:-1], g["offset"][:-1])
+    assert rel(gpu_ctx.an_mie(), g["anmie"]) < 1e-12
+
+
+@pytest.mark.parametrize("case", ["cfg1_direct", "dense_l2", "frac_agg"])
+def test_direct_matvec_fp64(gpu_ctx, case):
+    g = golden(case)
+    load_cluster(gpu_ctx, g)
+    out = gpu_ctx.sphereinteraction(g["ain"], [False, True], [True, True])
+    for r in range(2):
+        assert rel(out[r], g["aout"][r]) < FP64_TOL, (case, r)
+    h = gpu_ctx.sphereinteraction(g["ain"], [False, True], [False, False])
+    # bare H with con_tran: the reference's hout is H applied inside the
+    # con_tran branch, i.e. S H S x without conjugations
+    assert rel(h[0], g["hout"][0]) < FP64_TOL
+
+
+@pytest.mark.parametrize("case", ["cfg1_direct", "frac_agg"])
+def test_direct_matvec_fp32(gpu_ctx, case):
+    g = golden(case)
+    load_cluster(gpu_ctx, g)
+    from mstm_amd.solver import FP32
+
+    out = gpu_ctx.sphereinteraction(g["ain"], [False, True], [True, True], precision=FP32)
+    for r in range(2):
+        assert rel(out[r], g["aout"][r]) < FP32_TOL, (case, r)
+
+
+@pytest.mark.parametrize("case", ["cfg1_direct", "frac_agg"])
+def test_fixedorsoln_plain_matches_reference(gpu_ctx, case):
+    g = golden(case)
+    load_cluster(gpu_ctx, g)
+    r = gpu_ctx.fixedorsoln(eps=1e-6, niter=10000)
+    # identical BiCG -> identical iteration count and coefficients
+    assert r["iterations"] == g["iterations"]
+    assert rel(r["amnp"], g["amnp"]) < FP64_TOL
+    assert rel(r["qeff"], g["qeff"]) < FP64_TOL
+    from mstm_amd.solver import qtotcalc
+
+    assert rel(qtotcalc(g["radius"], r["qeff"]), g["qtot"]) < 1e-12
+
+
+def test_fixedorsoln_mixed_reaches_reference_residual(gpu_ctx):
+    g = golden("cfg1_direct")
+    load_cluster(gpu_ctx, g)
+    from mstm_amd.solver import SOLVE_MIXED, qtotcalc
+
+    r = gpu_ctx.fixedorsoln(eps=1e-6, niter=10000, solver=SOLVE_MIXED)
+    assert r["istat"] == 0 and r["maxerr"] < 1e-6
+    # both solutions satisfy |r|^2/|b|^2 < 1e-6: compare efficiencies, which
+    # are insensitive to the solution error at first order
+    q = qtotcalc(g["radius"], r["qeff"])
+    assert abs(q[0, 0] - g["qtot"][0, 0]) / g["qtot"][0, 0] < 1e-3
+    # mixed solve driven to a tight residual agrees with the FP64 oracle
+    rt = gpu_ctx.fixedorsoln(eps=1e-14, niter=10000, solver=SOLVE_MIXED)
+    rp = gpu_ctx.fixedorsoln(eps=1e-14, niter=10000)
+    assert rel(rt["amnp"], rp["amnp"]) < 1e-5
+    assert rel(qtotcalc(g["radius"], rt["qeff"]), qtotcalc(g["radius"], rp["qeff"])) < 1e-6
+
+
+def test_cbicg_entry_matches_oracle(gpu_ctx):
+    from oracle import oracle as O
+
+    g = golden("cfg1_direct")
+    load_cluster(gpu_ctx, g)
+    oc = O.OracleCluster(g["pos"], g["radius"], g["ri"])
+    f = oc.planewave()
+    b = oc.mie_mult(f[1])
+    xo, ito, eo = oc.cbicg(b, b, niter=200, eps=1e-8)
+    xg, itg, eg = gpu_ctx.cbicg(200, 1e-8, b, b)
+    assert itg == ito
+    assert rel(xg, xo) < FP64_TOL
+    assert abs(eg - eo) / eo < 1e-6
+
+
+def test_nonconvergence_flag(gpu_ctx):
+    g = golden("cfg1_direct")
+    load_cluster(gpu_ctx, g)
+    r = gpu_ctx.fixedorsoln(eps=1e-6, niter=2)
+    assert r["istat"] == 1 and r["iterations"] == 3
