@@ -87,7 +87,18 @@ def main():
     ap.add_argument("--nsphere", type=int, default=5000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (rocprof FETCH/WRITE calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        x = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        for _ in range(3):
+            y = x.clone()
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": x.numel() * 4, "launches": 3, "checksum": float(y[-1])}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
@@ -183,6 +194,7 @@ def main():
         "speedup_vs_plain_fp64": round(plain["s_per_solve"] / mixed_s, 3) if plain else None,
         "mixed": dict(iterations_last_pol=res["iterations"], maxerr=res["maxerr"], qext=q_mixed[0, 0],
                       qabs=q_mixed[0, 1], iters_fp32=st_mixed["iters_fp32"] / args.steps,
+                      iters_fp64_matvecs=st_mixed["matvecs_fp64"] / args.steps,
                       refinements=st_mixed["refinements"] / args.steps),
         "matvec": matvec,
         "setup_s": dict(geometry=round(t_geom, 2), tables_fp64_fp32=round(t_tables, 3),

@@ -57,32 +57,32 @@ __device__ __forceinline__ void load_sec(const T *__restrict__ p0, T *r)
     }
 }
 
-template <typename T, int L>
-__global__ __launch_bounds__(128) void direct_tile_kernel(const T *__restrict__ tab, const int2 *__restrict__ tiles,
-                                                         const cplx<T> *__restrict__ xin, cplx<T> *__restrict__ part,
-                                                         int npad)
+// occupancy target: FP32 tiles up to order 4 fit 2 waves/SIMD without
+// spilling; larger states use the full register file (AGPRs) instead
+template <typename T, int L> constexpr int tile_waves() { return (sizeof(T) == 4 && L <= 4) ? 2 : 1; }
+// One wave's work on a tile; DIR = 0 forward (I <- J), 1 reverse (J <- I).
+template <typename T, int L, int DIR>
+__device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, int tile, const cplx<T> *__restrict__ xin,
+                                          cplx<T> *__restrict__ part, int npad)
 {
     constexpr int VW = 16 / sizeof(T);
     constexpr int NB = L * (L + 2);
     constexpr int SL = slot_len(L, L, VW);
     constexpr int S = MSTM_SLOTS;
     const int lane = threadIdx.x & 63;
-    const int dir = threadIdx.x >> 6;  // wave-uniform
     const int kslot = lane >> 2, rhs = (lane >> 1) & 1, mode = lane & 1;
-    const int tile = blockIdx.x;
-    const int2 bt = tiles[tile];
     const bool diag = bt.x == bt.y;
-    if (diag && dir) return;
-    const int tgt = (dir ? bt.y : bt.x) * S + kslot;
+    const int tgt = (DIR ? bt.y : bt.x) * S + kslot;
 
     cplx<T> W[NB];
 #pragma unroll
     for (int q = 0; q < NB; q++) W[q] = cmk((T)0, (T)0);
 
+#pragma unroll 1
     for (int s = 0; s < S; s++) {
         if (diag && s == 0) continue;
         int slot, src;
-        if (!dir) {
+        if (!DIR) {
             slot = kslot;
             src = bt.y * S + ((kslot + s) & (S - 1));
         } else {
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(128) void direct_tile_kernel(const T *__restrict__ 
         // exp(i sigma k phi), sigma = +1 forward, -1 reverse (im of :6690)
         T pv[2];
         load_sec<T, phi_off(L, L, VW), 2>(p0, pv);
-        cplx<T> e1 = cmk(pv[0], dir ? -pv[1] : pv[1]);
+        cplx<T> e1 = cmk(pv[0], DIR ? -pv[1] : pv[1]);
         cplx<T> ph[2 * L + 1];
         ph[L] = cmk((T)1, (T)0);
 #pragma unroll
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(128) void direct_tile_kernel(const T *__restrict__ 
 #pragma unroll
             for (int k = -n; k <= n; k++) {
                 cplx<T> v;
-                if (!dir) v = xs[amn(k, n, L)];
+                if (!DIR) v = xs[amn(k, n, L)];
                 else {
                     v = xs[amn(-k, n, L)];
                     if (k & 1) v = cmk(-v.re, -v.im);
@@ -139,26 +139,20 @@ __global__ __launch_bounds__(128) void direct_tile_kernel(const T *__restrict__ 
                 for (int k = -n; k <= n; k++) crfma(acc, dr[ro + k + n], xp[nk(n, k)]);
                 y[n - m1] = acc;
             }
-            // axial translation of this m block (mode-specific matrix)
+            // axial translation of this m block (mode-specific matrix,
+            // transposed for the reverse direction)
             T da[AL];
             load_sec<T, E0 + pad_to(RL, VW), AL>(p0 + (int64_t)mode * (AP / VW) * S * VW, da);
             cplx<T> z[ND];
-            if (!dir) {
 #pragma unroll
-                for (int t = 0; t < ND; t++) {
-                    cplx<T> acc = cmk((T)0, (T)0);
+            for (int t = 0; t < ND; t++) {
+                cplx<T> acc = cmk((T)0, (T)0);
 #pragma unroll
-                    for (int q = 0; q < ND; q++) cfma(acc, cmk(da[2 * (t + ND * q)], da[2 * (t + ND * q) + 1]), y[q]);
-                    z[t] = acc;
+                for (int q = 0; q < ND; q++) {
+                    const int e = DIR ? 2 * (q + ND * t) : 2 * (t + ND * q);
+                    cfma(acc, cmk(da[e], da[e + 1]), y[q]);
                 }
-            } else {
-#pragma unroll
-                for (int t = 0; t < ND; t++) {
-                    cplx<T> acc = cmk((T)0, (T)0);
-#pragma unroll
-                    for (int q = 0; q < ND; q++) cfma(acc, cmk(da[2 * (q + ND * t)], da[2 * (q + ND * t) + 1]), y[q]);
-                    z[t] = acc;
-                }
+                z[t] = acc;
             }
             // back rotation with the output phase folded in:
             // W(t,k) += d^t(m,k) exp(-i sigma k phi) z(t)
@@ -176,20 +170,31 @@ __global__ __launch_bounds__(128) void direct_tile_kernel(const T *__restrict__ 
         });
     }
     // partial for (target, partner block); m-flip back for the reverse wave
-    const int partner = dir ? bt.x : bt.y;
+    const int partner = DIR ? bt.x : bt.y;
     cplx<T> *out = part + (((int64_t)partner * 2 + rhs) * npad + tgt) * 2 * NB + mode * NB;
 #pragma unroll
     for (int n = 1; n <= L; n++)
 #pragma unroll
         for (int k = -n; k <= n; k++) {
             cplx<T> v;
-            if (!dir) v = W[nk(n, k)];
+            if (!DIR) v = W[nk(n, k)];
             else {
                 v = W[nk(n, -k)];
                 if (k & 1) v = cmk(-v.re, -v.im);
             }
             out[amn(k, n, L)] = v;
         }
+}
+
+template <typename T, int L>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(tile_waves<T, L>()))) void direct_tile_kernel(
+    const T *__restrict__ tab, const int2 *__restrict__ tiles, const cplx<T> *__restrict__ xin,
+    cplx<T> *__restrict__ part, int npad)
+{
+    const int tile = blockIdx.x;
+    const int2 bt = tiles[tile];
+    if (threadIdx.x < 64) tile_body<T, L, 0>(tab, bt, tile, xin, part, npad);
+    else if (bt.x != bt.y) tile_body<T, L, 1>(tab, bt, tile, xin, part, npad);
 }
 
 template <typename T>
@@ -202,7 +207,11 @@ hipError_t launch_direct_tiles(const T *tab, const int2 *tiles, int ntiles, int 
     switch (L) {
 #define CASE(LL)                                                                                                       \
     case LL: hipLaunchKernelGGL((direct_tile_kernel<T, LL>), g, b, 0, st, tab, tiles, x, p, npad); break;
+#ifdef MSTM_TILE_L_ONLY
+        CASE(MSTM_TILE_L_ONLY)
+#else
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#endif
 #undef CASE
     default: return hipErrorInvalidValue;
     }
