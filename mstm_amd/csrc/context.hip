@@ -404,7 +404,7 @@ int mstm_hip_setup_fft(mstm_hip_ctx *c, int prec)
     FftGeometry g{c->nsph,          c->Lmax,    c->neqns,   c->pos.data(), c->radius.data(), c->order.data(),
                   c->offset.data(), c->rimed,   c->d_pos,   c->d_order,    c->d_offset,      c->d_an,
                   c->d_vcc,         c->vcc_LM};
-    (void)g;
+    fft_bind(c->fft, g);
     rc = fft_setup(c->fft, prec, c->st);
     if (rc) return fail(c, rc, "setup_fft: %s", fft_last_error(c->fft));
     return 0;

@@ -50,7 +50,13 @@ __device__ __forceinline__ void load_sec(const T *__restrict__ p0, T *r)
 #pragma unroll
     for (int c = 0; c < (LEN + VW - 1) / VW; c++) {
         const V *p = reinterpret_cast<const V *>(p0 + (int64_t)(E0 / VW + c) * MSTM_SLOTS * VW);
+#ifdef MSTM_TABLE_NT
         V v = __builtin_nontemporal_load(p);
+#else
+        // both waves of a tile read the same table group within a few
+        // microseconds: plain loads let the second one hit in L2
+        V v = *p;
+#endif
 #pragma unroll
         for (int q = 0; q < VW; q++)
             if (c * VW + q < LEN) r[c * VW + q] = v[q];

@@ -1,4 +1,10 @@
 // fft.hpp -- FFT-accelerated sphere interaction (node grid) interface.
+//
+// MI355X form of fft_translation (fft_translation-5.f90:1-1076): spheres
+// are J-translated to the centre of their node cell, node-to-node H is a
+// 3-D convolution evaluated with zero-padded FFTs against a precomputed
+// node-kernel spectrum, node fields are J-translated back, and sphere pairs
+// in neighbouring cells are translated directly.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -18,6 +24,8 @@ struct FftGeometry {
 };
 FftOp *fft_create(const FftGeometry &g, double fva, int node_order, int neighbor_model, int *cell_dim, double *d_cell,
                   int *node_order_out, int *err);
+// (re)binds the context's device pointers (vcc may have been regrown)
+void fft_bind(FftOp *f, const FftGeometry &g);
 int fft_setup(FftOp *f, int prec, hipStream_t st);
 int fft_matvec(FftOp *f, int nrhs, const double *ain, double *aout, const int *flags, int prec, hipStream_t st);
 void fft_clear(FftOp *f);

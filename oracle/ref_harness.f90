@@ -30,11 +30,11 @@
       implicit none
       character(len=16) :: mode
       character(len=256) :: posfile,infile,outprefix,jobfile
-      integer :: nsphere,niter,node_order_in,nbr_model,i,n,ios,istat,maxiter,ncol,reps,r
+      integer :: nsphere,niter,node_order_in,nbr_model,i,n,ios,istat,maxiter,ncol,reps,r,p
       real(8) :: mie_eps,eps,cell_vf,alpha,sinc,vr,maxerr,t0,t1,rmed
       real(8) :: line(6),tmin(3),tmax(3),xg
       logical :: use_fft,store_tables
-      complex(8), allocatable :: ain(:,:),aout(:,:),gout(:,:),amnp(:,:)
+      complex(8), allocatable :: ain(:,:),aout(:,:),gout(:,:),amnp(:,:),pmnp0(:,:),pmnpan(:)
       real(8), allocatable :: qeff(:,:,:),qvabs(:,:)
       real(8) :: qtot(3,3)
       namelist /job/ mode,posfile,nsphere,mie_eps,use_fft,cell_vf,node_order_in,nbr_model, &
@@ -141,8 +141,32 @@
          enddo
          xg=vr**.333333
          t0=mstm_mpi_wtime()
-         call fixedorsoln(alpha,sinc,1,eps,niter,amnp,qeff,3,maxerr,maxiter,0,istat, &
-            solution_method='i',initialize_solver=.true.)
+         if(use_fft) then
+!
+! fft_external_to_external_expansion sizes its saved node arrays by the nrhs
+! of its initial_run call (fft_translation-5.f90:190-194).  cbicg makes that
+! call with nrhs=1 and then applies 2-RHS products, writing past the arrays.
+! Here the initial call is made with nrhs=2, then the serial iterative branch
+! of fixedorsoln (mstm-solver-8.f90:468-525, 553-564) is replayed with
+! cbicg(initialize_solver=.false.), so every later call fits the allocation.
+!
+            allocate(ain(number_eqns,2),aout(number_eqns,2),pmnp0(number_eqns,2),pmnpan(number_eqns))
+            ain=(0.d0,0.d0)
+            call sphereinteraction(number_eqns,2,ain,aout,initial_run=.true., &
+               con_tran=(/.false.,.true./))
+            call sphereplanewavecoef(alpha,sinc,1,pmnp0)
+            do p=1,2
+               istat=0
+               call multmiecoeffmult(number_eqns,1,1,pmnp0(:,p),pmnpan)
+               amnp(:,p)=pmnpan
+               call cbicg(niter,eps,pmnpan,amnp(:,p),0,maxiter,maxerr,initialize_solver=.false.)
+               if(maxiter.gt.niter.or.maxerr.gt.eps) istat=1
+            enddo
+            call qefficiencyfactors(nsphere,2,amnp,pmnp0,qeff)
+         else
+            call fixedorsoln(alpha,sinc,1,eps,niter,amnp,qeff,3,maxerr,maxiter,0,istat, &
+               solution_method='i',initialize_solver=.true.)
+         endif
          t1=mstm_mpi_wtime()
          call qtotcalc(nsphere,3,xg,qeff,qvabs,qtot)
          write(20,'(a,i10)') 'iterations ',maxiter

@@ -13,7 +13,9 @@ Cases
   fft_cyl250    the first 250 spheres of the reference fixture
                 examples/random_in_cylinder_1000.pos (unit spheres,
                 m = 1.6+0.0123i as in the manual's example 6), FFT operator
-                with automatic cell size and node order; plus the direct H
+                with automatic cell size and node order; plus the direct H.
+                The FFT solve replays fixedorsoln with a 2-RHS initial call
+                (see oracle/ref_harness.f90 for the reference allocation bug)
   frac_agg      examples/frac_agg.pos (100 touching monomers) scaled to
                 x = 0.5, m = 1.75+0.4i: matvec + solve
 """
