@@ -1,0 +1,167 @@
+!
+! mstm_hip_solver.f90 -- drop-in GPU forms of MSTM's solver entry points.
+!
+! Compiled against MSTM's own modules (spheredata, mie, fft_translation,
+! numconstants, surface_subroutines), whose state it reads exactly as the
+! Fortran code sets it.  Each procedure has the argument list of the
+! reference procedure it replaces:
+!   hip_cbicg               <- cbicg               (mstm-solver-8.f90:654-655)
+!   hip_fixedorsoln         <- fixedorsoln         (mstm-solver-8.f90:350-352)
+!   hip_sphereinteraction   <- sphereinteraction   (mstm-scatprops-26.f90:25-27)
+! The GPU context is created on first use and refilled when
+! initialize_solver / initial_run is true (geometry, orders and an_mie are
+! taken from the MSTM modules, so the Mie data are the host's own).
+! fft_translation_option selects the node-grid operator; its grid is the
+! one node_selection left in the fft_translation module variables.
+!
+      module mstm_hip_solver
+      use iso_c_binding
+      use mstm_hip_c
+      use numconstants
+      use spheredata
+      use mie
+      use fft_translation
+      use surface_subroutines
+      implicit none
+      type(c_ptr), save :: hip_ctx=c_null_ptr
+      logical, save :: hip_loaded=.false.
+      integer, save :: hip_precision=0
+
+      contains
+
+         subroutine hip_check(rc,where)
+         integer(c_int) :: rc
+         character(*) :: where
+         if(rc.ne.0) then
+            write(run_print_unit,'('' mstm_hip: '',a,'' failed: '',a)') where, &
+               trim(mstm_hip_message(hip_ctx))
+            stop
+         endif
+         end subroutine hip_check
+!
+! push the MSTM sphere state into the GPU context
+!
+         subroutine hip_load_state()
+         integer :: i,cd(3),no
+         integer(c_int) :: rc
+         real(c_double), allocatable :: ri(:)
+         real(c_double) :: dc
+         if(.not.c_associated(hip_ctx)) then
+            call hip_check(mstm_hip_create(hip_ctx,0_c_int),'create')
+         endif
+         if(number_host_spheres.gt.0.or.number_plane_boundaries.gt.0) then
+            write(run_print_unit,'('' mstm_hip: host spheres / plane boundaries are not on the GPU path'')')
+            stop
+         endif
+         allocate(ri(2*number_spheres))
+         do i=1,number_spheres
+            ri(2*i-1)=dble(sphere_ref_index(1,i))
+            ri(2*i)=dimag(sphere_ref_index(1,i))
+         enddo
+         rc=mstm_hip_set_spheres(hip_ctx,number_spheres,sphere_position,sphere_radius,ri, &
+            dble(layer_ref_index(0)),1.d-6)
+         call hip_check(rc,'set_spheres')
+         call hip_check(mstm_hip_set_mie(hip_ctx,sphere_order,an_mie),'set_mie')
+         if(fft_translation_option) then
+            call hip_check(mstm_hip_node_selection(hip_ctx,cell_volume_fraction,node_order, &
+               neighbor_node_model,cd,dc,no),'node_selection')
+            call hip_check(mstm_hip_setup_fft(hip_ctx,int(hip_precision,c_int)),'setup_fft')
+         endif
+         deallocate(ri)
+         hip_loaded=.true.
+         end subroutine hip_load_state
+!
+! cbicg (mstm-solver-8.f90:654-949) on the GPU, FP64, iteration-identical
+!
+         subroutine hip_cbicg(niter,eps,pnp,anp,iterwrite,iter,errmax,initialize_solver,mpi_comm)
+         integer :: niter,iterwrite,iter
+         integer, optional :: mpi_comm
+         logical, optional :: initialize_solver
+         real(8) :: eps,errmax
+         complex(8) :: pnp(number_eqns),anp(number_eqns)
+         logical :: init
+         integer(c_int) :: it,rc,ufft
+         init=.true.
+         if(present(initialize_solver)) init=initialize_solver
+         if(init.or.(.not.hip_loaded)) call hip_load_state()
+         ufft=0
+         if(fft_translation_option) ufft=1
+         rc=mstm_hip_cbicg(hip_ctx,int(niter,c_int),eps,pnp,anp,ufft,it,errmax)
+         call hip_check(rc,'cbicg')
+         iter=it
+         end subroutine hip_cbicg
+!
+! fixedorsoln (mstm-solver-8.f90:350-569), iterative branch; solution_method
+! 'm' selects the mixed FP32/FP64 solver, anything else the FP64 one
+!
+         subroutine hip_fixedorsoln(alpha,sinc,dir,eps,niter,amnp,qeff,qeffdim,maxerr,maxiter, &
+            iterwrite,istat,mpi_comm,excited_spheres,solution_method,initialize_solver)
+         integer :: niter,istat,maxiter,iterwrite,dir,qeffdim
+         integer, optional :: mpi_comm
+         logical, optional :: excited_spheres(number_spheres),initialize_solver
+         character*1, optional :: solution_method
+         real(8) :: alpha,sinc,eps,qeff(3,qeffdim,number_spheres),maxerr
+         complex(8) :: amnp(number_eqns,2)
+         real(8), allocatable :: q(:,:,:)
+         logical :: init
+         integer(c_int) :: solver,mi,ist,ufft
+         init=.true.
+         if(present(initialize_solver)) init=initialize_solver
+         if(init.or.(.not.hip_loaded)) call hip_load_state()
+         solver=mstm_hip_solve_plain
+         if(present(solution_method)) then
+            if(solution_method.eq.'m') solver=mstm_hip_solve_mixed
+         endif
+         ufft=0
+         if(fft_translation_option) ufft=1
+         allocate(q(3,3,number_spheres))
+         call hip_check(mstm_hip_fixedorsoln(hip_ctx,alpha,sinc,int(dir,c_int),eps,int(niter,c_int), &
+            solver,ufft,amnp,q,maxerr,mi,ist),'fixedorsoln')
+         maxiter=mi
+         istat=ist
+         if(qeffdim.eq.1) then
+            qeff(:,1,:)=q(:,1,:)
+         else
+            qeff(:,1:3,:)=q(:,1:3,:)
+         endif
+         deallocate(q)
+         end subroutine hip_fixedorsoln
+!
+! sphereinteraction (mstm-scatprops-26.f90:25-180) for external spheres
+!
+         subroutine hip_sphereinteraction(neqns,nrhs,ain,aout,initial_run,rhs_list,mpi_comm, &
+            con_tran,mie_mult,fft_option,store_matrix_option,skip_external_translation)
+         integer :: neqns,nrhs
+         integer, optional :: mpi_comm
+         logical, optional :: initial_run,rhs_list(nrhs),con_tran(nrhs),mie_mult(nrhs), &
+            fft_option,store_matrix_option,skip_external_translation
+         complex(8) :: ain(neqns,nrhs),aout(neqns,nrhs)
+         integer(c_int) :: ct(nrhs),mm(nrhs),ufft
+         integer :: r
+         if(present(initial_run)) then
+            if(initial_run) call hip_load_state()
+         endif
+         if(.not.hip_loaded) call hip_load_state()
+         ct=0
+         mm=1
+         if(present(con_tran)) then
+            do r=1,nrhs
+               if(con_tran(r)) ct(r)=1
+            enddo
+         endif
+         if(present(mie_mult)) then
+            do r=1,nrhs
+               if(.not.mie_mult(r)) mm(r)=0
+            enddo
+         endif
+         ufft=0
+         if(fft_translation_option) ufft=1
+         if(present(fft_option)) then
+            ufft=0
+            if(fft_option) ufft=1
+         endif
+         call hip_check(mstm_hip_sphereinteraction(hip_ctx,int(nrhs,c_int),ain,aout,ct,mm,ufft, &
+            int(hip_precision,c_int)),'sphereinteraction')
+         end subroutine hip_sphereinteraction
+
+      end module mstm_hip_solver

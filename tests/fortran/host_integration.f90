@@ -1,0 +1,87 @@
+!
+! host_integration.f90 -- the unchanged MSTM Fortran modules driving the GPU
+! through the ISO_C_BINDING shim (mstm_amd/fortran).  The cluster state is
+! built by the reference's own setup calls (as main_calling_program does);
+! then the reference CPU procedures and their hip_* replacements run on the
+! same state and the results are compared.  Prints one line per check and
+! exits with status 1 on a mismatch.  Usage: host_integration <posfile> <n>
+!
+      program host_integration
+      use mpidefs
+      use numconstants
+      use specialfuncs
+      use surface_subroutines
+      use spheredata
+      use mie
+      use translation
+      use fft_translation
+      use scatprops
+      use solver
+      use mstm_hip_solver
+      implicit none
+      character(len=256) :: posfile,arg
+      integer :: n,i,iref,igpu,istat_r,istat_g,mi_r,mi_g,bad
+      real(8) :: line(6),err_r,err_g,d1,d2,d3
+      complex(8), allocatable :: amnp_r(:,:),amnp_g(:,:),ain(:,:),ao_r(:,:),ao_g(:,:),b(:),x_r(:),x_g(:)
+      real(8), allocatable :: q_r(:,:,:),q_g(:,:,:)
+      call get_command_argument(1,posfile)
+      call get_command_argument(2,arg)
+      read(arg,*) n
+      call mstm_mpi(mpi_command='init')
+      number_spheres=n
+      allocate(sphere_radius(n),sphere_position(3,n),sphere_ref_index(2,0:n),host_sphere(n), &
+         number_field_expansions(n))
+      open(11,file=trim(posfile),status='old')
+      do i=1,n
+         read(11,*) line(1:6)
+         sphere_position(:,i)=line(1:3)
+         sphere_radius(i)=line(4)
+         sphere_ref_index(1,i)=dcmplx(line(5),line(6))
+         sphere_ref_index(2,i)=sphere_ref_index(1,i)
+      enddo
+      close(11)
+      sphere_ref_index(:,0)=layer_ref_index(0)
+      call plane_boundary_initialization()
+      call findhostspheres()
+      call sphere_layer_initialization()
+      call miecoefcalc(1.d-6)
+      call init(max_mie_order)
+      call incident_field_initialization(0.d0,0.d0,1)
+      bad=0
+      ! fixedorsoln: reference CPU vs GPU
+      allocate(amnp_r(number_eqns,2),amnp_g(number_eqns,2),q_r(3,3,n),q_g(3,3,n))
+      call fixedorsoln(0.d0,0.d0,1,1.d-6,10000,amnp_r,q_r,3,err_r,mi_r,0,istat_r, &
+         solution_method='i',initialize_solver=.true.)
+      call hip_fixedorsoln(0.d0,0.d0,1,1.d-6,10000,amnp_g,q_g,3,err_g,mi_g,0,istat_g, &
+         solution_method='i',initialize_solver=.true.)
+      d1=maxval(abs(amnp_r-amnp_g))/maxval(abs(amnp_r))
+      d2=maxval(abs(q_r-q_g))/maxval(abs(q_r))
+      write(*,'(a,2i6,2es12.4)') 'fixedorsoln iterations(ref,gpu) rel_err(amnp,qeff): ',mi_r,mi_g,d1,d2
+      if(mi_r.ne.mi_g.or.d1.gt.1.d-10.or.d2.gt.1.d-10) bad=1
+      ! sphereinteraction on two columns (A x, A^H y)
+      allocate(ain(number_eqns,2),ao_r(number_eqns,2),ao_g(number_eqns,2))
+      do i=1,number_eqns
+         ain(i,1)=dcmplx(sin(0.37d0*i),cos(0.11d0*i))
+         ain(i,2)=dcmplx(cos(0.23d0*i),sin(0.05d0*i))
+      enddo
+      call sphereinteraction(number_eqns,2,ain,ao_r,initial_run=.true.,con_tran=(/.false.,.true./))
+      call hip_sphereinteraction(number_eqns,2,ain,ao_g,initial_run=.true.,con_tran=(/.false.,.true./))
+      d3=maxval(abs(ao_r-ao_g))/maxval(abs(ao_r))
+      write(*,'(a,es12.4)') 'sphereinteraction rel_err: ',d3
+      if(d3.gt.1.d-10) bad=1
+      ! cbicg from the same start
+      allocate(b(number_eqns),x_r(number_eqns),x_g(number_eqns))
+      call multmiecoeffmult(number_eqns,1,1,ain(:,1),b)
+      x_r=b
+      x_g=b
+      call cbicg(200,1.d-8,b,x_r,0,iref,err_r,initialize_solver=.true.)
+      call hip_cbicg(200,1.d-8,b,x_g,0,igpu,err_g,initialize_solver=.true.)
+      d1=maxval(abs(x_r-x_g))/maxval(abs(x_r))
+      write(*,'(a,2i6,es12.4)') 'cbicg iterations(ref,gpu) rel_err: ',iref,igpu,d1
+      if(iref.ne.igpu.or.d1.gt.1.d-10) bad=1
+      if(bad.ne.0) then
+         write(*,'(a)') 'HOST INTEGRATION FAILED'
+         call exit(1)
+      endif
+      write(*,'(a)') 'HOST INTEGRATION OK'
+      end program host_integration
