@@ -6,9 +6,10 @@
  * at sphere_offset(i) (mstm-modules-33.f90:2351-2377, 5354-5376).
  *
  * Each entry point names the reference procedure whose contract it takes
- * over (paths relative to MSTM's december2023/ tree).  The Fortran shim
- * module (mstm_amd/fortran/mstm_hip_mod.f90) binds all of them with
- * ISO_C_BINDING; INTEGRATION.md shows the reference-side call sites.
+ * over (paths relative to MSTM's december2023/ tree).  The Fortran module
+ * mstm_amd/fortran/mstm_hip_c.f90 binds them with ISO_C_BINDING, and
+ * mstm_amd/fortran/mstm_hip_solver.f90 wraps them in procedures with the
+ * argument lists of cbicg, fixedorsoln and sphereinteraction.
  *
  * Return codes: 0 = success, otherwise an MSTM_HIP_E* code and
  * mstm_hip_last_error() holds the message.  Calls on one context must come
@@ -33,6 +34,9 @@ extern "C" {
 #define MSTM_HIP_SOLVE_PLAIN 0   /* cbicg in FP64, iteration-identical         */
 #define MSTM_HIP_SOLVE_MIXED 1   /* FP32 BiCG inside FP64 iterative refinement */
 #define MSTM_HIP_SOLVE_LEARNED 2 /* MIXED started from the learned guess       */
+/* FP64 flexible GMRES(8) with FP32 BiCG inner solves; a loaded learned
+ * network is the inner preconditioner of the first Arnoldi vector */
+#define MSTM_HIP_SOLVE_FGMRES 3
 
 typedef struct mstm_hip_ctx mstm_hip_ctx;
 
@@ -107,6 +111,91 @@ int mstm_hip_clear(mstm_hip_ctx *ctx, int clear_tables);
  * of slots written. */
 int mstm_hip_stats(const mstm_hip_ctx *ctx, double *out, int nout);
 const char *mstm_hip_stats_names(void);
+
+/* ---------------------------------------------------------------------------
+ * Learned initial guess.  No counterpart in MSTM: the reference always starts
+ * BiCG from x0 = b (mstm-solver-8.f90:483).  A per-sphere FP32 MLP over the
+ * neighbour geometry (LIG_NFEAT = 64 features, `nlayers` SiLU layers of
+ * width `hidden`, linear output) predicts the solution block of every sphere
+ * for incidence along +z:
+ *     x0_i = exp(i kappa d_i) (b_i + exp(i k z_i) NN_i)
+ * with d_i the depth below the cluster surface and kappa a learned complex
+ * wavenumber shift.  Parameters: one flat float array, per layer W [out x in]
+ * row-major then bias [out], then kappa (re, im); Lnet is the sphere order
+ * the output layer covers (8 Lnet (Lnet+2) outputs, both polarizations).
+ * MSTM_HIP_SOLVE_LEARNED evaluates the prediction and b in FP64, starts the
+ * mixed solver from the better one, and hands over to the plain FP64 solver
+ * from b if the mixed solve does not reach eps; other incidence directions
+ * start from b.
+ * ------------------------------------------------------------------------- */
+int mstm_hip_lig_nparams(int hidden, int nlayers, int Lnet);
+/* load (params != NULL) or unload the network of the learned solver */
+int mstm_hip_set_learned(mstm_hip_ctx *ctx, const float *params, int nparams, int hidden, int nlayers, int Lnet);
+/* per-sphere features feat[nsphere][64] and depths depth[nsphere] */
+int mstm_hip_lig_features(mstm_hip_ctx *ctx, float *feat, float *depth);
+/* training pairs from a solution amnp(number_eqns,2) for incidence along +z:
+ * target = a_i exp(-i k z_i), base = b_i exp(-i k z_i) in the order-Lnet
+ * layout, rows of 8 Lnet (Lnet+2) floats per sphere */
+int mstm_hip_lig_samples(mstm_hip_ctx *ctx, const double *amnp, int Lnet, float *target, float *base);
+/* the learned starting vectors x0(number_eqns,2) for incidence along +z */
+int mstm_hip_lig_guess(mstm_hip_ctx *ctx, double *x0);
+
+/* Trainer: the network's forward, backward and Adam steps on the GPU. */
+typedef struct mstm_hip_trainer mstm_hip_trainer;
+int mstm_hip_trainer_create(mstm_hip_trainer **t, int device, int hidden, int nlayers, int Lnet, unsigned seed);
+int mstm_hip_trainer_destroy(mstm_hip_trainer *t);
+const char *mstm_hip_trainer_last_error(const mstm_hip_trainer *t);
+/* get (set = 0) or set (set = 1, also resets the Adam moments) the parameters */
+int mstm_hip_trainer_params(mstm_hip_trainer *t, float *params, int nparams, int set);
+/* training set: feat[n][64], target/base[n][8 Lnet (Lnet+2)], depth[n] */
+int mstm_hip_trainer_set_data(mstm_hip_trainer *t, int nsamp, const float *feat, const float *target,
+                              const float *base, const float *depth);
+/* one shuffled pass with Adam; mean_loss = mean |yhat - y|^2 / mean |y|^2 */
+int mstm_hip_trainer_epoch(mstm_hip_trainer *t, int batch, float lr, unsigned seed, double *mean_loss);
+/* loss over the training set without an update */
+int mstm_hip_trainer_loss(mstm_hip_trainer *t, int batch, double *loss);
+/* gradient of the mean loss over the given samples (kappa last) */
+int mstm_hip_trainer_grad(mstm_hip_trainer *t, int nsamp, const float *feat, const float *target, const float *base,
+                          const float *depth, float *grad, double *loss);
+/* raw network outputs out[n][8 Lnet (Lnet+2)] */
+int mstm_hip_trainer_forward(mstm_hip_trainer *t, int nsamp, const float *feat, float *out);
+
+/* ---------------------------------------------------------------------------
+ * Batched solve: many independent clusters, each with the same ndir
+ * incidence directions x 2 polarizations (R = 2 ndir <= 64 right-hand sides
+ * per cluster).  MSTM does one fixedorsoln (mstm-solver-8.f90:350-569) per
+ * cluster and direction; here each cluster's H is built once as dense FP64
+ * blocks (stored FP64 and FP32) and every BiCG product of all clusters and
+ * right-hand sides is one grouped complex GEMM on MFMA (FP32), inside FP64
+ * iterative refinement.  Spheres of one cluster must share one order.
+ * Host vectors (batch_vectors, batch_product): per cluster an (R, neqns_c)
+ * complex array in MSTM layout, clusters concatenated; column 2d+q is
+ * direction d, polarization q.
+ * ------------------------------------------------------------------------- */
+typedef struct mstm_hip_batch mstm_hip_batch;
+int mstm_hip_batch_create(mstm_hip_batch **b, int device);
+int mstm_hip_batch_destroy(mstm_hip_batch *b);
+const char *mstm_hip_batch_last_error(const mstm_hip_batch *b);
+/* counts[ncluster]; spheres concatenated: pos[3n], radius[n], ri[2n];
+ * directions alpha[ndir], beta[ndir] (radians, beta in [0, pi]) */
+int mstm_hip_batch_set(mstm_hip_batch *b, int ncluster, const int *counts, const double *pos, const double *radius,
+                       const double *ri, double ri_medium, double mie_eps, int ndir, const double *alpha,
+                       const double *beta);
+/* sphere order of each cluster (Lorenz-Mie criterion of miecoefcalc) */
+int mstm_hip_batch_orders(const mstm_hip_batch *b, int *order);
+/* dense operators, right-hand sides b = T f, and (tune != 0) the GEMM autotune */
+int mstm_hip_batch_setup(mstm_hip_batch *b, int tune);
+/* solve all to |r|^2/|b|^2 < eps; iters/err per (cluster, column) */
+int mstm_hip_batch_solve(mstm_hip_batch *b, double eps, int niter, int use_graph, int *iters, double *err);
+/* which: 0 solution, 1 right-hand side T f, 2 incident field f */
+int mstm_hip_batch_vectors(mstm_hip_batch *b, int which, double *out);
+/* y = H x (op 0) or H^H x (op 1); precision MSTM_HIP_FP32 (MFMA) or MSTM_HIP_FP64 (op 0) */
+int mstm_hip_batch_product(mstm_hip_batch *b, int precision, int op, const double *x, double *y);
+/* qtot per (cluster, direction): q[(c ndir + d) 9 + 3k + j], k unpol/pol1/pol2, j ext/abs/sca */
+int mstm_hip_batch_qeff(mstm_hip_batch *b, double *q);
+int mstm_hip_batch_stats(const mstm_hip_batch *b, double *out, int nout);
+const char *mstm_hip_batch_stats_names(void);
+int mstm_hip_batch_tune_table(const mstm_hip_batch *b, double *out, int nout);
 
 #ifdef __cplusplus
 }

@@ -3,6 +3,7 @@
 // BiCG driver of cbicg (mstm-solver-8.f90:654-949), the mixed-precision
 // refinement around it, and fixedorsoln (mstm-solver-8.f90:350-569).
 #include <chrono>
+#include <complex>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -15,6 +16,7 @@
 #include "kernels.hpp"
 #include "specfun.hpp"
 #include "fft.hpp"
+#include "learned.hpp"
 
 namespace {
 
@@ -37,6 +39,20 @@ struct Stats {
     double table_bytes[2] = {0, 0};
     double fft_setup_ms = 0, fft_conv_ms = 0;
     long fft_launches = 0;
+    long lig_solves = 0, lig_fallbacks = 0, fgmres_steps = 0;
+    double lig_ms = 0, lig_err0 = 0, lig_errb = 0;
+};
+
+// learned initial guess: network parameters and per-sphere work buffers
+struct LearnedNet {
+    bool ready = false;
+    MlpShape s{};
+    int Lnet = 0;
+    float *params = nullptr;
+    size_t np = 0;
+    int cap = 0;
+    float *feat = nullptr, *depth = nullptr, *out = nullptr;
+    std::vector<float *> H;
 };
 
 }  // namespace
@@ -58,6 +74,9 @@ struct mstm_hip_ctx {
     int vcc_LM = 0;
     DirectOp dop[2];
     FftOp *fft = nullptr;
+    LearnedNet lig;
+    double *d_fg = nullptr;  // FGMRES basis V (m+1) and Z (m), complex FP64
+    size_t fg_bytes = 0;
     // Krylov work vectors (complex FP64, neqns each)
     double *d_kv = nullptr;
     size_t kv_n = 0;
@@ -127,12 +146,10 @@ static long double clebsch(int j1, int m1, int j2, int m2, int J)
                                           lfac(J - j2 + m1 + k) * lfac(J - j1 - m2 + k));
     return sqrtl(pre) * s;
 }
-static int ensure_vcc(mstm_hip_ctx *c, int LM)
+void vcc_table(int LM, std::vector<double> &h)
 {
-    if (c->d_vcc && c->vcc_LM >= LM) return 0;
-    dfree(c->d_vcc);
     size_t n = (size_t)(LM + 1) * (LM + 1) * (2 * LM + 1) * (2 * LM + 1);
-    std::vector<double> h(n, 0.0);
+    h.assign(n, 0.0);
     for (int nn = 1; nn <= LM; nn++)
         for (int l = 1; l <= LM; l++) {
             long double c1 = sqrtl((long double)(2 * nn + 1)) * sqrtl((long double)(2 * l + 1));
@@ -145,6 +162,15 @@ static int ensure_vcc(mstm_hip_ctx *c, int LM)
                         (double)(c2 * clebsch(nn, -m, l, m, w) * clebsch(nn, -1, l, 1, w) * sg);
                 }
         }
+}
+
+static int ensure_vcc(mstm_hip_ctx *c, int LM)
+{
+    if (c->d_vcc && c->vcc_LM >= LM) return 0;
+    dfree(c->d_vcc);
+    std::vector<double> h;
+    vcc_table(LM, h);
+    const size_t n = h.size();
     HIPCHK(dalloc(&c->d_vcc, n * sizeof(double)));
     HIPCHK(hipMemcpy(c->d_vcc, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
     c->vcc_LM = LM;
@@ -159,8 +185,19 @@ static void release_direct(mstm_hip_ctx *c, int p)
     c->dop[p].ready = false;
 }
 
+static void release_lig_buffers(mstm_hip_ctx *c)
+{
+    dfree(c->lig.feat);
+    dfree(c->lig.depth);
+    dfree(c->lig.out);
+    for (auto &q : c->lig.H) dfree(q);
+    c->lig.H.clear();
+    c->lig.cap = 0;
+}
+
 static void release_spheres(mstm_hip_ctx *c)
 {
+    release_lig_buffers(c);
     release_direct(c, 0);
     release_direct(c, 1);
     if (c->fft) {
@@ -177,6 +214,8 @@ static void release_spheres(mstm_hip_ctx *c)
     dfree(c->d_an);
     dfree(c->d_tiles);
     dfree(c->d_kv);
+    dfree(c->d_fg);
+    c->fg_bytes = 0;
     c->kv_n = 0;
     c->nsph = 0;
 }
@@ -206,6 +245,7 @@ int mstm_hip_destroy(mstm_hip_ctx *c)
     hipSetDevice(c->device);
     hipStreamSynchronize(c->st);
     release_spheres(c);
+    dfree(c->lig.params);
     dfree(c->d_vcc);
     dfree(c->d_partials);
     for (auto &e : c->ev) hipEventDestroy(e);
@@ -574,8 +614,9 @@ static int bicg_dev(mstm_hip_ctx *c, int niter, double eps, const double *pnp, d
 // Mixed precision: x <- x + d with A d = r solved by FP32 BiCG, r = b - A x
 // evaluated with the FP64 operator; stops on the reference criterion
 // |r|^2/|b|^2 < eps measured in FP64.
+// r_in (optional) is b - A x at the starting x, already evaluated in FP64.
 static int refine_dev(mstm_hip_ctx *c, int niter, double eps, const double *pnp, double *anp, int use_fft, int *iters,
-                      double *errmax)
+                      double *errmax, const double *r_in = nullptr)
 {
     const int n = c->neqns;
     const size_t V = 2 * c->kv_n;
@@ -590,12 +631,16 @@ static int refine_dev(mstm_hip_ctx *c, int niter, double eps, const double *pnp,
     int f1[2] = {2, 0};
     const int max_ref = 6;
     for (int ref = 0; ref <= max_ref; ref++) {
-        rc = matvec(c, 1, anp, y, f1, MSTM_HIP_FP64, use_fft);  // y = T H x
-        if (rc) return rc;
-        // r = b - x + y
-        HIPCHK(hipMemcpyAsync(r, pnp, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, c->st));
-        HIPCHK(launch_zaxpby(-1.0, 0.0, anp, 1.0, 0.0, r, n, c->st));
-        HIPCHK(launch_zaxpby(1.0, 0.0, y, 1.0, 0.0, r, n, c->st));
+        if (ref == 0 && r_in) {
+            HIPCHK(hipMemcpyAsync(r, r_in, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, c->st));
+        } else {
+            rc = matvec(c, 1, anp, y, f1, MSTM_HIP_FP64, use_fft);  // y = T H x
+            if (rc) return rc;
+            // r = b - x + y
+            HIPCHK(hipMemcpyAsync(r, pnp, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, c->st));
+            HIPCHK(launch_zaxpby(-1.0, 0.0, anp, 1.0, 0.0, r, n, c->st));
+            HIPCHK(launch_zaxpby(1.0, 0.0, y, 1.0, 0.0, r, n, c->st));
+        }
         double rr[2];
         HIPCHK(launch_zdot(r, r, n, 1, c->d_partials, nullptr, c->st));
         rc = sum_partials(c, 1, rr);
@@ -615,6 +660,211 @@ static int refine_dev(mstm_hip_ctx *c, int niter, double eps, const double *pnp,
         if (rc) return rc;
         *iters += std::min(it, niter - *iters);
         HIPCHK(launch_zaxpby(1.0, 0.0, d, 1.0, 0.0, anp, n, c->st));
+    }
+    return 0;
+}
+
+// features + network forward for the current spheres (learned.hpp model)
+static int lig_predict(mstm_hip_ctx *c)
+{
+    LearnedNet &g = c->lig;
+    const int n = c->nsph;
+    if (g.cap < n) {
+        release_lig_buffers(c);
+        HIPCHK(dalloc(&g.feat, sizeof(float) * (size_t)n * LIG_NFEAT));
+        HIPCHK(dalloc(&g.depth, sizeof(float) * (size_t)n));
+        HIPCHK(dalloc(&g.out, sizeof(float) * (size_t)n * g.s.nout));
+        g.H.assign(g.s.nlayers, nullptr);
+        for (auto &q : g.H) HIPCHK(dalloc(&q, sizeof(float) * (size_t)n * g.s.hidden));
+        g.cap = n;
+    }
+    HIPCHK(launch_lig_features(n, c->d_pos, c->d_radius, c->d_mre, c->d_mim, c->rimed, g.feat, g.depth, c->st));
+    HIPCHK(mlp_forward(g.s, g.params, g.feat, n, g.H.data(), nullptr, g.out, c->st));
+    return 0;
+}
+
+// Starting vector of the learned solver for polarization p: the network's
+// prediction xl and the default b are both evaluated in FP64 (one two-column
+// product) and the one with the smaller residual is kept in x, its residual
+// in r.  Returns through *used whether the prediction was kept.
+static int lig_start(mstm_hip_ctx *c, int p, const double *b, double *x, double *r, int use_fft, bool *used)
+{
+    const int n = c->neqns;
+    const size_t V = 2 * c->kv_n;
+    double *xl = c->d_kv + V * 6, *xb = xl + V, *y = c->d_kv, *rl = c->d_kv + V * 2, *rb = rl + V;
+    HIPCHK(launch_lig_combine(c->nsph, p, c->d_order, c->d_offset, c->d_pos, c->rimed, c->lig.out, c->lig.Lnet,
+                              c->lig.depth, c->lig.params + c->lig.np - 2, b, xl, c->st));
+    HIPCHK(hipMemcpyAsync(xb, b, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+    int f2[2] = {2, 2};
+    int rc = matvec(c, 2, xl, y, f2, MSTM_HIP_FP64, use_fft);
+    if (rc) return rc;
+    double e[2];
+    for (int q = 0; q < 2; q++) {
+        double *rq = q ? rb : rl, d2[2];
+        HIPCHK(hipMemcpyAsync(rq, b, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(launch_zaxpby(-1.0, 0.0, q ? xb : xl, 1.0, 0.0, rq, n, c->st));
+        HIPCHK(launch_zaxpby(1.0, 0.0, y + V * q, 1.0, 0.0, rq, n, c->st));
+        HIPCHK(launch_zdot(rq, rq, n, 1, c->d_partials, nullptr, c->st));
+        rc = sum_partials(c, 1, d2);
+        if (rc) return rc;
+        e[q] = d2[0];
+    }
+    double bb[2];
+    HIPCHK(launch_zdot(b, b, n, 1, c->d_partials, nullptr, c->st));
+    rc = sum_partials(c, 1, bb);
+    if (rc) return rc;
+    const double bn = bb[0] > 0.0 ? bb[0] : 1.0;
+    c->stats.lig_err0 = e[0] / bn;
+    c->stats.lig_errb = e[1] / bn;
+    *used = e[0] <= e[1];
+    HIPCHK(hipMemcpyAsync(x, *used ? xl : xb, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(r, *used ? rl : rb, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+    return 0;
+}
+
+// dst = s * src (dst cleared first: the update kernel reads it)
+static int scaled_copy(mstm_hip_ctx *c, double *dst, const double *src, double sr, double si)
+{
+    HIPCHK(hipMemsetAsync(dst, 0, sizeof(double) * 2 * c->neqns, c->st));
+    HIPCHK(launch_zaxpby(sr, si, src, 0.0, 0.0, dst, c->neqns, c->st));
+    return 0;
+}
+
+static int zdot_host(mstm_hip_ctx *c, const double *a, const double *b, std::complex<double> &out)
+{
+    double v[2];
+    HIPCHK(launch_zdot(a, b, c->neqns, 1, c->d_partials, nullptr, c->st));
+    int rc = sum_partials(c, 1, v);
+    out = std::complex<double>(v[0], v[1]);
+    return rc;
+}
+
+// Flexible GMRES(m) (Saad, SIAM J. Sci. Comput. 14, 461 (1993)) in FP64 on
+// A = I - T H from x = 0, right-preconditioned with a different map for every
+// Arnoldi vector v_j: for the first vector of the first cycle (v_1 = b/|b|, a
+// plane-wave right-hand side) the learned network's prediction when lig is
+// set; for every other one an FP32 BiCG solve of A z = v_j to the relative
+// accuracy the outer residual still needs.  Each cycle starts from the FP64
+// residual b - A x, so an answer is accepted only on the reference criterion
+// |r|^2/|b|^2 < eps.  iters counts the inner FP32 BiCG iterations.
+static int fgmres_dev(mstm_hip_ctx *c, int niter, double eps, const double *b, double *x, int use_fft, bool lig,
+                      int pol, int *iters, double *errmax)
+{
+    typedef std::complex<double> zc;
+    const int m = 8, max_cycles = 8;
+    const int n = c->neqns;
+    const size_t V = 2 * c->kv_n;
+    const size_t need = sizeof(double) * V * (2 * m + 1);
+    if (c->fg_bytes < need) {
+        dfree(c->d_fg);
+        HIPCHK(dalloc(&c->d_fg, need));
+        c->fg_bytes = need;
+    }
+    auto Vv = [&](int i) { return c->d_fg + V * i; };
+    auto Zv = [&](int i) { return c->d_fg + V * (m + 1 + i); };
+    double *w = c->d_kv + V * 6, *y = w + V, *r = y + V;
+    zc bbz;
+    int rc = zdot_host(c, b, b, bbz);
+    if (rc) return rc;
+    const double bb = bbz.real();
+    *iters = 0;
+    *errmax = 0.0;
+    HIPCHK(hipMemsetAsync(x, 0, sizeof(double) * V, c->st));
+    if (bb == 0.0) return 0;
+    int f1[2] = {2, 0};
+    for (int cycle = 0; cycle <= max_cycles; cycle++) {
+        double beta;
+        if (cycle == 0) {
+            HIPCHK(hipMemcpyAsync(r, b, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+            beta = std::sqrt(bb);
+        } else {
+            rc = matvec(c, 1, x, y, f1, MSTM_HIP_FP64, use_fft);  // y = T H x
+            if (rc) return rc;
+            HIPCHK(hipMemcpyAsync(r, b, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+            HIPCHK(launch_zaxpby(-1.0, 0.0, x, 1.0, 0.0, r, n, c->st));
+            HIPCHK(launch_zaxpby(1.0, 0.0, y, 1.0, 0.0, r, n, c->st));
+            zc rr;
+            rc = zdot_host(c, r, r, rr);
+            if (rc) return rc;
+            beta = std::sqrt(std::max(rr.real(), 0.0));
+        }
+        *errmax = beta * beta / bb;
+        if (*errmax < eps || *iters >= niter || cycle == max_cycles) break;
+        rc = scaled_copy(c, Vv(0), r, 1.0 / beta, 0.0);
+        if (rc) return rc;
+        std::vector<zc> H((size_t)(m + 1) * m, zc(0.0)), g(m + 1, zc(0.0)), sn(m, zc(0.0));
+        std::vector<double> cs(m, 0.0);
+        auto h = [&](int i, int j) -> zc & { return H[(size_t)i * m + j]; };
+        g[0] = beta;
+        int k = 0;  // steps taken in this cycle
+        for (int j = 0; j < m; j++) {
+            const double rho = std::norm(g[j]);
+            if (cycle == 0 && j == 0 && lig) {
+                // learned prediction of A^{-1} b, scaled like v_1 = b/|b|
+                HIPCHK(launch_lig_combine(c->nsph, pol, c->d_order, c->d_offset, c->d_pos, c->rimed, c->lig.out,
+                                          c->lig.Lnet, c->lig.depth, c->lig.params + c->lig.np - 2, b, y, c->st));
+                rc = scaled_copy(c, Zv(0), y, 1.0 / beta, 0.0);
+                if (rc) return rc;
+            } else {
+                HIPCHK(hipMemcpyAsync(Zv(j), Vv(j), sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+                const double eps_in = std::max(0.5 * eps * bb / rho, 1e-12);
+                int it = 0;
+                double ein = 0.0;
+                rc = bicg_dev(c, std::max(1, niter - *iters), eps_in, Vv(j), Zv(j), MSTM_HIP_FP32, use_fft, 0, &it,
+                              &ein);
+                if (rc) return rc;
+                *iters += std::min(it, std::max(1, niter - *iters));
+            }
+            // w = A z_j in FP64
+            rc = matvec(c, 1, Zv(j), y, f1, MSTM_HIP_FP64, use_fft);
+            if (rc) return rc;
+            HIPCHK(hipMemcpyAsync(w, Zv(j), sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+            HIPCHK(launch_zaxpby(-1.0, 0.0, y, 1.0, 0.0, w, n, c->st));
+            for (int i = 0; i <= j; i++) {  // modified Gram-Schmidt
+                rc = zdot_host(c, Vv(i), w, h(i, j));
+                if (rc) return rc;
+                HIPCHK(launch_zaxpby(-h(i, j).real(), -h(i, j).imag(), Vv(i), 1.0, 0.0, w, n, c->st));
+            }
+            zc ww;
+            rc = zdot_host(c, w, w, ww);
+            if (rc) return rc;
+            const double hn = std::sqrt(std::max(ww.real(), 0.0));
+            h(j + 1, j) = hn;
+            c->stats.fgmres_steps++;
+            k = j + 1;
+            if (hn > 0.0) {
+                rc = scaled_copy(c, Vv(j + 1), w, 1.0 / hn, 0.0);
+                if (rc) return rc;
+            }
+            for (int i = 0; i < j; i++) {  // earlier rotations on column j
+                const zc a = h(i, j), bq = h(i + 1, j);
+                h(i, j) = cs[i] * a + sn[i] * bq;
+                h(i + 1, j) = -std::conj(sn[i]) * a + cs[i] * bq;
+            }
+            const zc a = h(j, j), bq = h(j + 1, j);
+            const double nu = std::sqrt(std::norm(a) + std::norm(bq));
+            if (nu == 0.0) break;
+            if (std::abs(a) == 0.0) {
+                cs[j] = 0.0;
+                sn[j] = 1.0;
+            } else {
+                cs[j] = std::abs(a) / nu;
+                sn[j] = (a / std::abs(a)) * std::conj(bq) / nu;
+            }
+            h(j, j) = cs[j] * a + sn[j] * bq;
+            h(j + 1, j) = 0.0;
+            g[j + 1] = -std::conj(sn[j]) * g[j];
+            g[j] = cs[j] * g[j];
+            if (std::norm(g[j + 1]) / bb < eps || hn == 0.0 || *iters >= niter) break;
+        }
+        // x += Z y with R y = g
+        std::vector<zc> yv(k);
+        for (int i = k - 1; i >= 0; i--) {
+            zc s = g[i];
+            for (int q = i + 1; q < k; q++) s -= h(i, q) * yv[q];
+            yv[i] = s / h(i, i);
+        }
+        for (int i = 0; i < k; i++) HIPCHK(launch_zaxpby(yv[i].real(), yv[i].imag(), Zv(i), 1.0, 0.0, x, n, c->st));
     }
     return 0;
 }
@@ -665,18 +915,31 @@ int mstm_hip_fixedorsoln(mstm_hip_ctx *c, double alpha, double sinc, int dir, do
                          int use_fft, double *amnp, double *qeff, double *maxerr, int *maxiter, int *istat)
 {
     if (!c || !c->nsph) return fail(c, MSTM_HIP_ESTATE, "fixedorsoln: no spheres");
-    if (dir != 1) return fail(c, MSTM_HIP_EUNSUP, "fixedorsoln: only dir = 1 (no plane boundaries)");
+    if (dir != 1 && dir != 2) return fail(c, MSTM_HIP_EARG, "fixedorsoln: dir must be 1 or 2");
     if (niter <= 0) return fail(c, MSTM_HIP_EUNSUP, "fixedorsoln: niter <= 0 not supported");
-    if (solver != MSTM_HIP_SOLVE_PLAIN && solver != MSTM_HIP_SOLVE_MIXED)
+    if (solver != MSTM_HIP_SOLVE_PLAIN && solver != MSTM_HIP_SOLVE_MIXED && solver != MSTM_HIP_SOLVE_LEARNED &&
+        solver != MSTM_HIP_SOLVE_FGMRES)
         return fail(c, MSTM_HIP_EUNSUP, "fixedorsoln: solver %d not available", solver);
     if (!amnp) return fail(c, MSTM_HIP_EARG, "fixedorsoln: amnp is required");
+    if (solver == MSTM_HIP_SOLVE_LEARNED && !c->lig.ready)
+        return fail(c, MSTM_HIP_ESTATE, "fixedorsoln: the learned solver needs a network (mstm_hip_set_learned)");
     HIPCHK(hipSetDevice(c->device));
     const int n = c->neqns;
     const size_t V = 2 * (size_t)n;
     // work-vector slots: 0-5 BiCG, 6-8 refinement, 9-10 pmnp, 11 x, 12 b, 13-14 staging
     double *pmnp = c->d_kv + V * 9, *x = c->d_kv + V * 11, *b = c->d_kv + V * 12, *stage = c->d_kv + V * 13;
     auto t0 = std::chrono::steady_clock::now();
-    HIPCHK(launch_planewave(pmnp, 2 * n, c->nsph, c->d_pos, c->d_order, c->d_offset, alpha, sinc, c->rimed, c->st));
+    HIPCHK(launch_planewave(pmnp, 2 * n, c->nsph, c->d_pos, c->d_order, c->d_offset, alpha, sinc, dir, c->rimed, c->st));
+    // the network is trained for incidence along +z (beta = 0)
+    const bool lig_on = (solver == MSTM_HIP_SOLVE_LEARNED || solver == MSTM_HIP_SOLVE_FGMRES) && c->lig.ready &&
+                        sinc == 0.0 && dir == 1;
+    if (lig_on) {
+        auto tl = std::chrono::steady_clock::now();
+        int rc = lig_predict(c);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(c->st));
+        c->stats.lig_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tl).count();
+    }
     for (int p = 0; p < 2; p++) {
         // istat, maxiter, maxerr are reset per polarization (:471-475)
         *istat = 0;
@@ -687,8 +950,42 @@ int mstm_hip_fixedorsoln(mstm_hip_ctx *c, double alpha, double sinc, int dir, do
         HIPCHK(hipMemcpyAsync(x, b, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
         int it = 0, rc;
         double err = 0;
-        if (solver == MSTM_HIP_SOLVE_PLAIN) rc = bicg_dev(c, niter, eps, b, x, MSTM_HIP_FP64, use_fft, 0, &it, &err);
-        else rc = refine_dev(c, niter, eps, b, x, use_fft, &it, &err);
+        if (solver == MSTM_HIP_SOLVE_PLAIN) {
+            rc = bicg_dev(c, niter, eps, b, x, MSTM_HIP_FP64, use_fft, 0, &it, &err);
+        } else if (solver == MSTM_HIP_SOLVE_MIXED) {
+            rc = refine_dev(c, niter, eps, b, x, use_fft, &it, &err);
+        } else if (solver == MSTM_HIP_SOLVE_FGMRES) {
+            rc = fgmres_dev(c, niter, eps, b, x, use_fft, lig_on, p, &it, &err);
+            if (rc) return rc;
+            if (err >= eps && it < niter) {  // stalled: the plain FP64 solver from b
+                c->stats.lig_fallbacks++;
+                HIPCHK(hipMemcpyAsync(x, b, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+                int it2 = 0;
+                rc = bicg_dev(c, niter - it, eps, b, x, MSTM_HIP_FP64, use_fft, 0, &it2, &err);
+                it += it2;
+            }
+        } else {
+            // learned start (or b where the prediction is worse), mixed
+            // refinement from it; the plain FP64 solver from b takes over if
+            // the refinement does not reach eps
+            c->stats.lig_solves++;
+            bool used = false;
+            double *r0 = c->d_kv + V * 4;
+            if (lig_on) {
+                rc = lig_start(c, p, b, x, r0, use_fft, &used);
+                if (rc) return rc;
+            }
+            if (!used) c->stats.lig_fallbacks++;
+            rc = refine_dev(c, niter, eps, b, x, use_fft, &it, &err, lig_on ? r0 : nullptr);
+            if (rc) return rc;
+            if (err >= eps && it < niter) {
+                c->stats.lig_fallbacks++;
+                HIPCHK(hipMemcpyAsync(x, b, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+                int it2 = 0;
+                rc = bicg_dev(c, niter - it, eps, b, x, MSTM_HIP_FP64, use_fft, 0, &it2, &err);
+                it += it2;
+            }
+        }
         if (rc) return rc;
         HIPCHK(hipMemcpyAsync(stage + V * p, x, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
         *maxiter = it;
@@ -712,6 +1009,99 @@ int mstm_hip_fixedorsoln(mstm_hip_ctx *c, double alpha, double sinc, int dir, do
     return 0;
 }
 
+int mstm_hip_set_learned(mstm_hip_ctx *c, const float *params, int nparams, int hidden, int nlayers, int Lnet)
+{
+    if (!c) return MSTM_HIP_EARG;
+    HIPCHK(hipSetDevice(c->device));
+    release_lig_buffers(c);
+    dfree(c->lig.params);
+    c->lig.ready = false;
+    if (!params) return 0;
+    if (hidden <= 0 || hidden > 1024 || nlayers <= 0 || nlayers > 8 || Lnet <= 0 || Lnet > MSTM_MAX_L)
+        return fail(c, MSTM_HIP_EARG, "set_learned: bad shape (hidden %d, layers %d, Lnet %d)", hidden, nlayers, Lnet);
+    MlpShape s{LIG_NFEAT, hidden, nlayers, 8 * Lnet * (Lnet + 2)};
+    if ((size_t)nparams != mlp_nparams(s))
+        return fail(c, MSTM_HIP_EARG, "set_learned: expected %zu parameters, got %d", mlp_nparams(s), nparams);
+    HIPCHK(dalloc(&c->lig.params, sizeof(float) * nparams));
+    HIPCHK(hipMemcpy(c->lig.params, params, sizeof(float) * nparams, hipMemcpyHostToDevice));
+    c->lig.s = s;
+    c->lig.Lnet = Lnet;
+    c->lig.np = nparams;
+    c->lig.ready = true;
+    return 0;
+}
+
+int mstm_hip_lig_features(mstm_hip_ctx *c, float *feat, float *depth)
+{
+    if (!c || !c->nsph) return fail(c, MSTM_HIP_ESTATE, "lig_features: no spheres");
+    if (!feat || !depth) return fail(c, MSTM_HIP_EARG, "lig_features: null output");
+    HIPCHK(hipSetDevice(c->device));
+    const int n = c->nsph;
+    float *df = nullptr, *dd = nullptr;
+    HIPCHK(dalloc(&df, sizeof(float) * (size_t)n * LIG_NFEAT));
+    HIPCHK(dalloc(&dd, sizeof(float) * (size_t)n));
+    hipError_t e = launch_lig_features(n, c->d_pos, c->d_radius, c->d_mre, c->d_mim, c->rimed, df, dd, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(feat, df, sizeof(float) * (size_t)n * LIG_NFEAT, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(depth, dd, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    dfree(df);
+    dfree(dd);
+    if (e != hipSuccess) return fail(c, MSTM_HIP_EHIP, "lig_features: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int mstm_hip_lig_samples(mstm_hip_ctx *c, const double *amnp, int Lnet, float *target, float *base)
+{
+    if (!c || !c->nsph) return fail(c, MSTM_HIP_ESTATE, "lig_samples: no spheres");
+    if (!amnp || !target || !base || Lnet <= 0 || Lnet > MSTM_MAX_L) return fail(c, MSTM_HIP_EARG, "lig_samples: args");
+    HIPCHK(hipSetDevice(c->device));
+    const int n = c->neqns;
+    const size_t V = 2 * (size_t)n, row = (size_t)8 * Lnet * (Lnet + 2);
+    double *pmnp = c->d_kv + V * 9, *x = c->d_kv + V * 11, *b = c->d_kv + V * 12;
+    float *dt = nullptr, *db = nullptr;
+    HIPCHK(dalloc(&dt, sizeof(float) * row * c->nsph));
+    HIPCHK(dalloc(&db, sizeof(float) * row * c->nsph));
+    hipError_t e = launch_planewave(pmnp, 2 * n, c->nsph, c->d_pos, c->d_order, c->d_offset, 0.0, 0.0, 1, c->rimed, c->st);
+    for (int p = 0; p < 2 && e == hipSuccess; p++) {
+        e = launch_mie_mult(pmnp + V * p, b, c->nsph, c->d_order, c->d_offset, c->d_mieoff, c->d_an, 1, 0, c->Lmax, c->st);
+        if (e == hipSuccess) e = hipMemcpyAsync(x, amnp + V * p, sizeof(double) * V, hipMemcpyHostToDevice, c->st);
+        if (e == hipSuccess)
+            e = launch_lig_samples(c->nsph, p, c->d_order, c->d_offset, c->d_pos, c->rimed, x, b, Lnet, dt, db, c->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    }
+    if (e == hipSuccess) e = hipMemcpy(target, dt, sizeof(float) * row * c->nsph, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(base, db, sizeof(float) * row * c->nsph, hipMemcpyDeviceToHost);
+    dfree(dt);
+    dfree(db);
+    if (e != hipSuccess) return fail(c, MSTM_HIP_EHIP, "lig_samples: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int mstm_hip_lig_guess(mstm_hip_ctx *c, double *x0)
+{
+    if (!c || !c->nsph) return fail(c, MSTM_HIP_ESTATE, "lig_guess: no spheres");
+    if (!c->lig.ready) return fail(c, MSTM_HIP_ESTATE, "lig_guess: no network (mstm_hip_set_learned)");
+    if (!x0) return fail(c, MSTM_HIP_EARG, "lig_guess: null output");
+    HIPCHK(hipSetDevice(c->device));
+    const int n = c->neqns;
+    const size_t V = 2 * (size_t)n;
+    double *pmnp = c->d_kv + V * 9, *x = c->d_kv + V * 11, *b = c->d_kv + V * 12;
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = lig_predict(c);
+    if (rc) return rc;
+    HIPCHK(launch_planewave(pmnp, 2 * n, c->nsph, c->d_pos, c->d_order, c->d_offset, 0.0, 0.0, 1, c->rimed, c->st));
+    for (int p = 0; p < 2; p++) {
+        HIPCHK(launch_mie_mult(pmnp + V * p, b, c->nsph, c->d_order, c->d_offset, c->d_mieoff, c->d_an, 1, 0, c->Lmax,
+                               c->st));
+        HIPCHK(launch_lig_combine(c->nsph, p, c->d_order, c->d_offset, c->d_pos, c->rimed, c->lig.out, c->lig.Lnet,
+                                  c->lig.depth, c->lig.params + c->lig.np - 2, b, x, c->st));
+        HIPCHK(hipMemcpyAsync(x0 + V * p, x, sizeof(double) * V, hipMemcpyDeviceToHost, c->st));
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->stats.lig_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
 int mstm_hip_clear(mstm_hip_ctx *c, int clear_tables)
 {
     if (!c) return MSTM_HIP_EARG;
@@ -728,7 +1118,8 @@ const char *mstm_hip_stats_names(void)
 {
     return "tables_ms_fp64,tables_ms_fp32,tile_ms_fp64,tile_ms_fp32,tile_launches_fp64,tile_launches_fp32,"
            "matvecs_fp64,matvecs_fp32,solve_ms,iters_fp64,iters_fp32,refinements,table_bytes_fp64,table_bytes_fp32,"
-           "fft_setup_ms,fft_conv_ms,fft_launches,npad,ntiles,nblocks";
+           "fft_setup_ms,fft_conv_ms,fft_launches,npad,ntiles,nblocks,lig_solves,lig_fallbacks,lig_ms,lig_err0,"
+           "lig_errb,fgmres_steps";
 }
 
 int mstm_hip_stats(const mstm_hip_ctx *c, double *out, int nout)
@@ -753,7 +1144,13 @@ int mstm_hip_stats(const mstm_hip_ctx *c, double *out, int nout)
                   c->fft ? fft_stat(c->fft, 2) : 0.0,
                   (double)c->npad,
                   (double)c->ntiles,
-                  (double)c->nblocks};
+                  (double)c->nblocks,
+                  (double)c->stats.lig_solves,
+                  (double)c->stats.lig_fallbacks,
+                  c->stats.lig_ms,
+                  c->stats.lig_err0,
+                  c->stats.lig_errb,
+                  (double)c->stats.fgmres_steps};
     int k = std::min(nout, (int)(sizeof v / sizeof v[0]));
     for (int q = 0; q < k; q++) out[q] = v[q];
     return k;

@@ -174,12 +174,13 @@ hipError_t launch_mie_mult(const double *ain, double *aout, int nsph, const int 
 // thread per sphere.  pmnp(:,pol) columns of length ld (complex).
 __global__ void planewave_kernel(cplx<double> *__restrict__ pmnp, int ld, int nsph, const double *__restrict__ pos,
                                  const int *__restrict__ order, const int *__restrict__ offset, double alpha,
-                                 double sinc, double rimed)
+                                 double sinc, int dir, double rimed)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nsph) return;
     int L = order[i], NB = L * (L + 2);
-    double cb = sqrt((1.0 - sinc / rimed) * (1.0 + sinc / rimed));
+    // dir = 1: incidence from below (cos beta > 0), dir = 2: from above (:3934)
+    double cb = sqrt((1.0 - sinc / rimed) * (1.0 + sinc / rimed)) * (double)(3 - 2 * dir);
     const double *x = pos + 3 * i;
     double ph = sinc * (cos(alpha) * x[0] + sin(alpha) * x[1]) + rimed * cb * x[2];
     cplx<double> eph = cmk(cos(ph), sin(ph));
@@ -246,11 +247,12 @@ __global__ void planewave_kernel(cplx<double> *__restrict__ pmnp, int ld, int ns
 }
 
 hipError_t launch_planewave(double *pmnp, int ld, int nsph, const double *pos, const int *order, const int *offset,
-                            double alpha, double sinc, double rimed, hipStream_t st)
+                            double alpha, double sinc, int dir, double rimed, hipStream_t st)
 {
     int bs = 64;
     hipLaunchKernelGGL(planewave_kernel, dim3((nsph + bs - 1) / bs), dim3(bs), 0, st,
-                       reinterpret_cast<cplx<double> *>(pmnp), ld / 2, nsph, pos, order, offset, alpha, sinc, rimed);
+                       reinterpret_cast<cplx<double> *>(pmnp), ld / 2, nsph, pos, order, offset, alpha, sinc, dir,
+                       rimed);
     return hipGetLastError();
 }
 

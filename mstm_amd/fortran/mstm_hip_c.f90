@@ -9,7 +9,7 @@
       implicit none
       integer(c_int), parameter :: mstm_hip_fp64=0, mstm_hip_fp32=1
       integer(c_int), parameter :: mstm_hip_solve_plain=0, mstm_hip_solve_mixed=1, &
-         mstm_hip_solve_learned=2
+         mstm_hip_solve_learned=2, mstm_hip_solve_fgmres=3
       interface
          integer(c_int) function mstm_hip_create(ctx,device) bind(c,name='mstm_hip_create')
          import :: c_ptr, c_int
@@ -89,6 +89,13 @@
          real(c_double) :: qeff(*),maxerr
          integer(c_int) :: maxiter,istat
          end function mstm_hip_fixedorsoln
+         integer(c_int) function mstm_hip_set_learned(ctx,params,nparams,hidden,nlayers,lnet) &
+            bind(c,name='mstm_hip_set_learned')
+         import :: c_ptr, c_int, c_float
+         type(c_ptr), value :: ctx
+         real(c_float) :: params(*)
+         integer(c_int), value :: nparams,hidden,nlayers,lnet
+         end function mstm_hip_set_learned
          integer(c_int) function mstm_hip_clear(ctx,clear_tables) bind(c,name='mstm_hip_clear')
          import :: c_ptr, c_int
          type(c_ptr), value :: ctx

@@ -91,8 +91,30 @@
          iter=it
          end subroutine hip_cbicg
 !
+! network of the learned solver from a raw weight file (int32 hidden,
+! nlayers, lnet, nparams, then nparams float32), as mstm_amd.learned writes it
+!
+         subroutine hip_load_network(fname)
+         character(*) :: fname
+         integer :: u,hd(4)
+         real(c_float), allocatable :: p(:)
+         if(.not.c_associated(hip_ctx)) then
+            call hip_check(mstm_hip_create(hip_ctx,0_c_int),'create')
+         endif
+         open(newunit=u,file=fname,access='stream',form='unformatted',status='old')
+         read(u) hd
+         allocate(p(hd(4)))
+         read(u) p
+         close(u)
+         call hip_check(mstm_hip_set_learned(hip_ctx,p,int(hd(4),c_int),int(hd(1),c_int),int(hd(2),c_int), &
+            int(hd(3),c_int)),'set_learned')
+         deallocate(p)
+         end subroutine hip_load_network
+!
 ! fixedorsoln (mstm-solver-8.f90:350-569), iterative branch; solution_method
-! 'm' selects the mixed FP32/FP64 solver, anything else the FP64 one
+! 'm' selects the mixed FP32/FP64 solver, 'l' the mixed solver from the
+! learned start (hip_load_network first), 'g' flexible GMRES, anything else
+! the FP64 one
 !
          subroutine hip_fixedorsoln(alpha,sinc,dir,eps,niter,amnp,qeff,qeffdim,maxerr,maxiter, &
             iterwrite,istat,mpi_comm,excited_spheres,solution_method,initialize_solver)
@@ -111,6 +133,8 @@
          solver=mstm_hip_solve_plain
          if(present(solution_method)) then
             if(solution_method.eq.'m') solver=mstm_hip_solve_mixed
+            if(solution_method.eq.'l') solver=mstm_hip_solve_learned
+            if(solution_method.eq.'g') solver=mstm_hip_solve_fgmres
          endif
          ufft=0
          if(fft_translation_option) ufft=1

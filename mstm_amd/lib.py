@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libmstm_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "mstm_hip.h")
 
 FP64, FP32 = 0, 1
-SOLVE_PLAIN, SOLVE_MIXED, SOLVE_LEARNED = 0, 1, 2
+SOLVE_PLAIN, SOLVE_MIXED, SOLVE_LEARNED, SOLVE_FGMRES = 0, 1, 2, 3
 
 
 class MstmHipError(RuntimeError):
@@ -26,6 +26,7 @@ _lib = None
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _vp = C.c_void_p
+_fp = C.POINTER(C.c_float)
 
 _SIGNATURES = {
     "mstm_hip_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
@@ -46,6 +47,33 @@ _SIGNATURES = {
     "mstm_hip_clear": (C.c_int, [_vp, C.c_int]),
     "mstm_hip_stats": (C.c_int, [_vp, _dp, C.c_int]),
     "mstm_hip_stats_names": (C.c_char_p, []),
+    "mstm_hip_lig_nparams": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+    "mstm_hip_set_learned": (C.c_int, [_vp, _fp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "mstm_hip_lig_features": (C.c_int, [_vp, _fp, _fp]),
+    "mstm_hip_lig_samples": (C.c_int, [_vp, _dp, C.c_int, _fp, _fp]),
+    "mstm_hip_lig_guess": (C.c_int, [_vp, _dp]),
+    "mstm_hip_trainer_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint]),
+    "mstm_hip_trainer_destroy": (C.c_int, [_vp]),
+    "mstm_hip_trainer_last_error": (C.c_char_p, [_vp]),
+    "mstm_hip_trainer_params": (C.c_int, [_vp, _fp, C.c_int, C.c_int]),
+    "mstm_hip_trainer_set_data": (C.c_int, [_vp, C.c_int, _fp, _fp, _fp, _fp]),
+    "mstm_hip_trainer_epoch": (C.c_int, [_vp, C.c_int, C.c_float, C.c_uint, _dp]),
+    "mstm_hip_trainer_loss": (C.c_int, [_vp, C.c_int, _dp]),
+    "mstm_hip_trainer_grad": (C.c_int, [_vp, C.c_int, _fp, _fp, _fp, _fp, _fp, _dp]),
+    "mstm_hip_trainer_forward": (C.c_int, [_vp, C.c_int, _fp, _fp]),
+    "mstm_hip_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    "mstm_hip_batch_destroy": (C.c_int, [_vp]),
+    "mstm_hip_batch_last_error": (C.c_char_p, [_vp]),
+    "mstm_hip_batch_set": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int, _dp, _dp]),
+    "mstm_hip_batch_setup": (C.c_int, [_vp, C.c_int]),
+    "mstm_hip_batch_orders": (C.c_int, [_vp, _ip]),
+    "mstm_hip_batch_solve": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int, _ip, _dp]),
+    "mstm_hip_batch_vectors": (C.c_int, [_vp, C.c_int, _dp]),
+    "mstm_hip_batch_product": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
+    "mstm_hip_batch_qeff": (C.c_int, [_vp, _dp]),
+    "mstm_hip_batch_stats": (C.c_int, [_vp, _dp, C.c_int]),
+    "mstm_hip_batch_stats_names": (C.c_char_p, []),
+    "mstm_hip_batch_tune_table": (C.c_int, [_vp, _dp, C.c_int]),
 }
 
 

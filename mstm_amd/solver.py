@@ -11,7 +11,7 @@ import ctypes as C
 
 import numpy as np
 
-from .lib import FP32, FP64, SOLVE_LEARNED, SOLVE_MIXED, SOLVE_PLAIN, MstmHipError, load_library  # noqa: F401
+from .lib import FP32, FP64, SOLVE_FGMRES, SOLVE_LEARNED, SOLVE_MIXED, SOLVE_PLAIN, MstmHipError, load_library  # noqa: F401
 
 
 def _d(a):
@@ -121,17 +121,58 @@ class HipContext:
                                         int(bool(fft_option)), _i(it), _d(err)), "cbicg")
         return x, int(it[0]), float(err[0])
 
-    def fixedorsoln(self, alpha=0.0, sinc=0.0, eps=1e-6, niter=10000, solver=SOLVE_PLAIN, fft_option=False):
+    def fixedorsoln(self, alpha=0.0, sinc=0.0, eps=1e-6, niter=10000, solver=SOLVE_PLAIN, fft_option=False,
+                    direction=1):
         ne = self.number_eqns
         amnp = np.zeros((2, ne), np.complex128)
         qeff = np.zeros((self.nsphere, 3, 3))
         maxerr = np.zeros(1)
         maxiter = np.zeros(1, np.int32)
         istat = np.zeros(1, np.int32)
-        self._chk(self.L.mstm_hip_fixedorsoln(self.h, alpha, sinc, 1, eps, niter, solver, int(bool(fft_option)),
+        self._chk(self.L.mstm_hip_fixedorsoln(self.h, alpha, sinc, direction, eps, niter, solver, int(bool(fft_option)),
                                               _d(amnp.view(np.float64)), _d(qeff), _d(maxerr), _i(maxiter),
                                               _i(istat)), "fixedorsoln")
         return dict(amnp=amnp, qeff=qeff, maxerr=float(maxerr[0]), iterations=int(maxiter[0]), istat=int(istat[0]))
+
+    # -- learned initial guess (no reference counterpart) ---------------------
+    def set_learned(self, params=None, hidden: int = 128, nlayers: int = 3, lnet: int = 4):
+        """Load a network for SOLVE_LEARNED (None unloads it)."""
+        if params is None:
+            self._chk(self.L.mstm_hip_set_learned(self.h, None, 0, 0, 0, 0), "set_learned")
+            return
+        p = np.ascontiguousarray(params, np.float32)
+        self._lig_params = p
+        self._chk(self.L.mstm_hip_set_learned(self.h, p.ctypes.data_as(C.POINTER(C.c_float)), len(p), hidden, nlayers,
+                                              lnet), "set_learned")
+
+    def load_learned(self, path: str | None = None):
+        """Load a weight file written by mstm_amd.learned (default: the shipped one)."""
+        from . import learned
+
+        p, hidden, nlayers, lnet, meta = learned.load_weights(path or learned.DEFAULT_WEIGHTS)
+        self.set_learned(p, hidden, nlayers, lnet)
+        return meta
+
+    def lig_features_into(self, feat, depth):
+        self._chk(self.L.mstm_hip_lig_features(self.h, feat.ctypes.data_as(C.POINTER(C.c_float)),
+                                               depth.ctypes.data_as(C.POINTER(C.c_float))), "lig_features")
+
+    def lig_features(self):
+        feat = np.zeros((self.nsphere, 64), np.float32)
+        depth = np.zeros(self.nsphere, np.float32)
+        self.lig_features_into(feat, depth)
+        return feat, depth
+
+    def lig_samples_into(self, amnp, lnet, target, base):
+        a = np.ascontiguousarray(amnp, np.complex128)
+        self._chk(self.L.mstm_hip_lig_samples(self.h, _d(a.view(np.float64)), lnet,
+                                              target.ctypes.data_as(C.POINTER(C.c_float)),
+                                              base.ctypes.data_as(C.POINTER(C.c_float))), "lig_samples")
+
+    def lig_guess(self):
+        x0 = np.zeros((2, self.number_eqns), np.complex128)
+        self._chk(self.L.mstm_hip_lig_guess(self.h, _d(x0.view(np.float64))), "lig_guess")
+        return x0
 
     def clear(self, clear_tables=True):
         self._chk(self.L.mstm_hip_clear(self.h, int(bool(clear_tables))), "clear")

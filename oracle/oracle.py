@@ -146,12 +146,13 @@ class OracleCluster:
         it = lib().orc_cbicg(self.h, int(use_fft), niter, eps, pnp.ctypes.data, anp.ctypes.data, _d(err))
         return anp, it, float(err[0])
 
-    def fixedorsoln(self, alpha=0.0, sinc=0.0, eps=1e-6, niter=10000, use_fft=False):
+    def fixedorsoln(self, alpha=0.0, sinc=0.0, eps=1e-6, niter=10000, use_fft=False, direction=1):
         amnp = np.zeros((2, self.neqns), np.complex128)
         qeff = np.zeros((self.n, 3, 3))
         maxerr = np.zeros(1)
         maxiter = np.zeros(1, np.int32)
-        istat = lib().orc_fixedorsoln(self.h, int(use_fft), alpha, sinc, 1, eps, niter, amnp.ctypes.data, _d(qeff),
+        istat = lib().orc_fixedorsoln(self.h, int(use_fft), alpha, sinc, direction, eps, niter, amnp.ctypes.data,
+                                      _d(qeff),
                                       _d(maxerr), _i(maxiter))
         return dict(amnp=amnp, qeff=qeff, maxerr=float(maxerr[0]), iterations=int(maxiter[0]), istat=istat)
 

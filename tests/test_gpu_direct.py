@@ -110,3 +110,39 @@ def test_nonconvergence_flag(gpu_ctx):
     load_cluster(gpu_ctx, g)
     r = gpu_ctx.fixedorsoln(eps=1e-6, niter=2)
     assert r["istat"] == 1 and r["iterations"] == 3
+
+
+def test_fixedorsoln_oblique_and_from_above_matches_oracle(gpu_ctx):
+    """Incidence at beta = 40 deg from below (dir 1) and beta = 130 deg from
+    above (dir 2, cos beta < 0; layerplanewavecoef :3934): the plain FP64
+    solve follows the oracle iteration for iteration."""
+    from oracle import oracle as O
+
+    g = golden("cfg1_direct")
+    load_cluster(gpu_ctx, g)
+    oc = O.OracleCluster(g["pos"], g["radius"], g["ri"])
+    for beta, alpha, sd in ((np.radians(40.0), 0.7, 1), (np.radians(130.0), 2.1, 2)):
+        sinc = float(np.sin(beta))
+        r = gpu_ctx.fixedorsoln(alpha=alpha, sinc=sinc, eps=1e-6, niter=10000, direction=sd)
+        ro = oc.fixedorsoln(alpha=alpha, sinc=sinc, eps=1e-6, niter=10000, direction=sd)
+        assert r["iterations"] == ro["iterations"]
+        assert rel(r["amnp"], ro["amnp"]) < FP64_TOL
+        assert rel(r["qeff"], ro["qeff"]) < FP64_TOL
+
+
+def test_fixedorsoln_fgmres_reaches_reference_residual(gpu_ctx):
+    """Flexible GMRES (FP64 outer, FP32 BiCG inner) ends on the reference
+    criterion and agrees with the plain FP64 solution to what that residual
+    allows; driven to a tight residual it agrees to 1e-5."""
+    from mstm_amd.solver import SOLVE_FGMRES, qtotcalc
+
+    g = golden("cfg1_direct")
+    load_cluster(gpu_ctx, g)
+    r = gpu_ctx.fixedorsoln(eps=1e-6, niter=10000, solver=SOLVE_FGMRES)
+    assert r["istat"] == 0 and r["maxerr"] < 1e-6
+    q = qtotcalc(g["radius"], r["qeff"])
+    assert abs(q[0, 0] - g["qtot"][0, 0]) / g["qtot"][0, 0] < 1e-3
+    rt = gpu_ctx.fixedorsoln(eps=1e-14, niter=10000, solver=SOLVE_FGMRES)
+    rp = gpu_ctx.fixedorsoln(eps=1e-14, niter=10000)
+    assert rt["istat"] == 0
+    assert rel(rt["amnp"], rp["amnp"]) < 1e-5
