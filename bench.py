@@ -8,10 +8,13 @@ reference's stopping rule |r|^2/|b|^2 < 1e-6.
 
 A step is one full fixed-orientation solve (fixedorsoln): incident-field
 RHS, T-matrix multiply, solve for both polarizations, efficiencies.  The
-mixed path (FP32 translation tables and arithmetic inside FP64 iterative
-refinement) is timed for K steps; the plain FP64 GPU solve is timed once
-beside it.  Translation tables are built once per geometry (setup, reported
-separately, like the reference's store_translation_matrix option).
+learned + mixed path (the shipped per-sphere network's starting vector, FP32
+translation tables and arithmetic inside FP64 iterative refinement) is timed
+for K steps; the mixed path without the network and the plain FP64 GPU solve
+are timed beside it, and a plain FP64 solve driven to |r|^2/|b|^2 < 1e-10 is
+the reference for the Q_ext/Q_abs errors.  Translation tables are built once
+per geometry (setup, reported separately, like the reference's
+store_translation_matrix option).
 
 Prints one JSON line.  --gpus N > 1 is not supported (one-GPU build).
 """
@@ -87,6 +90,7 @@ def main():
     ap.add_argument("--nsphere", type=int, default=5000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-tight", action="store_true", help="skip the tight FP64 reference solve")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many MiB (rocprof FETCH/WRITE calibration)")
     args = ap.parse_args()
@@ -104,7 +108,7 @@ def main():
         sys.exit(2)
 
     from mstm_amd import geometry as G
-    from mstm_amd.solver import FP32, FP64, SOLVE_MIXED, SOLVE_PLAIN, HipContext, qtotcalc
+    from mstm_amd.solver import FP32, FP64, SOLVE_LEARNED, SOLVE_MIXED, SOLVE_PLAIN, HipContext, qtotcalc
 
     t0 = time.time()
     cl = G.rsa_in_sphere(args.nsphere, 2.0, 0.30, seed=5, ri=1.31 + 0.0j)
@@ -121,20 +125,34 @@ def main():
     st0 = ctx.stats()
     print(f"[bench] tables built in {t_tables:.2f} s", file=sys.stderr, flush=True)
 
-    # ---- mixed path: warmup + timed steps ----------------------------------
+    meta = ctx.load_learned()
+
+    # ---- learned + mixed path: warmup + timed steps --------------------------
     for _ in range(args.warmup):
-        ctx.fixedorsoln(eps=1e-6, niter=10000, solver=SOLVE_MIXED)
+        ctx.fixedorsoln(eps=1e-6, niter=10000, solver=SOLVE_LEARNED)
     ctx.clear(clear_tables=False)  # reset counters only
     times, res = [], None
     for k in range(args.steps):
         t0 = time.perf_counter()
-        res = ctx.fixedorsoln(eps=1e-6, niter=10000, solver=SOLVE_MIXED)
+        res = ctx.fixedorsoln(eps=1e-6, niter=10000, solver=SOLVE_LEARNED)
         times.append(time.perf_counter() - t0)
-        print(f"[bench] mixed step {k}: {times[-1]:.3f} s, iters {res['iterations']}, err {res['maxerr']:.2e}",
+        print(f"[bench] learned step {k}: {times[-1]:.3f} s, iters {res['iterations']}, err {res['maxerr']:.2e}",
               file=sys.stderr, flush=True)
+    st_lig = ctx.stats()
+    lig_s = float(np.mean(times))
+    q_lig = qtotcalc(cl.radius, res["qeff"])
+
+    # ---- mixed path without the network --------------------------------------
+    ctx.clear(clear_tables=False)
+    mtimes = []
+    for k in range(max(1, min(2, args.steps))):
+        t0 = time.perf_counter()
+        rm = ctx.fixedorsoln(eps=1e-6, niter=10000, solver=SOLVE_MIXED)
+        mtimes.append(time.perf_counter() - t0)
     st_mixed = ctx.stats()
-    mixed_s = float(np.mean(times))
-    q_mixed = qtotcalc(cl.radius, res["qeff"])
+    mixed_s = float(np.mean(mtimes))
+    q_mixed = qtotcalc(cl.radius, rm["qeff"])
+    print(f"[bench] mixed (no network): {mixed_s:.3f} s, iters {rm['iterations']}", file=sys.stderr, flush=True)
 
     # ---- plain FP64 path (once) ----------------------------------------------
     plain = None
@@ -151,12 +169,26 @@ def main():
                      iters_fp64=st_plain["iters_fp64"])
         print(f"[bench] plain FP64 solve: {plain_s:.3f} s, iters {rp['iterations']}", file=sys.stderr, flush=True)
 
+    # ---- tight FP64 reference for the efficiency errors ----------------------
+    qerr = None
+    if not args.no_tight:
+        ctx.clear(clear_tables=False)
+        rt = ctx.fixedorsoln(eps=1e-10, niter=10000, solver=SOLVE_PLAIN)
+        q_ref = qtotcalc(cl.radius, rt["qeff"])
+        qerr = {}
+        for name, q in (("learned", q_lig), ("mixed", q_mixed)) + ((("plain", q_plain),) if plain else ()):
+            qerr[name] = dict(qext=abs(q[0, 0] - q_ref[0, 0]) / abs(q_ref[0, 0]),
+                              qsca=abs(q[0, 2] - q_ref[0, 2]) / abs(q_ref[0, 2]),
+                              qabs_abs=abs(q[0, 1] - q_ref[0, 1]))
+        qerr["reference"] = dict(eps=1e-10, qext=q_ref[0, 0], qabs=q_ref[0, 1], qsca=q_ref[0, 2])
+        print(f"[bench] tight FP64 reference done", file=sys.stderr, flush=True)
+
     # ---- roofline of the dominant kernel (FP32 tile kernel) ------------------
     npairs = cl.n * (cl.n - 1) // 2
     vec_bytes = 2 * 2 * ne * 8
     alg32 = npairs * table_bytes_per_pair(L, 4) + vec_bytes
     alg64 = npairs * table_bytes_per_pair(L, 8) + 2 * 2 * ne * 16
-    t32 = st_mixed["tile_ms_fp32"] / max(1, st_mixed["tile_launches_fp32"])
+    t32 = st_lig["tile_ms_fp32"] / max(1, st_lig["tile_launches_fp32"])
     ach32 = alg32 / (t32 * 1e-3) / 1e9
     traffic = None
     pmc = os.path.join(HERE, "profiles", "tile_fp32_cfg5_pmc.json")
@@ -174,28 +206,40 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(cl, plain["iters_fp64"] // 2 if plain else 60)
 
+    lsolves = max(1.0, st_lig["lig_solves"])
     out = {
         "metric": METRIC,
-        "value": round(mixed_s, 4),
+        "value": round(lig_s, 4),
         "unit": "s/solve",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(mixed_s * 1e3, 2),
+        "ms_per_step": round(lig_s * 1e3, 2),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64 translation (FP32) + f64 refinement",
         "data": "synthetic (seeded random sequential addition)",
         "config": {"workload": f"{cl.n} spheres x=2.0 m=1.31 RSA sphere fv=0.30 seed=5, direct translation, "
-                               "plane wave +z, 2 pol, |r|^2/|b|^2<1e-6 (mixed FP32/FP64)",
+                               "plane wave +z, 2 pol, |r|^2/|b|^2<1e-6 (learned start + mixed FP32/FP64)",
                    "nsphere": cl.n, "order": L, "number_eqns": ne, "pairs": npairs},
         "plain_fp64": plain,
-        "speedup_vs_plain_fp64": round(plain["s_per_solve"] / mixed_s, 3) if plain else None,
-        "mixed": dict(iterations_last_pol=res["iterations"], maxerr=res["maxerr"], qext=q_mixed[0, 0],
-                      qabs=q_mixed[0, 1], iters_fp32=st_mixed["iters_fp32"] / args.steps,
-                      iters_fp64_matvecs=st_mixed["matvecs_fp64"] / args.steps,
-                      refinements=st_mixed["refinements"] / args.steps),
+        "speedup_vs_plain_fp64": round(plain["s_per_solve"] / lig_s, 3) if plain else None,
+        "learned": dict(iterations_last_pol=res["iterations"], maxerr=res["maxerr"], qext=q_lig[0, 0],
+                        qabs=q_lig[0, 1], iters_fp32=st_lig["iters_fp32"] / args.steps,
+                        iters_fp64_matvecs=st_lig["matvecs_fp64"] / args.steps,
+                        refinements=st_lig["refinements"] / args.steps,
+                        fallback_rate=st_lig["lig_fallbacks"] / lsolves,
+                        start_residual_learned=st_lig["lig_err0"], start_residual_default=st_lig["lig_errb"],
+                        network_ms=st_lig["lig_ms"], network=dict(samples=meta.get("samples"),
+                                                                 configs=meta.get("configs"),
+                                                                 loss=meta.get("loss_final"))),
+        "mixed": dict(s_per_solve=round(mixed_s, 4), iterations_last_pol=rm["iterations"], maxerr=rm["maxerr"],
+                      qext=q_mixed[0, 0], qabs=q_mixed[0, 1], iters_fp32=st_mixed["iters_fp32"] / len(mtimes),
+                      iters_fp64_matvecs=st_mixed["matvecs_fp64"] / len(mtimes),
+                      refinements=st_mixed["refinements"] / len(mtimes),
+                      speedup_vs_plain_fp64=round(plain["s_per_solve"] / mixed_s, 3) if plain else None),
+        "q_error_vs_tight_fp64": qerr,
         "matvec": matvec,
         "setup_s": dict(geometry=round(t_geom, 2), tables_fp64_fp32=round(t_tables, 3),
                         tables_fp64_ms=st0["tables_ms_fp64"], tables_fp32_ms=st0["tables_ms_fp32"],

@@ -97,16 +97,25 @@ def _rsa(n: int, a: float, sample, rng: np.random.Generator, max_tries: int = 2_
 
 
 def rsa_in_sphere(n: int, a: float, fv: float, seed: int, ri: complex, ri_medium: complex = 1.0) -> Cluster:
-    """n spheres of radius a, centres uniform in |r| <= R - a, R = a (n/fv)^(1/3)."""
+    """n spheres of radius a, centres uniform in |r| <= R - a, R = a (n/fv)^(1/3).
+    Small clusters at high fv can jam (few centres fit in R - a): the bounding
+    radius then grows by 3% per retry, same seed, until the addition completes."""
     R = a * (n / fv) ** (1.0 / 3.0)
-    rin = R - a
+    for attempt in range(40):
+        rin = R - a
 
-    def sample(rng, m):
-        v = rng.normal(size=(m, 3))
-        v /= np.linalg.norm(v, axis=1)[:, None]
-        return v * (rin * rng.random(m) ** (1.0 / 3.0))[:, None]
+        def sample(rng, m):
+            v = rng.normal(size=(m, 3))
+            v /= np.linalg.norm(v, axis=1)[:, None]
+            return v * (rin * rng.random(m) ** (1.0 / 3.0))[:, None]
 
-    pos = _rsa(n, a, sample, np.random.default_rng(seed))
+        try:
+            pos = _rsa(n, a, sample, np.random.default_rng(seed), max_tries=2_000_000 if attempt == 0 else 200_000)
+            break
+        except RuntimeError:
+            R *= 1.03
+    else:
+        raise RuntimeError(f"RSA jammed for n={n}, fv={fv}")
     return Cluster(pos, np.full(n, a), np.full(n, ri, dtype=complex), ri_medium)
 
 
