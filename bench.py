@@ -82,6 +82,135 @@ def cpu_baseline(cluster, iters_plain: int, budget_s: float = 20.0):
                 s_per_matvec_full=full_matvec)
 
 
+def _line(metric, value, unit, steps, warmup, ms, dtype, workload, extra, higher=False):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": ms, "higher_is_better": higher, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic (seeded)", "config": {"workload": workload}}
+    out.update(extra)
+    print(json.dumps(out))
+
+
+def bench_batch(args):
+    """256 clusters (N log-uniform 10-300, x 0.5-3, fv 0.05-0.30, m 1.33+0.001i),
+    32 Fibonacci directions x 2 polarizations each, |r|^2/|b|^2 < 1e-6."""
+    from concurrent.futures import ProcessPoolExecutor
+
+    from mstm_amd import geometry as G
+    from mstm_amd.batch import BatchSolver, random_clusters
+
+    with ProcessPoolExecutor(8) as ex:  # geometry before the GPU is touched
+        cl = list(ex.map(_batch_cluster, range(256), chunksize=8))
+    b = BatchSolver(0)
+    dirs = G.fibonacci_directions(32)
+    t0 = time.perf_counter()
+    b.set(cl, dirs)
+    b.setup(tune=True)
+    t_setup = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        b.solve(eps=1e-6, niter=3000)
+    times = []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        r = b.solve(eps=1e-6, niter=3000)
+        times.append(time.perf_counter() - t0)
+        print(f"[bench] batch step {k}: {times[-1]:.3f} s", file=sys.stderr, flush=True)
+    st = b.stats()
+    nsolve = len(cl) * 64
+    s_all = float(np.mean(times))
+    q = b.qeff()
+    tf = st["gemm32_flops"] / (st["gemm32_ms"] * 1e-3) / 1e12
+    _line("s per batched solve (256 clusters x 64 right-hand sides) to residual 1e-6", round(s_all, 4), "s",
+          args.steps, args.warmup, round(s_all * 1e3, 2), "c64 MFMA contraction + f64 refinement",
+          "256 clusters N~logU[10,300] x~U[0.5,3] fv~U[0.05,0.3] m=1.33+0.001i, 32 Fibonacci directions x 2 pol",
+          {"solves": nsolve, "s_per_solve": s_all / nsolve, "spheres": int(sum(c.n for c in cl)),
+           "orders": np.bincount(b.order).tolist(), "setup_s": round(t_setup, 3),
+           "build_ms": st["build_ms"], "autotune": {"grid": st["grid32"], "longest_first": bool(st["lpt"]),
+                                                   "table": b.tune_table().tolist()},
+           "iterations": {"max": int(r["iterations"].max()), "mean": float(r["iterations"].mean())},
+           "max_err": float(r["err"].max()), "outer_steps": st["outer_steps"],
+           "dense_operator_GB": {"fp32": st["hsize"] * 8 / 1e9, "fp64": st["hsize"] * 16 / 1e9},
+           "qext_mean": float(q[:, :, 0, 0].mean()),
+           "roofline": {"kernel": "gemm32_kernel (grouped complex GEMM, v_mfma_f32_32x32x2f32)", "bound": "mfma",
+                        "achieved": round(tf, 2), "peak": 157.3, "unit": "TFLOP/s", "frac": round(tf / 157.3, 4),
+                        "traffic": None, "flops_per_launch": st["gemm32_flops"],
+                        "algorithmic_bytes": st["gemm32_bytes"], "avg_launch_ms": st["gemm32_ms"]},
+           "cpu_baseline": None})
+    b.close()
+
+
+def _batch_cluster(k):
+    from mstm_amd.batch import random_clusters  # noqa: F401
+    from mstm_amd import geometry as G
+
+    rng = np.random.default_rng(3)
+    ns = np.round(np.exp(rng.uniform(np.log(10), np.log(300), 256))).astype(int)
+    xs = rng.uniform(0.5, 3.0, 256)
+    fvs = rng.uniform(0.05, 0.30, 256)
+    return G.rsa_in_sphere(int(ns[k]), float(xs[k]), float(fvs[k]), seed=3 * 100003 + k, ri=1.33 + 0.001j)
+
+
+def bench_fft(args):
+    """FFT node-grid operator: FP32 translation/FFT inside FP64 refinement vs
+    the plain FP64 FFT solve, both checked against the FP64 direct solve."""
+    from mstm_amd import geometry as G
+    from mstm_amd.solver import FP32, FP64, SOLVE_MIXED, SOLVE_PLAIN, HipContext, qtotcalc
+
+    if args.workload == "fft-cylinder":
+        cl = G.rsa_in_cylinder(1000, 3.0, 0.25, seed=2, ri=1.6 + 0.02j)
+        dirs = [(0.0, 0.0)]
+        name = "1000 spheres x=3 m=1.6+0.02i RSA cylinder (length 2 diameters) fv=0.25 seed=2, FFT, +z, 2 pol"
+    else:
+        cl = G.fractal_aggregate(2000, 0.5, 1.8, 1.3, seed=4, ri=1.75 + 0.4j)
+        rng = np.random.default_rng(4)
+        dirs = [(float(np.arccos(1 - 2 * rng.random())), float(2 * np.pi * rng.random())) for _ in range(8)]
+        name = "2000-monomer aggregate Df=1.8 kf=1.3 x=0.5 m=1.75+0.4i seed=4, FFT, 8 orientations x 2 pol"
+    ctx = HipContext(0)
+    order, _ = ctx.set_spheres(cl.pos, cl.radius, cl.ri, 1.0, 1e-6)
+    grid = ctx.node_selection()
+    t0 = time.perf_counter()
+    ctx.setup_fft(FP64)
+    ctx.setup_fft(FP32)
+    t_setup = time.perf_counter() - t0
+    print(f"[bench] {cl.n} spheres, order {order.max()}, grid {grid}, FFT setup {t_setup:.2f} s", file=sys.stderr,
+          flush=True)
+
+    def solve_all(solver, fft):
+        t0 = time.perf_counter()
+        qs, its = [], []
+        for beta, alpha in dirs:
+            sd = 1 if np.cos(beta) >= 0 else 2
+            r = ctx.fixedorsoln(alpha=alpha, sinc=float(np.sin(beta)), eps=1e-6, niter=10000, solver=solver,
+                                fft_option=fft, direction=sd)
+            qs.append(qtotcalc(cl.radius, r["qeff"]))
+            its.append(r["iterations"])
+        return (time.perf_counter() - t0) / len(dirs), np.array(qs), its
+
+    for _ in range(args.warmup):
+        solve_all(SOLVE_MIXED, True)
+    times = []
+    for _ in range(args.steps):
+        t, q_mix, it_mix = solve_all(SOLVE_MIXED, True)
+        times.append(t)
+    t_mix = float(np.mean(times))
+    t_plain, q_plain, it_plain = solve_all(SOLVE_PLAIN, True)
+    ctx.setup_direct(FP64)
+    t_dir, q_dir, it_dir = solve_all(SOLVE_PLAIN, False)
+    rel = lambda a, b: float(np.abs(a - b).max() / np.abs(b).max())  # noqa: E731
+    _line("s/solve (both polarizations) to residual 1e-6, FFT operator", round(t_mix, 4), "s/solve", args.steps,
+          args.warmup, round(t_mix * 1e3, 2), "c64 FFT/translation + f64 refinement", name,
+          {"nsphere": cl.n, "order": int(order.max()), "node_grid": {k: (v.tolist() if hasattr(v, "tolist") else v)
+                                                                     for k, v in grid.items()},
+           "fft_setup_s": round(t_setup, 3), "iterations_mixed": it_mix,
+           "plain_fft_fp64": {"s_per_solve": round(t_plain, 4), "iterations": it_plain},
+           "direct_fp64": {"s_per_solve": round(t_dir, 4), "iterations": it_dir},
+           "speedup_vs_plain_fft_fp64": round(t_plain / t_mix, 3),
+           "q_error": {"mixed_fft_vs_plain_fft": rel(q_mix[:, 0, :2], q_plain[:, 0, :2]),
+                       "mixed_fft_vs_direct": rel(q_mix[:, 0, :2], q_dir[:, 0, :2]),
+                       "plain_fft_vs_direct": rel(q_plain[:, 0, :2], q_dir[:, 0, :2])},
+           "roofline": None, "cpu_baseline": None})
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +222,12 @@ def main():
     ap.add_argument("--no-tight", action="store_true", help="skip the tight FP64 reference solve")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many MiB (rocprof FETCH/WRITE calibration)")
+    ap.add_argument("--workload", default="direct5000",
+                    choices=["direct5000", "fft-cylinder", "fractal", "batch"],
+                    help="direct5000 (default, the headline line); the others print their own line: "
+                         "fft-cylinder = 1000 spheres x=3 m=1.6+0.02i in a cylinder at fv 0.25 (FFT path), "
+                         "fractal = 2000-monomer aggregate x=0.5 m=1.75+0.4i (FFT path, 8 orientations), "
+                         "batch = 256 clusters x 32 directions x 2 polarizations (batched MFMA solver)")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -106,6 +241,11 @@ def main():
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload in ("fft-cylinder", "fractal"):
+        return bench_fft(args)
 
     from mstm_amd import geometry as G
     from mstm_amd.solver import FP32, FP64, SOLVE_LEARNED, SOLVE_MIXED, SOLVE_PLAIN, HipContext, qtotcalc
@@ -163,9 +303,12 @@ def main():
         plain_s = time.perf_counter() - t0
         st_plain = ctx.stats()
         q_plain = qtotcalc(cl.radius, rp["qeff"])
+        n4 = st_plain.get("tile4_launches_fp64", 0)
         plain = dict(s_per_solve=plain_s, iterations_last_pol=rp["iterations"], maxerr=rp["maxerr"],
                      qext=q_plain[0, 0], qabs=q_plain[0, 1],
-                     tile_ms_avg=st_plain["tile_ms_fp64"] / max(1, st_plain["tile_launches_fp64"]),
+                     tile_columns=4 if n4 > 0 else 2,
+                     tile_ms_avg=(st_plain["tile4_ms_fp64"] / n4 if n4 > 0 else
+                                  st_plain["tile_ms_fp64"] / max(1, st_plain["tile_launches_fp64"])),
                      iters_fp64=st_plain["iters_fp64"])
         print(f"[bench] plain FP64 solve: {plain_s:.3f} s, iters {rp['iterations']}", file=sys.stderr, flush=True)
 
@@ -185,19 +328,35 @@ def main():
 
     # ---- roofline of the dominant kernel (FP32 tile kernel) ------------------
     npairs = cl.n * (cl.n - 1) // 2
-    vec_bytes = 2 * 2 * ne * 8
+    # both polarizations' BiCG products run as one four-column launch while
+    # both are iterating; the roofline is quoted on that launch when present
+    ncol = 4 if st_lig.get("tile4_launches_fp32", 0) > 0 else 2
+    vec_bytes = ncol * 2 * ne * 8
     alg32 = npairs * table_bytes_per_pair(L, 4) + vec_bytes
-    alg64 = npairs * table_bytes_per_pair(L, 8) + 2 * 2 * ne * 16
-    t32 = st_lig["tile_ms_fp32"] / max(1, st_lig["tile_launches_fp32"])
+    alg64 = npairs * table_bytes_per_pair(L, 8) + (plain["tile_columns"] if plain else 2) * 2 * ne * 16
+    if ncol == 4:
+        t32 = st_lig["tile4_ms_fp32"] / st_lig["tile4_launches_fp32"]
+    else:
+        t32 = st_lig["tile_ms_fp32"] / max(1, st_lig["tile_launches_fp32"])
     ach32 = alg32 / (t32 * 1e-3) / 1e9
     traffic = None
     pmc = os.path.join(HERE, "profiles", "tile_fp32_cfg5_pmc.json")
     if os.path.exists(pmc):
         traffic = json.load(open(pmc)).get("hbm_bytes_per_launch")
-    roof = dict(kernel="direct_tile_kernel<float,L>", bound="hbm", achieved=round(ach32, 1), peak=HBM_PEAK_GBPS,
-                unit="GB/s", frac=round(ach32 / HBM_PEAK_GBPS, 4), traffic=traffic, algorithmic_bytes=alg32,
-                avg_launch_ms=round(t32, 4), flops_per_launch=npairs * tile_flops_per_pair(L))
-    matvec = dict(fp32_tile_ms=round(t32, 4), fp32_hbm_GBps=round(ach32, 1))
+    roof = dict(kernel=f"direct_tile_kernel<float,{L},{ncol}>", bound="hbm", achieved=round(ach32, 1),
+                peak=HBM_PEAK_GBPS, unit="GB/s", frac=round(ach32 / HBM_PEAK_GBPS, 4), traffic=traffic,
+                algorithmic_bytes=alg32, columns=ncol, avg_launch_ms=round(t32, 4),
+                flops_per_launch=npairs * tile_flops_per_pair(L, ncol),
+                tflops=round(npairs * tile_flops_per_pair(L, ncol) / (t32 * 1e-3) / 1e12, 2),
+                flop_per_byte=round(npairs * tile_flops_per_pair(L, ncol) / alg32, 2),
+                note=("algorithmic intensity below the FP32 ridge (157.3 TF / 8 TB/s = 19.7 flop/B), so quoted "
+                      "against HBM; the measured limiter is VALU issue (packed FP32 is disabled on this pool, "
+                      "SQ counters in profiles/direct_tile_cfg5_sq_counters.txt)"))
+    matvec = dict(fp32_tile_ms=round(t32, 4), fp32_columns=ncol, fp32_hbm_GBps=round(ach32, 1))
+    if st_lig.get("tile_launches_fp32", 0) > 0:
+        t2 = st_lig["tile_ms_fp32"] / st_lig["tile_launches_fp32"]
+        a2 = npairs * table_bytes_per_pair(L, 4) + 2 * 2 * ne * 8
+        matvec.update(fp32_2col_tile_ms=round(t2, 4), fp32_2col_hbm_GBps=round(a2 / (t2 * 1e-3) / 1e9, 1))
     if plain is not None:
         t64 = plain["tile_ms_avg"]
         matvec.update(fp64_tile_ms=round(t64, 4), fp64_hbm_GBps=round(alg64 / (t64 * 1e-3) / 1e9, 1))

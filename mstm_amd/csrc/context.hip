@@ -30,8 +30,10 @@ struct DirectOp {
 
 struct Stats {
     double tables_ms[2] = {0, 0};
-    double tile_ms[2] = {0, 0};
+    double tile_ms[2] = {0, 0};  // two-column launches
     long tile_launches[2] = {0, 0};
+    double tile4_ms[2] = {0, 0};  // four-column launches (both polarizations)
+    long tile4_launches[2] = {0, 0};
     double matvec_ms = 0;
     long matvecs[2] = {0, 0};
     double solve_ms = 0;
@@ -53,6 +55,23 @@ struct LearnedNet {
     int cap = 0;
     float *feat = nullptr, *depth = nullptr, *out = nullptr;
     std::vector<float *> H;
+};
+
+// Krylov work-vector slots (complex FP64, neqns each).  Blocks that one
+// product reads or writes are contiguous.
+enum {
+    KV_PW = 0,    // 0-3   product inputs  [p_0, w_0, p_1, w_1] of the two BiCG systems
+    KV_CAPS = 4,  // 4-7   product outputs [A p_0, A^H w_0, A p_1, A^H w_1]
+    KV_RB = 8,    // 8-9   BiCG residuals r_0, r_1
+    KV_QB = 10,   // 10-11 BiCG shadow residuals q_0, q_1
+    KV_X = 12,    // 12-13 solutions (both polarizations)
+    KV_Y = 14,    // 14-15 refinement products
+    KV_R = 16,    // 16-17 refinement residuals
+    KV_D = 18,    // 18-19 refinement corrections
+    KV_B = 20,    // 20-21 right-hand sides b = T f
+    KV_PM = 22,   // 22-23 incident field f (pmnp)
+    KV_ST = 24,   // 24-27 host staging
+    KV_N = 28
 };
 
 }  // namespace
@@ -233,8 +252,8 @@ int mstm_hip_create(mstm_hip_ctx **ctx, int device)
     c->device = device;
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
-    HIPCHK(dalloc(&c->d_partials, sizeof(double) * 4 * 2 * krylov_grid()));
-    c->h_partials.resize(4 * 2 * krylov_grid());
+    HIPCHK(dalloc(&c->d_partials, sizeof(double) * 16 * krylov_grid()));  // 8 regions of one value per block
+    c->h_partials.resize(16 * krylov_grid());
     *ctx = c;
     return 0;
 }
@@ -286,10 +305,10 @@ static int finish_sphere_layout(mstm_hip_ctx *c)
     dfree(c->d_tiles);
     HIPCHK(dalloc(&c->d_tiles, sizeof(int2) * tiles.size()));
     HIPCHK(hipMemcpy(c->d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice));
-    // Krylov work space: 15 vectors (slot plan in mstm_hip_fixedorsoln)
+    // Krylov work space (slot plan KV_*)
     c->kv_n = (size_t)c->neqns;
     dfree(c->d_kv);
-    HIPCHK(dalloc(&c->d_kv, sizeof(double) * 2 * c->kv_n * 15));
+    HIPCHK(dalloc(&c->d_kv, sizeof(double) * 2 * c->kv_n * KV_N));
     return 0;
 }
 
@@ -400,8 +419,8 @@ int mstm_hip_setup_direct(mstm_hip_ctx *c, int prec)
     d.tab_bytes = (size_t)c->ntiles * MSTM_SLOTS * MSTM_SLOTS * slot * esz;
     size_t nbp = (size_t)L * (L + 2);
     HIPCHK(dalloc(&d.tab, d.tab_bytes));
-    HIPCHK(dalloc(&d.xin, esz * 2 * 2 * 2 * (size_t)c->npad * nbp));
-    HIPCHK(dalloc(&d.part, esz * 2 * (size_t)c->nblocks * 2 * 2 * c->npad * nbp));
+    HIPCHK(dalloc(&d.xin, esz * 2 * 4 * 2 * (size_t)c->npad * nbp));
+    HIPCHK(dalloc(&d.part, esz * 2 * (size_t)c->nblocks * 4 * 2 * c->npad * nbp));
     auto t0 = std::chrono::steady_clock::now();
     if (prec == MSTM_HIP_FP64)
         HIPCHK(launch_direct_tables<double>((double *)d.tab, c->d_tiles, c->ntiles, L, c->nsph, c->d_pos, c->rimed,
@@ -464,7 +483,8 @@ static hipEvent_t next_event(mstm_hip_ctx *c)
 }
 
 // Device-side sphereinteraction on the direct operator: ain/aout are
-// (neqns, nrhs) complex FP64 device arrays, nrhs <= 2.
+// (neqns, nrhs) complex FP64 device arrays, nrhs <= 4; 3-4 columns run as
+// one four-column tile launch.
 static int direct_matvec(mstm_hip_ctx *c, int nrhs, const double *ain, double *aout, const int *flags, int prec)
 {
     DirectOp &d = c->dop[prec];
@@ -472,7 +492,8 @@ static int direct_matvec(mstm_hip_ctx *c, int nrhs, const double *ain, double *a
         int rc = mstm_hip_setup_direct(c, prec);
         if (rc) return rc;
     }
-    const int L = c->Lmax;
+    if (nrhs < 1 || nrhs > 4) return fail(c, MSTM_HIP_EARG, "direct product: %d columns", nrhs);
+    const int L = c->Lmax, nr = nrhs > 2 ? 4 : 2;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {
         e0 = next_event(c);
@@ -483,7 +504,7 @@ static int direct_matvec(mstm_hip_ctx *c, int nrhs, const double *ain, double *a
                                    c->d_offset, c->d_an, c->st));
         if (e0) hipEventRecord(e0, c->st);
         HIPCHK(launch_direct_tiles<double>((const double *)d.tab, c->d_tiles, c->ntiles, L, (const double *)d.xin,
-                                           (double *)d.part, c->npad, c->st));
+                                           (double *)d.part, c->npad, nr, c->st));
         if (e1) hipEventRecord(e1, c->st);
         HIPCHK(launch_reduce_post<double>((const double *)d.part, c->nblocks, aout, 2 * c->neqns, nrhs, flags, c->nsph,
                                           c->npad, L, c->d_order, c->d_offset, c->d_an, c->st));
@@ -492,12 +513,12 @@ static int direct_matvec(mstm_hip_ctx *c, int nrhs, const double *ain, double *a
                                   c->d_offset, c->d_an, c->st));
         if (e0) hipEventRecord(e0, c->st);
         HIPCHK(launch_direct_tiles<float>((const float *)d.tab, c->d_tiles, c->ntiles, L, (const float *)d.xin,
-                                          (float *)d.part, c->npad, c->st));
+                                          (float *)d.part, c->npad, nr, c->st));
         if (e1) hipEventRecord(e1, c->st);
         HIPCHK(launch_reduce_post<float>((const float *)d.part, c->nblocks, aout, 2 * c->neqns, nrhs, flags, c->nsph,
                                          c->npad, L, c->d_order, c->d_offset, c->d_an, c->st));
     }
-    if (e0) c->ev_prec.push_back(prec);
+    if (e0) c->ev_prec.push_back(prec + 2 * (nr == 4));
     c->stats.matvecs[prec]++;
     return 0;
 }
@@ -506,9 +527,13 @@ static int matvec(mstm_hip_ctx *c, int nrhs, const double *ain, double *aout, co
 {
     if (use_fft) {
         if (!c->fft) return fail(c, MSTM_HIP_ESTATE, "FFT operator not set up (call node_selection + setup_fft)");
-        int rc = fft_matvec(c->fft, nrhs, ain, aout, flags, prec, c->st);
-        if (rc) return fail(c, rc, "fft matvec: %s", fft_last_error(c->fft));
-        c->stats.matvecs[prec]++;
+        const size_t V = 2 * (size_t)c->neqns;
+        for (int r0 = 0; r0 < nrhs; r0 += 2) {  // the node-grid operator takes two columns per call
+            const int nr = std::min(2, nrhs - r0);
+            int rc = fft_matvec(c->fft, nr, ain + V * r0, aout + V * r0, flags + r0, prec, c->st);
+            if (rc) return fail(c, rc, "fft matvec: %s", fft_last_error(c->fft));
+            c->stats.matvecs[prec]++;
+        }
         return 0;
     }
     return direct_matvec(c, nrhs, ain, aout, flags, prec);
@@ -520,32 +545,52 @@ static void collect_events(mstm_hip_ctx *c)
     for (size_t q = 0; q + 1 < c->ev_used && q / 2 < c->ev_prec.size(); q += 2) {
         float ms = 0;
         hipEventElapsedTime(&ms, c->ev[q], c->ev[q + 1]);
-        int p = c->ev_prec[q / 2];
-        c->stats.tile_ms[p] += ms;
-        c->stats.tile_launches[p]++;
+        const int p = c->ev_prec[q / 2] & 1;
+        if (c->ev_prec[q / 2] & 2) {
+            c->stats.tile4_ms[p] += ms;
+            c->stats.tile4_launches[p]++;
+        } else {
+            c->stats.tile_ms[p] += ms;
+            c->stats.tile_launches[p]++;
+        }
     }
     c->ev_used = 0;
     c->ev_prec.clear();
 }
 
-// sum of per-block complex partials written by the Krylov kernels
-static int sum_partials(mstm_hip_ctx *c, int nvals, double *out)
+// Krylov partial sums: each kernel launch writes one complex value per
+// block of its grid (krylov_grid()) into a region of d_partials; regions are
+// fetched together and added on the host in a fixed order (deterministic).
+static int fetch_partials(mstm_hip_ctx *c, int ncomplex)
 {
-    int g = krylov_grid();
-    HIPCHK(hipMemcpyAsync(c->h_partials.data(), c->d_partials, sizeof(double) * 2 * nvals * g, hipMemcpyDeviceToHost,
+    HIPCHK(hipMemcpyAsync(c->h_partials.data(), c->d_partials, sizeof(double) * 2 * ncomplex, hipMemcpyDeviceToHost,
                           c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+    return 0;
+}
+// sum of nvals interleaved values of the region starting at complex offset off
+static void reduce_region(const mstm_hip_ctx *c, int off, int nvals, double *out)
+{
+    const int g = krylov_grid();
     for (int v = 0; v < nvals; v++) {
         double re = 0, im = 0;
         for (int b = 0; b < g; b++) {
-            re += c->h_partials[2 * (nvals * b + v)];
-            im += c->h_partials[2 * (nvals * b + v) + 1];
+            re += c->h_partials[2 * (off + nvals * b + v)];
+            im += c->h_partials[2 * (off + nvals * b + v) + 1];
         }
         out[2 * v] = re;
         out[2 * v + 1] = im;
     }
+}
+static int sum_partials(mstm_hip_ctx *c, int nvals, double *out)
+{
+    int rc = fetch_partials(c, nvals * krylov_grid());
+    if (rc) return rc;
+    reduce_region(c, 0, nvals, out);
     return 0;
 }
+static inline double *pregion(mstm_hip_ctx *c, int s) { return c->d_partials + (size_t)4 * krylov_grid() * s; }
+static inline int pregion_off(int s) { return 2 * krylov_grid() * s; }
 
 static inline void zdiv(double ar, double ai, double br, double bi, double &cr, double &ci)
 {
@@ -554,112 +599,199 @@ static inline void zdiv(double ar, double ai, double br, double bi, double &cr, 
     ci = (ai * br - ar * bi) / d;
 }
 
-// cbicg on device vectors: pnp (b), anp (x in/out); returns iterations as
-// the Fortran do-variable (niter+1 if the loop ran out).  Work vectors come
-// from the context pool starting at slot ws.
-static int bicg_dev(mstm_hip_ctx *c, int niter, double eps, const double *pnp, double *anp, int prec, int use_fft,
-                    int ws, int *iter_out, double *errmax)
+// cbicg (mstm-solver-8.f90:654-949) for ns = 1 or 2 independent systems
+// A x_s = b_s at once (the two polarizations of fixedorsoln).  b and x are
+// blocks of ns contiguous vectors; every system keeps the reference's own
+// recurrences and scalars, so each one runs exactly the iterations it would
+// run alone, while both share every table stream of the product (one
+// four-column launch while both are active).  it[s]: iterations as the
+// Fortran do-variable (niter+1 if the loop ran out); err[s]: last |r|^2/|b|^2.
+static int bicg2_dev(mstm_hip_ctx *c, int ns, int niter, const double *eps, const double *b, double *x, int prec,
+                     int use_fft, int *it, double *err)
 {
     const int n = c->neqns;
     const size_t V = 2 * c->kv_n;
-    double *cr = c->d_kv + V * ws, *cq = cr + V, *cpw = cq + V, *caps = cpw + 2 * V;
-    double *cp = cpw, *cw = cpw + V, *cap = caps, *caw = caps + V;
-    *errmax = 0.0;
-    *iter_out = 0;
-    double bb[2];
-    HIPCHK(launch_zdot(pnp, pnp, n, 1, c->d_partials, nullptr, c->st));
-    int rc = sum_partials(c, 1, bb);
+    double *kv = c->d_kv;
+    double *P[2], *W[2], *AP[2], *AW[2], *Rr[2], *Q[2];
+    const double *B[2];
+    double *X[2];
+    for (int s = 0; s < ns; s++) {
+        P[s] = kv + V * (KV_PW + 2 * s);
+        W[s] = P[s] + V;
+        AP[s] = kv + V * (KV_CAPS + 2 * s);
+        AW[s] = AP[s] + V;
+        Rr[s] = kv + V * (KV_RB + s);
+        Q[s] = kv + V * (KV_QB + s);
+        B[s] = b + V * s;
+        X[s] = x + V * s;
+        it[s] = 0;
+        err[s] = 0.0;
+    }
+    double enorm[2] = {0, 0}, csk[2][2];
+    bool act[2] = {false, false};
+    // |b|^2
+    for (int s = 0; s < ns; s++) HIPCHK(launch_zdot(B[s], B[s], n, 1, pregion(c, s), nullptr, c->st));
+    int rc = fetch_partials(c, 2 * krylov_grid() * ns);
     if (rc) return rc;
-    if (bb[0] == 0.0) return 0;
-    const double enorm = bb[0];
-    int f1[2] = {2, 0};
-    rc = matvec(c, 1, anp, cr, f1, prec, use_fft);
+    for (int s = 0; s < ns; s++) {
+        double v[2];
+        reduce_region(c, pregion_off(s), 1, v);
+        enorm[s] = v[0];
+        act[s] = v[0] != 0.0;
+    }
+    if (!act[0] && (ns < 2 || !act[1])) return 0;
+    // r = b - A x (:827-834), one product for the active systems' x
+    {
+        int f[2] = {2, 2};
+        if (ns == 2 && act[0] && act[1]) rc = matvec(c, 2, X[0], Rr[0], f, prec, use_fft);
+        else
+            for (int s = 0; s < ns && !rc; s++)
+                if (act[s]) rc = matvec(c, 1, X[s], Rr[s], f, prec, use_fft);
+        if (rc) return rc;
+    }
+    for (int s = 0; s < ns; s++)
+        if (act[s]) HIPCHK(launch_bicg_init(Rr[s], Q[s], W[s], P[s], B[s], X[s], n, pregion(c, s), c->st));
+    rc = fetch_partials(c, 2 * krylov_grid() * ns);
     if (rc) return rc;
-    HIPCHK(launch_bicg_init(cr, cq, cw, cp, pnp, anp, n, c->d_partials, c->st));
-    double csk[2];
-    rc = sum_partials(c, 1, csk);
-    if (rc) return rc;
-    if (csk[0] == 0.0 && csk[1] == 0.0) return 0;
-    int f2[2] = {2, 3};
+    for (int s = 0; s < ns; s++) {
+        if (!act[s]) continue;
+        reduce_region(c, pregion_off(s), 1, csk[s]);
+        if (csk[s][0] == 0.0 && csk[s][1] == 0.0) act[s] = false;  // it stays 0
+    }
     int iter;
     for (iter = 1; iter <= niter; iter++) {
-        rc = matvec(c, 2, cpw, caps, f2, prec, use_fft);
+        const bool both = ns == 2 && act[0] && act[1];
+        if (!act[0] && !(ns == 2 && act[1])) break;
+        int f4[4] = {2, 3, 2, 3};
+        if (both) rc = matvec(c, 4, P[0], AP[0], f4, prec, use_fft);
+        else rc = matvec(c, 2, act[0] ? P[0] : P[1], act[0] ? AP[0] : AP[1], f4, prec, use_fft);
         if (rc) return rc;
-        HIPCHK(launch_bicg_a(cap, caw, cp, cw, n, c->d_partials, c->st));
-        double den[2];
-        rc = sum_partials(c, 1, den);
+        for (int s = 0; s < ns; s++)
+            if (act[s]) HIPCHK(launch_bicg_a(AP[s], AW[s], P[s], W[s], n, pregion(c, s), c->st));
+        rc = fetch_partials(c, 2 * krylov_grid() * ns);
         if (rc) return rc;
-        if (den[0] == 0.0 && den[1] == 0.0) break;
-        double cak[2];
-        zdiv(csk[0], csk[1], den[0], den[1], cak[0], cak[1]);
-        HIPCHK(launch_bicg_b(anp, cr, cq, cp, cap, caw, cak[0], cak[1], n, c->d_partials, c->st));
-        double d2[4];
-        rc = sum_partials(c, 2, d2);
+        bool step[2] = {false, false};
+        for (int s = 0; s < ns; s++) {
+            if (!act[s]) continue;
+            double den[2];
+            reduce_region(c, pregion_off(s), 1, den);
+            if (den[0] == 0.0 && den[1] == 0.0) {  // breakdown: the Fortran loop exits here
+                act[s] = false;
+                it[s] = iter;
+                continue;
+            }
+            double cak[2];
+            zdiv(csk[s][0], csk[s][1], den[0], den[1], cak[0], cak[1]);
+            HIPCHK(launch_bicg_b(X[s], Rr[s], Q[s], P[s], AP[s], AW[s], cak[0], cak[1], n, pregion(c, s), c->st));
+            step[s] = true;
+        }
+        if (!step[0] && !(ns == 2 && step[1])) continue;
+        rc = fetch_partials(c, 2 * krylov_grid() * ns);
         if (rc) return rc;
-        double eerr = d2[2] / enorm;
-        *errmax = eerr;
-        if (prec == MSTM_HIP_FP64) c->stats.iters_fp64++;
-        else c->stats.iters_fp32++;
-        if (eerr < eps || (csk[0] == 0.0 && csk[1] == 0.0)) break;
-        double cbk[2];
-        zdiv(d2[0], d2[1], csk[0], csk[1], cbk[0], cbk[1]);
-        csk[0] = d2[0];
-        csk[1] = d2[1];
-        HIPCHK(launch_bicg_c(cp, cw, cr, cq, cbk[0], cbk[1], n, c->st));
+        for (int s = 0; s < ns; s++) {
+            if (!step[s]) continue;
+            double d2[4];
+            reduce_region(c, pregion_off(s), 2, d2);
+            err[s] = d2[2] / enorm[s];
+            if (prec == MSTM_HIP_FP64) c->stats.iters_fp64++;
+            else c->stats.iters_fp32++;
+            if (err[s] < eps[s] || (csk[s][0] == 0.0 && csk[s][1] == 0.0)) {
+                act[s] = false;
+                it[s] = iter;
+                continue;
+            }
+            double cbk[2];
+            zdiv(d2[0], d2[1], csk[s][0], csk[s][1], cbk[0], cbk[1]);
+            csk[s][0] = d2[0];
+            csk[s][1] = d2[1];
+            HIPCHK(launch_bicg_c(P[s], W[s], Rr[s], Q[s], cbk[0], cbk[1], n, c->st));
+        }
     }
-    *iter_out = iter;
+    for (int s = 0; s < ns; s++)
+        if (act[s]) it[s] = iter;  // ran out: niter + 1
     return 0;
 }
 
-// Mixed precision: x <- x + d with A d = r solved by FP32 BiCG, r = b - A x
-// evaluated with the FP64 operator; stops on the reference criterion
-// |r|^2/|b|^2 < eps measured in FP64.
-// r_in (optional) is b - A x at the starting x, already evaluated in FP64.
-static int refine_dev(mstm_hip_ctx *c, int niter, double eps, const double *pnp, double *anp, int use_fft, int *iters,
-                      double *errmax, const double *r_in = nullptr)
+// Mixed precision for ns = 1 or 2 systems: x_s <- x_s + d_s with A d_s = r_s
+// solved by FP32 BiCG, r_s = b_s - A x_s evaluated with the FP64 operator;
+// a system stops on the reference criterion |r|^2/|b|^2 < eps measured in
+// FP64.  r_in (optional): the residual block of the starting x, already
+// evaluated in FP64.  iters[s] counts inner FP32 iterations.
+static int refine2_dev(mstm_hip_ctx *c, int ns, int niter, double eps, const double *b, double *x, int use_fft,
+                       int *iters, double *errmax, const double *r_in = nullptr)
 {
     const int n = c->neqns;
     const size_t V = 2 * c->kv_n;
-    double *y = c->d_kv + V * 6, *r = y + V, *d = r + V;  // slots 6..8; bicg_dev uses ws=0 (slots 0..5)
-    double bb[2];
-    HIPCHK(launch_zdot(pnp, pnp, n, 1, c->d_partials, nullptr, c->st));
-    int rc = sum_partials(c, 1, bb);
+    double *Y = c->d_kv + V * KV_Y, *R = c->d_kv + V * KV_R, *D = c->d_kv + V * KV_D;
+    double bb[2] = {0, 0};
+    bool open[2] = {false, false};
+    for (int s = 0; s < ns; s++) HIPCHK(launch_zdot(b + V * s, b + V * s, n, 1, pregion(c, s), nullptr, c->st));
+    int rc = fetch_partials(c, 2 * krylov_grid() * ns);
     if (rc) return rc;
-    *iters = 0;
-    *errmax = 0;
-    if (bb[0] == 0.0) return 0;
-    int f1[2] = {2, 0};
+    for (int s = 0; s < ns; s++) {
+        double v[2];
+        reduce_region(c, pregion_off(s), 1, v);
+        bb[s] = v[0];
+        open[s] = bb[s] != 0.0;
+        iters[s] = 0;
+        errmax[s] = 0.0;
+    }
+    int f2[2] = {2, 2};
     const int max_ref = 6;
     for (int ref = 0; ref <= max_ref; ref++) {
+        if (!open[0] && !(ns == 2 && open[1])) break;
         if (ref == 0 && r_in) {
-            HIPCHK(hipMemcpyAsync(r, r_in, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, c->st));
+            for (int s = 0; s < ns; s++)
+                if (open[s] && r_in + V * s != R + V * s)
+                    HIPCHK(hipMemcpyAsync(R + V * s, r_in + V * s, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
         } else {
-            rc = matvec(c, 1, anp, y, f1, MSTM_HIP_FP64, use_fft);  // y = T H x
+            if (ns == 2 && open[0] && open[1]) rc = matvec(c, 2, x, Y, f2, MSTM_HIP_FP64, use_fft);  // y = T H x
+            else
+                for (int s = 0; s < ns && !rc; s++)
+                    if (open[s]) rc = matvec(c, 1, x + V * s, Y + V * s, f2, MSTM_HIP_FP64, use_fft);
             if (rc) return rc;
-            // r = b - x + y
-            HIPCHK(hipMemcpyAsync(r, pnp, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, c->st));
-            HIPCHK(launch_zaxpby(-1.0, 0.0, anp, 1.0, 0.0, r, n, c->st));
-            HIPCHK(launch_zaxpby(1.0, 0.0, y, 1.0, 0.0, r, n, c->st));
+            for (int s = 0; s < ns; s++) {
+                if (!open[s]) continue;  // r = b - x + y
+                HIPCHK(hipMemcpyAsync(R + V * s, b + V * s, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+                HIPCHK(launch_zaxpby(-1.0, 0.0, x + V * s, 1.0, 0.0, R + V * s, n, c->st));
+                HIPCHK(launch_zaxpby(1.0, 0.0, Y + V * s, 1.0, 0.0, R + V * s, n, c->st));
+            }
         }
-        double rr[2];
-        HIPCHK(launch_zdot(r, r, n, 1, c->d_partials, nullptr, c->st));
-        rc = sum_partials(c, 1, rr);
+        for (int s = 0; s < ns; s++)
+            if (open[s]) HIPCHK(launch_zdot(R + V * s, R + V * s, n, 1, pregion(c, s), nullptr, c->st));
+        rc = fetch_partials(c, 2 * krylov_grid() * ns);
         if (rc) return rc;
-        double err = rr[0] / bb[0];
-        *errmax = err;
-        if (err < eps || ref == max_ref || *iters > niter) break;
+        double rr[2] = {0, 0}, eps_in[2] = {1, 1};
+        int nin = 0, sidx[2];
+        for (int s = 0; s < ns; s++) {
+            if (!open[s]) continue;
+            double v[2];
+            reduce_region(c, pregion_off(s), 1, v);
+            rr[s] = v[0];
+            errmax[s] = rr[s] / bb[s];
+            if (errmax[s] < eps || ref == max_ref || iters[s] > niter) {
+                open[s] = false;
+                continue;
+            }
+            // inner FP32 solve of A d = r from d = r, to half the outer target:
+            // the FP32 operator's relative error (~1e-7) is far below the 1e-3
+            // relative residual asked for, so one refinement step normally ends
+            eps_in[nin] = std::max(0.5 * eps * bb[s] / rr[s], 1e-12);
+            sidx[nin++] = s;
+            HIPCHK(hipMemcpyAsync(D + V * s, R + V * s, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+        }
+        if (!nin) continue;
         c->stats.refinements++;
-        // inner FP32 solve of A d = r from d = r, to half the outer target:
-        // the FP32 operator's relative error (~1e-7) is far below the 1e-3
-        // relative residual asked for, so one refinement step normally ends
-        HIPCHK(hipMemcpyAsync(d, r, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, c->st));
-        double eps_in = std::max(0.5 * eps * bb[0] / rr[0], 1e-12);
-        int it = 0;
-        double ein = 0;
-        rc = bicg_dev(c, niter - *iters, eps_in, r, d, MSTM_HIP_FP32, use_fft, 0, &it, &ein);
+        int it_in[2] = {0, 0};
+        double e_in[2] = {0, 0};
+        const int budget = std::max(1, niter - std::max(iters[sidx[0]], nin > 1 ? iters[sidx[1]] : 0));
+        rc = bicg2_dev(c, nin, budget, eps_in, R + V * sidx[0], D + V * sidx[0], MSTM_HIP_FP32, use_fft, it_in, e_in);
         if (rc) return rc;
-        *iters += std::min(it, niter - *iters);
-        HIPCHK(launch_zaxpby(1.0, 0.0, d, 1.0, 0.0, anp, n, c->st));
+        for (int q = 0; q < nin; q++) {
+            const int s = sidx[q];
+            iters[s] += std::min(it_in[q], std::max(0, niter - iters[s]));
+            HIPCHK(launch_zaxpby(1.0, 0.0, D + V * s, 1.0, 0.0, x + V * s, n, c->st));
+        }
     }
     return 0;
 }
@@ -683,42 +815,51 @@ static int lig_predict(mstm_hip_ctx *c)
     return 0;
 }
 
-// Starting vector of the learned solver for polarization p: the network's
-// prediction xl and the default b are both evaluated in FP64 (one two-column
-// product) and the one with the smaller residual is kept in x, its residual
-// in r.  Returns through *used whether the prediction was kept.
-static int lig_start(mstm_hip_ctx *c, int p, const double *b, double *x, double *r, int use_fft, bool *used)
+// Starting vectors of the learned solver for both polarizations: the
+// network's prediction xl_p and the default b_p are evaluated in FP64 (one
+// four-column product) and, per polarization, the one with the smaller
+// residual goes to x_p and its residual to r_p.  used[p]: prediction kept.
+static int lig_start2(mstm_hip_ctx *c, const double *b, double *x, double *r, int use_fft, bool *used)
 {
     const int n = c->neqns;
     const size_t V = 2 * c->kv_n;
-    double *xl = c->d_kv + V * 6, *xb = xl + V, *y = c->d_kv, *rl = c->d_kv + V * 2, *rb = rl + V;
-    HIPCHK(launch_lig_combine(c->nsph, p, c->d_order, c->d_offset, c->d_pos, c->rimed, c->lig.out, c->lig.Lnet,
-                              c->lig.depth, c->lig.params + c->lig.np - 2, b, xl, c->st));
-    HIPCHK(hipMemcpyAsync(xb, b, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
-    int f2[2] = {2, 2};
-    int rc = matvec(c, 2, xl, y, f2, MSTM_HIP_FP64, use_fft);
-    if (rc) return rc;
-    double e[2];
-    for (int q = 0; q < 2; q++) {
-        double *rq = q ? rb : rl, d2[2];
-        HIPCHK(hipMemcpyAsync(rq, b, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
-        HIPCHK(launch_zaxpby(-1.0, 0.0, q ? xb : xl, 1.0, 0.0, rq, n, c->st));
-        HIPCHK(launch_zaxpby(1.0, 0.0, y + V * q, 1.0, 0.0, rq, n, c->st));
-        HIPCHK(launch_zdot(rq, rq, n, 1, c->d_partials, nullptr, c->st));
-        rc = sum_partials(c, 1, d2);
-        if (rc) return rc;
-        e[q] = d2[0];
+    double *in = c->d_kv + V * KV_PW, *y = c->d_kv + V * KV_CAPS;  // [xl_0, xb_0, xl_1, xb_1] -> products
+    double *rt = c->d_kv + V * KV_ST;                                // 4 residuals
+    for (int p = 0; p < 2; p++) {
+        HIPCHK(launch_lig_combine(c->nsph, p, c->d_order, c->d_offset, c->d_pos, c->rimed, c->lig.out, c->lig.Lnet,
+                                  c->lig.depth, c->lig.params + c->lig.np - 2, b + V * p, in + V * 2 * p, c->st));
+        HIPCHK(hipMemcpyAsync(in + V * (2 * p + 1), b + V * p, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
     }
-    double bb[2];
-    HIPCHK(launch_zdot(b, b, n, 1, c->d_partials, nullptr, c->st));
-    rc = sum_partials(c, 1, bb);
+    int f4[4] = {2, 2, 2, 2};
+    int rc = matvec(c, 4, in, y, f4, MSTM_HIP_FP64, use_fft);
     if (rc) return rc;
-    const double bn = bb[0] > 0.0 ? bb[0] : 1.0;
-    c->stats.lig_err0 = e[0] / bn;
-    c->stats.lig_errb = e[1] / bn;
-    *used = e[0] <= e[1];
-    HIPCHK(hipMemcpyAsync(x, *used ? xl : xb, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(r, *used ? rl : rb, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+    for (int q = 0; q < 4; q++) {
+        double *rq = rt + V * q;
+        HIPCHK(hipMemcpyAsync(rq, b + V * (q / 2), sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(launch_zaxpby(-1.0, 0.0, in + V * q, 1.0, 0.0, rq, n, c->st));
+        HIPCHK(launch_zaxpby(1.0, 0.0, y + V * q, 1.0, 0.0, rq, n, c->st));
+        HIPCHK(launch_zdot(rq, rq, n, 1, c->d_partials + (size_t)2 * krylov_grid() * q, nullptr, c->st));
+    }
+    for (int p = 0; p < 2; p++)
+        HIPCHK(launch_zdot(b + V * p, b + V * p, n, 1, c->d_partials + (size_t)2 * krylov_grid() * (4 + p), nullptr,
+                           c->st));
+    rc = fetch_partials(c, 6 * krylov_grid());
+    if (rc) return rc;
+    double e[6];
+    for (int q = 0; q < 6; q++) {
+        double v[2];
+        reduce_region(c, krylov_grid() * q, 1, v);
+        e[q] = v[0];
+    }
+    for (int p = 0; p < 2; p++) {
+        const double bn = e[4 + p] > 0.0 ? e[4 + p] : 1.0;
+        c->stats.lig_err0 = e[2 * p] / bn;  // the second polarization's values remain, as maxerr does
+        c->stats.lig_errb = e[2 * p + 1] / bn;
+        used[p] = e[2 * p] <= e[2 * p + 1];
+        const int k = used[p] ? 2 * p : 2 * p + 1;
+        HIPCHK(hipMemcpyAsync(x + V * p, in + V * k, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(r + V * p, rt + V * k, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+    }
     return 0;
 }
 
@@ -738,6 +879,7 @@ static int zdot_host(mstm_hip_ctx *c, const double *a, const double *b, std::com
     out = std::complex<double>(v[0], v[1]);
     return rc;
 }
+
 
 // Flexible GMRES(m) (Saad, SIAM J. Sci. Comput. 14, 461 (1993)) in FP64 on
 // A = I - T H from x = 0, right-preconditioned with a different map for every
@@ -762,7 +904,7 @@ static int fgmres_dev(mstm_hip_ctx *c, int niter, double eps, const double *b, d
     }
     auto Vv = [&](int i) { return c->d_fg + V * i; };
     auto Zv = [&](int i) { return c->d_fg + V * (m + 1 + i); };
-    double *w = c->d_kv + V * 6, *y = w + V, *r = y + V;
+    double *w = c->d_kv + V * KV_D, *y = c->d_kv + V * KV_Y, *r = c->d_kv + V * KV_R;
     zc bbz;
     int rc = zdot_host(c, b, b, bbz);
     if (rc) return rc;
@@ -810,8 +952,8 @@ static int fgmres_dev(mstm_hip_ctx *c, int niter, double eps, const double *b, d
                 const double eps_in = std::max(0.5 * eps * bb / rho, 1e-12);
                 int it = 0;
                 double ein = 0.0;
-                rc = bicg_dev(c, std::max(1, niter - *iters), eps_in, Vv(j), Zv(j), MSTM_HIP_FP32, use_fft, 0, &it,
-                              &ein);
+                rc = bicg2_dev(c, 1, std::max(1, niter - *iters), &eps_in, Vv(j), Zv(j), MSTM_HIP_FP32, use_fft, &it,
+                               &ein);
                 if (rc) return rc;
                 *iters += std::min(it, std::max(1, niter - *iters));
             }
@@ -878,10 +1020,10 @@ int mstm_hip_sphereinteraction(mstm_hip_ctx *c, int nrhs, const double *ain, dou
     if (nrhs < 1 || !ain || !aout || !con_tran || !mie_mult) return fail(c, MSTM_HIP_EARG, "sphereinteraction: args");
     HIPCHK(hipSetDevice(c->device));
     const size_t V = 2 * (size_t)c->neqns;
-    double *din = c->d_kv + V * 13, *dout = c->d_kv;  // staging slots 13-14 in, 0-1 out
-    for (int r0 = 0; r0 < nrhs; r0 += 2) {
-        int nr = std::min(2, nrhs - r0);
-        int flags[2] = {0, 0};
+    double *din = c->d_kv + V * KV_ST, *dout = c->d_kv + V * KV_PW;  // up to four columns per product
+    for (int r0 = 0; r0 < nrhs; r0 += 4) {
+        int nr = std::min(4, nrhs - r0);
+        int flags[4] = {0, 0, 0, 0};
         for (int q = 0; q < nr; q++) flags[q] = (con_tran[r0 + q] ? 1 : 0) | (mie_mult[r0 + q] ? 2 : 0);
         HIPCHK(hipMemcpyAsync(din, ain + V * r0, sizeof(double) * V * nr, hipMemcpyHostToDevice, c->st));
         int rc = matvec(c, nr, din, dout, flags, prec, use_fft);
@@ -900,10 +1042,10 @@ int mstm_hip_cbicg(mstm_hip_ctx *c, int niter, double eps, const double *pnp, do
     if (niter <= 0) return fail(c, MSTM_HIP_EUNSUP, "cbicg: niter <= 0 (order-of-scattering) not supported");
     HIPCHK(hipSetDevice(c->device));
     const size_t V = 2 * (size_t)c->neqns;
-    double *db = c->d_kv + V * 12, *dx = c->d_kv + V * 11;
+    double *db = c->d_kv + V * KV_B, *dx = c->d_kv + V * KV_X;
     HIPCHK(hipMemcpyAsync(db, pnp, sizeof(double) * V, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(dx, anp, sizeof(double) * V, hipMemcpyHostToDevice, c->st));
-    int rc = bicg_dev(c, niter, eps, db, dx, MSTM_HIP_FP64, use_fft, 0, iter, errmax);
+    int rc = bicg2_dev(c, 1, niter, &eps, db, dx, MSTM_HIP_FP64, use_fft, iter, errmax);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(anp, dx, sizeof(double) * V, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
@@ -926,8 +1068,7 @@ int mstm_hip_fixedorsoln(mstm_hip_ctx *c, double alpha, double sinc, int dir, do
     HIPCHK(hipSetDevice(c->device));
     const int n = c->neqns;
     const size_t V = 2 * (size_t)n;
-    // work-vector slots: 0-5 BiCG, 6-8 refinement, 9-10 pmnp, 11 x, 12 b, 13-14 staging
-    double *pmnp = c->d_kv + V * 9, *x = c->d_kv + V * 11, *b = c->d_kv + V * 12, *stage = c->d_kv + V * 13;
+    double *pmnp = c->d_kv + V * KV_PM, *x = c->d_kv + V * KV_X, *b = c->d_kv + V * KV_B, *r = c->d_kv + V * KV_R;
     auto t0 = std::chrono::steady_clock::now();
     HIPCHK(launch_planewave(pmnp, 2 * n, c->nsph, c->d_pos, c->d_order, c->d_offset, alpha, sinc, dir, c->rimed, c->st));
     // the network is trained for incidence along +z (beta = 0)
@@ -940,65 +1081,65 @@ int mstm_hip_fixedorsoln(mstm_hip_ctx *c, double alpha, double sinc, int dir, do
         HIPCHK(hipStreamSynchronize(c->st));
         c->stats.lig_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tl).count();
     }
-    for (int p = 0; p < 2; p++) {
-        // istat, maxiter, maxerr are reset per polarization (:471-475)
-        *istat = 0;
-        *maxerr = 0.0;
-        *maxiter = 0;
-        HIPCHK(launch_mie_mult(pmnp + V * p, b, c->nsph, c->d_order, c->d_offset, c->d_mieoff, c->d_an, 1, 0, c->Lmax,
-                               c->st));
-        HIPCHK(hipMemcpyAsync(x, b, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
-        int it = 0, rc;
-        double err = 0;
-        if (solver == MSTM_HIP_SOLVE_PLAIN) {
-            rc = bicg_dev(c, niter, eps, b, x, MSTM_HIP_FP64, use_fft, 0, &it, &err);
-        } else if (solver == MSTM_HIP_SOLVE_MIXED) {
-            rc = refine_dev(c, niter, eps, b, x, use_fft, &it, &err);
-        } else if (solver == MSTM_HIP_SOLVE_FGMRES) {
-            rc = fgmres_dev(c, niter, eps, b, x, use_fft, lig_on, p, &it, &err);
+    // b_p = T f_p and x_p = b_p (:471-483) for both polarizations; the two
+    // systems are solved together (each with its own scalars)
+    for (int p = 0; p < 2; p++)
+        HIPCHK(launch_mie_mult(pmnp + V * p, b + V * p, c->nsph, c->d_order, c->d_offset, c->d_mieoff, c->d_an, 1, 0,
+                               c->Lmax, c->st));
+    HIPCHK(hipMemcpyAsync(x, b, sizeof(double) * 2 * V, hipMemcpyDeviceToDevice, c->st));
+    int it[2] = {0, 0}, rc = 0;
+    double err[2] = {0, 0};
+    const double eps2[2] = {eps, eps};
+    if (solver == MSTM_HIP_SOLVE_PLAIN) {
+        rc = bicg2_dev(c, 2, niter, eps2, b, x, MSTM_HIP_FP64, use_fft, it, err);
+    } else if (solver == MSTM_HIP_SOLVE_MIXED) {
+        rc = refine2_dev(c, 2, niter, eps, b, x, use_fft, it, err);
+    } else if (solver == MSTM_HIP_SOLVE_LEARNED) {
+        // learned start (or b where the prediction is worse), mixed refinement
+        // from it; the plain FP64 solver from b takes over for a polarization
+        // whose refinement does not reach eps
+        c->stats.lig_solves += 2;
+        bool used[2] = {false, false};
+        if (lig_on) {
+            rc = lig_start2(c, b, x, r, use_fft, used);
             if (rc) return rc;
-            if (err >= eps && it < niter) {  // stalled: the plain FP64 solver from b
+        }
+        for (int p = 0; p < 2; p++)
+            if (!used[p]) c->stats.lig_fallbacks++;
+        rc = refine2_dev(c, 2, niter, eps, b, x, use_fft, it, err, lig_on ? r : nullptr);
+        for (int p = 0; p < 2 && !rc; p++)
+            if (err[p] >= eps && it[p] < niter) {
                 c->stats.lig_fallbacks++;
-                HIPCHK(hipMemcpyAsync(x, b, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+                HIPCHK(hipMemcpyAsync(x + V * p, b + V * p, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
                 int it2 = 0;
-                rc = bicg_dev(c, niter - it, eps, b, x, MSTM_HIP_FP64, use_fft, 0, &it2, &err);
-                it += it2;
+                rc = bicg2_dev(c, 1, niter - it[p], &eps, b + V * p, x + V * p, MSTM_HIP_FP64, use_fft, &it2, &err[p]);
+                it[p] += it2;
             }
-        } else {
-            // learned start (or b where the prediction is worse), mixed
-            // refinement from it; the plain FP64 solver from b takes over if
-            // the refinement does not reach eps
-            c->stats.lig_solves++;
-            bool used = false;
-            double *r0 = c->d_kv + V * 4;
-            if (lig_on) {
-                rc = lig_start(c, p, b, x, r0, use_fft, &used);
-                if (rc) return rc;
-            }
-            if (!used) c->stats.lig_fallbacks++;
-            rc = refine_dev(c, niter, eps, b, x, use_fft, &it, &err, lig_on ? r0 : nullptr);
-            if (rc) return rc;
-            if (err >= eps && it < niter) {
+    } else {
+        for (int p = 0; p < 2 && !rc; p++) {
+            rc = fgmres_dev(c, niter, eps, b + V * p, x + V * p, use_fft, lig_on, p, &it[p], &err[p]);
+            if (!rc && err[p] >= eps && it[p] < niter) {  // stalled: the plain FP64 solver from b
                 c->stats.lig_fallbacks++;
-                HIPCHK(hipMemcpyAsync(x, b, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+                HIPCHK(hipMemcpyAsync(x + V * p, b + V * p, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
                 int it2 = 0;
-                rc = bicg_dev(c, niter - it, eps, b, x, MSTM_HIP_FP64, use_fft, 0, &it2, &err);
-                it += it2;
+                rc = bicg2_dev(c, 1, niter - it[p], &eps, b + V * p, x + V * p, MSTM_HIP_FP64, use_fft, &it2, &err[p]);
+                it[p] += it2;
             }
         }
-        if (rc) return rc;
-        HIPCHK(hipMemcpyAsync(stage + V * p, x, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
-        *maxiter = it;
-        *maxerr = err;
-        if (it > niter || err > eps) *istat = 1;
     }
+    if (rc) return rc;
+    // istat, maxiter, maxerr are reset per polarization (:471-475): they
+    // report the second solve
+    *maxiter = it[1];
+    *maxerr = err[1];
+    *istat = (it[1] > niter || err[1] > eps) ? 1 : 0;
     HIPCHK(hipStreamSynchronize(c->st));
     c->stats.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    HIPCHK(hipMemcpyAsync(amnp, stage, sizeof(double) * 2 * V, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(amnp, x, sizeof(double) * 2 * V, hipMemcpyDeviceToHost, c->st));
     if (qeff) {
         double *dq = nullptr;
         HIPCHK(dalloc(&dq, sizeof(double) * 9 * c->nsph));
-        HIPCHK(launch_qeff(stage, pmnp, 2 * n, c->nsph, c->d_radius, c->d_order, c->d_offset, c->d_mieoff, c->d_an,
+        HIPCHK(launch_qeff(x, pmnp, 2 * n, c->nsph, c->d_radius, c->d_order, c->d_offset, c->d_mieoff, c->d_an,
                            c->rimed, dq, c->st));
         HIPCHK(hipMemcpyAsync(qeff, dq, sizeof(double) * 9 * c->nsph, hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
@@ -1057,7 +1198,7 @@ int mstm_hip_lig_samples(mstm_hip_ctx *c, const double *amnp, int Lnet, float *t
     HIPCHK(hipSetDevice(c->device));
     const int n = c->neqns;
     const size_t V = 2 * (size_t)n, row = (size_t)8 * Lnet * (Lnet + 2);
-    double *pmnp = c->d_kv + V * 9, *x = c->d_kv + V * 11, *b = c->d_kv + V * 12;
+    double *pmnp = c->d_kv + V * KV_PM, *x = c->d_kv + V * KV_X, *b = c->d_kv + V * KV_B;
     float *dt = nullptr, *db = nullptr;
     HIPCHK(dalloc(&dt, sizeof(float) * row * c->nsph));
     HIPCHK(dalloc(&db, sizeof(float) * row * c->nsph));
@@ -1085,7 +1226,7 @@ int mstm_hip_lig_guess(mstm_hip_ctx *c, double *x0)
     HIPCHK(hipSetDevice(c->device));
     const int n = c->neqns;
     const size_t V = 2 * (size_t)n;
-    double *pmnp = c->d_kv + V * 9, *x = c->d_kv + V * 11, *b = c->d_kv + V * 12;
+    double *pmnp = c->d_kv + V * KV_PM, *x = c->d_kv + V * KV_X, *b = c->d_kv + V * KV_B;
     auto t0 = std::chrono::steady_clock::now();
     int rc = lig_predict(c);
     if (rc) return rc;
@@ -1119,7 +1260,7 @@ const char *mstm_hip_stats_names(void)
     return "tables_ms_fp64,tables_ms_fp32,tile_ms_fp64,tile_ms_fp32,tile_launches_fp64,tile_launches_fp32,"
            "matvecs_fp64,matvecs_fp32,solve_ms,iters_fp64,iters_fp32,refinements,table_bytes_fp64,table_bytes_fp32,"
            "fft_setup_ms,fft_conv_ms,fft_launches,npad,ntiles,nblocks,lig_solves,lig_fallbacks,lig_ms,lig_err0,"
-           "lig_errb,fgmres_steps";
+           "lig_errb,fgmres_steps,tile4_ms_fp64,tile4_ms_fp32,tile4_launches_fp64,tile4_launches_fp32";
 }
 
 int mstm_hip_stats(const mstm_hip_ctx *c, double *out, int nout)
@@ -1150,7 +1291,11 @@ int mstm_hip_stats(const mstm_hip_ctx *c, double *out, int nout)
                   c->stats.lig_ms,
                   c->stats.lig_err0,
                   c->stats.lig_errb,
-                  (double)c->stats.fgmres_steps};
+                  (double)c->stats.fgmres_steps,
+                  c->stats.tile4_ms[0],
+                  c->stats.tile4_ms[1],
+                  (double)c->stats.tile4_launches[0],
+                  (double)c->stats.tile4_launches[1]};
     int k = std::min(nout, (int)(sizeof v / sizeof v[0]));
     for (int q = 0; q < k; q++) out[q] = v[q];
     return k;

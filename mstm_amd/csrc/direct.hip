@@ -66,17 +66,18 @@ __device__ __forceinline__ void load_sec(const T *__restrict__ p0, T *r)
 // occupancy target: FP32 tiles up to order 4 fit 2 waves/SIMD without
 // spilling; larger states use the full register file (AGPRs) instead
 template <typename T, int L> constexpr int tile_waves() { return (sizeof(T) == 4 && L <= 4) ? 2 : 1; }
-// One wave's work on a tile; DIR = 0 forward (I <- J), 1 reverse (J <- I).
-template <typename T, int L, int DIR>
+// One wave's work on a tile; DIR = 0 forward (I <- J), 1 reverse (J <- I);
+// the wave serves right-hand sides 2*rpair and 2*rpair+1 of NR.
+template <typename T, int L, int DIR, int NR>
 __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, int tile, const cplx<T> *__restrict__ xin,
-                                          cplx<T> *__restrict__ part, int npad)
+                                          cplx<T> *__restrict__ part, int npad, int rpair)
 {
     constexpr int VW = 16 / sizeof(T);
     constexpr int NB = L * (L + 2);
     constexpr int SL = slot_len(L, L, VW);
     constexpr int S = MSTM_SLOTS;
     const int lane = threadIdx.x & 63;
-    const int kslot = lane >> 2, rhs = (lane >> 1) & 1, mode = lane & 1;
+    const int kslot = lane >> 2, rhs = 2 * rpair + ((lane >> 1) & 1), mode = lane & 1;
     const bool diag = bt.x == bt.y;
     const int tgt = (DIR ? bt.y : bt.x) * S + kslot;
 
@@ -86,6 +87,10 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
 
 #pragma unroll 1
     for (int s = 0; s < S; s++) {
+        // all waves of the tile read the same table group in step s: keep them
+        // in step so that the group is fetched from HBM once and the other
+        // waves hit in L2
+        __syncthreads();
         if (diag && s == 0) continue;
         int slot, src;
         if (!DIR) {
@@ -177,7 +182,7 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
     }
     // partial for (target, partner block); m-flip back for the reverse wave
     const int partner = DIR ? bt.x : bt.y;
-    cplx<T> *out = part + (((int64_t)partner * 2 + rhs) * npad + tgt) * 2 * NB + mode * NB;
+    cplx<T> *out = part + (((int64_t)partner * NR + rhs) * npad + tgt) * 2 * NB + mode * NB;
 #pragma unroll
     for (int n = 1; n <= L; n++)
 #pragma unroll
@@ -192,27 +197,38 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
         }
 }
 
-template <typename T, int L>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(tile_waves<T, L>()))) void direct_tile_kernel(
+// NR = 2 (one BiCG product pair): 2 waves, forward and reverse.  NR = 4
+// (both polarizations' pairs at once): 4 waves; waves 0/1 serve right-hand
+// sides 0-1 and waves 2/3 sides 2-3 from the same table stream, so every
+// table byte read from HBM feeds four columns.
+template <typename T, int L, int NR>
+__global__ __launch_bounds__(64 * NR) __attribute__((amdgpu_waves_per_eu(tile_waves<T, L>()))) void direct_tile_kernel(
     const T *__restrict__ tab, const int2 *__restrict__ tiles, const cplx<T> *__restrict__ xin,
     cplx<T> *__restrict__ part, int npad)
 {
     const int tile = blockIdx.x;
     const int2 bt = tiles[tile];
-    if (threadIdx.x < 64) tile_body<T, L, 0>(tab, bt, tile, xin, part, npad);
-    else if (bt.x != bt.y) tile_body<T, L, 1>(tab, bt, tile, xin, part, npad);
+    const int w = threadIdx.x >> 6;
+    if (!(w & 1)) tile_body<T, L, 0, NR>(tab, bt, tile, xin, part, npad, w >> 1);
+    else if (bt.x != bt.y) tile_body<T, L, 1, NR>(tab, bt, tile, xin, part, npad, w >> 1);
+    else
+        for (int s = 0; s < MSTM_SLOTS; s++) __syncthreads();  // idle reverse wave of a diagonal tile: same barriers
 }
 
 template <typename T>
 hipError_t launch_direct_tiles(const T *tab, const int2 *tiles, int ntiles, int L, const T *xin, T *partial, int npad,
-                               hipStream_t st)
+                               int nr, hipStream_t st)
 {
     const cplx<T> *x = reinterpret_cast<const cplx<T> *>(xin);
     cplx<T> *p = reinterpret_cast<cplx<T> *>(partial);
-    dim3 g(ntiles), b(128);
-    switch (L) {
+    dim3 g(ntiles);
+    if (nr != 2 && nr != 4) return hipErrorInvalidValue;
+    switch (L * 8 + nr) {
 #define CASE(LL)                                                                                                       \
-    case LL: hipLaunchKernelGGL((direct_tile_kernel<T, LL>), g, b, 0, st, tab, tiles, x, p, npad); break;
+    case LL * 8 + 2: hipLaunchKernelGGL((direct_tile_kernel<T, LL, 2>), g, dim3(128), 0, st, tab, tiles, x, p, npad);    \
+        break;                                                                                                         \
+    case LL * 8 + 4: hipLaunchKernelGGL((direct_tile_kernel<T, LL, 4>), g, dim3(256), 0, st, tab, tiles, x, p, npad);    \
+        break;
 #ifdef MSTM_TILE_L_ONLY
         CASE(MSTM_TILE_L_ONLY)
 #else
@@ -223,10 +239,10 @@ hipError_t launch_direct_tiles(const T *tab, const int2 *tiles, int ntiles, int 
     }
     return hipGetLastError();
 }
-template hipError_t launch_direct_tiles<float>(const float *, const int2 *, int, int, const float *, float *, int,
+template hipError_t launch_direct_tiles<float>(const float *, const int2 *, int, int, const float *, float *, int, int,
                                                hipStream_t);
 template hipError_t launch_direct_tiles<double>(const double *, const int2 *, int, int, const double *, double *, int,
-                                                hipStream_t);
+                                                int, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Input preparation (sphereinteraction :88-97): for a con_tran column
@@ -235,8 +251,8 @@ template hipError_t launch_direct_tiles<double>(const double *, const int2 *, in
 // uniform-order copy (order Lp = max order, npad spheres); coefficients of
 // degree n > order(i) and padding spheres are zero, which is exact because
 // a translation table of order Lp contains those of lower orders.
-// flags[r] (host array) bit0 = con_tran, bit1 = mie_mult.  One thread per
-// (r, sphere, e).
+// flags[r] (host array, r < nrhs <= 4) bit0 = con_tran, bit1 = mie_mult.
+// One thread per (r, sphere, e), r < nr (2 or 4 columns in xin).
 // an: Mie matrices, 4 complex per degree, at (sphere*Lp + n-1)*4.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void decode_mn(int e, int L, int &m, int &n)
@@ -253,12 +269,12 @@ __device__ __forceinline__ void decode_mn(int e, int L, int &m, int &n)
 
 template <typename T>
 __global__ void prep_kernel(const cplx<double> *__restrict__ ain, int ldain, cplx<T> *__restrict__ xin, int nrhs,
-                            int flag0, int flag1, int nsph, int npad, int Lp, const int *__restrict__ order,
+                            int4 flags, int nr, int nsph, int npad, int Lp, const int *__restrict__ order,
                             const int *__restrict__ offset, const cplx<double> *__restrict__ an)
 {
     const int NBP = Lp * (Lp + 2);
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (int64_t)2 * npad * NBP) return;
+    if (q >= (int64_t)nr * npad * NBP) return;
     int e = (int)(q % NBP);
     int64_t t = q / NBP;
     int sph = (int)(t % npad);
@@ -272,7 +288,7 @@ __global__ void prep_kernel(const cplx<double> *__restrict__ ain, int ldain, cpl
         return;
     }
     const int L = order[sph], NB = L * (L + 2);
-    const int f = r == 0 ? flag0 : flag1;
+    const int f = r == 0 ? flags.x : (r == 1 ? flags.y : (r == 2 ? flags.z : flags.w));
     const cplx<double> *a = ain + (int64_t)r * ldain + offset[sph];
     if (!(f & 1)) {
         int es = amn(m, n, L);
@@ -296,17 +312,24 @@ __global__ void prep_kernel(const cplx<double> *__restrict__ ain, int ldain, cpl
     o[e + NBP] = cmk((T)(sg * v1.re), (T)(sg * v1.im));
 }
 
+static int4 flags4(const int *flags, int nrhs)
+{
+    return make_int4(flags[0], nrhs > 1 ? flags[1] : 0, nrhs > 2 ? flags[2] : 0, nrhs > 3 ? flags[3] : 0);
+}
+
 template <typename T>
 hipError_t launch_prep(const double *ain, int ldain, T *xin, int nrhs, const int *flags, int nsph, int npad, int Lp,
                        const int *order, const int *offset, const double *an, hipStream_t st)
 {
-    int f0 = flags[0], f1 = nrhs > 1 ? flags[1] : 0;
+    if (nrhs < 1 || nrhs > 4) return hipErrorInvalidValue;
+    const int nr = nrhs > 2 ? 4 : 2;
     int NBP = Lp * (Lp + 2);
-    int64_t n = (int64_t)2 * npad * NBP;
+    int64_t n = (int64_t)nr * npad * NBP;
     int bs = 256;
     hipLaunchKernelGGL(prep_kernel<T>, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, st,
                        reinterpret_cast<const cplx<double> *>(ain), ldain / 2, reinterpret_cast<cplx<T> *>(xin), nrhs,
-                       f0, f1, nsph, npad, Lp, order, offset, reinterpret_cast<const cplx<double> *>(an));
+                       flags4(flags, nrhs), nr, nsph, npad, Lp, order, offset,
+                       reinterpret_cast<const cplx<double> *>(an));
     return hipGetLastError();
 }
 template hipError_t launch_prep<float>(const double *, int, float *, int, const int *, int, int, int, const int *,
@@ -321,7 +344,7 @@ template hipError_t launch_prep<double>(const double *, int, double *, int, cons
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void reduce_post_kernel(const cplx<T> *__restrict__ part, int nparts, cplx<double> *__restrict__ aout,
-                                   int ldaout, int nrhs, int flag0, int flag1, int nsph, int npad, int Lp,
+                                   int ldaout, int nrhs, int4 flags, int nr, int nsph, int npad, int Lp,
                                    const int *__restrict__ order, const int *__restrict__ offset,
                                    const cplx<double> *__restrict__ an)
 {
@@ -337,10 +360,10 @@ __global__ void reduce_post_kernel(const cplx<T> *__restrict__ part, int nparts,
     const int L = order[sph];
     if (n > L) return;
     const int NB = L * (L + 2);
-    const int f = r == 0 ? flag0 : flag1;
+    const int f = r == 0 ? flags.x : (r == 1 ? flags.y : (r == 2 ? flags.z : flags.w));
     int es = (f & 1) ? amn(-m, n, Lp) : e;
     cplx<double> h0 = cmk(0.0, 0.0), h1 = cmk(0.0, 0.0);
-    const int64_t stride = (int64_t)2 * npad * 2 * NBP;
+    const int64_t stride = (int64_t)nr * npad * 2 * NBP;
     const cplx<T> *p = part + ((int64_t)r * npad + sph) * 2 * NBP;
     for (int b = 0; b < nparts; b++) {
         cplx<T> a0 = p[(int64_t)b * stride + es], a1 = p[(int64_t)b * stride + es + NBP];
@@ -370,13 +393,14 @@ hipError_t launch_reduce_post(const T *partial, int nparts, double *aout, int ld
                               int nsph, int npad, int Lp, const int *order, const int *offset, const double *an,
                               hipStream_t st)
 {
-    int f0 = flags[0], f1 = nrhs > 1 ? flags[1] : 0;
+    if (nrhs < 1 || nrhs > 4) return hipErrorInvalidValue;
+    const int nr = nrhs > 2 ? 4 : 2;
     int NBP = Lp * (Lp + 2);
     int64_t n = (int64_t)nrhs * nsph * NBP;
     int bs = 256;
     hipLaunchKernelGGL(reduce_post_kernel<T>, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, st,
                        reinterpret_cast<const cplx<T> *>(partial), nparts, reinterpret_cast<cplx<double> *>(aout),
-                       ldaout / 2, nrhs, f0, f1, nsph, npad, Lp, order, offset,
+                       ldaout / 2, nrhs, flags4(flags, nrhs), nr, nsph, npad, Lp, order, offset,
                        reinterpret_cast<const cplx<double> *>(an));
     return hipGetLastError();
 }

@@ -18,7 +18,7 @@ hipError_t launch_list_tables(T *tab, const double *rtran, int nslots, int LT, i
 // direct.hip
 template <typename T>
 hipError_t launch_direct_tiles(const T *tab, const int2 *tiles, int ntiles, int L, const T *xin, T *partial, int npad,
-                               hipStream_t st);
+                               int nr, hipStream_t st);
 template <typename T>
 hipError_t launch_prep(const double *ain, int ldain, T *xin, int nrhs, const int *flags, int nsph, int npad, int Lp,
                        const int *order, const int *offset, const double *an, hipStream_t st);
