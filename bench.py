@@ -388,6 +388,7 @@ def main():
                         qabs=q_lig[0, 1], iters_fp32=st_lig["iters_fp32"] / args.steps,
                         iters_fp64_matvecs=st_lig["matvecs_fp64"] / args.steps,
                         refinements=st_lig["refinements"] / args.steps,
+                        inner_solver="gmres" if st_lig.get("inner_solver", 0) == 1 else "bicg",
                         fallback_rate=st_lig["lig_fallbacks"] / lsolves,
                         start_residual_learned=st_lig["lig_err0"], start_residual_default=st_lig["lig_errb"],
                         network_ms=st_lig["lig_ms"], network=dict(samples=meta.get("samples"),

@@ -103,6 +103,11 @@ int mstm_hip_cbicg(mstm_hip_ctx *ctx, int niter, double eps, const double *pnp, 
 int mstm_hip_fixedorsoln(mstm_hip_ctx *ctx, double alpha, double sinc, int dir, double eps, int niter, int solver,
                          int use_fft, double *amnp, double *qeff, double *maxerr, int *maxiter, int *istat);
 
+/* Inner solver of the mixed refinement (MSTM_HIP_SOLVE_MIXED / _LEARNED; no
+ * counterpart in MSTM): 0 = BiCG (the reference's recurrences in FP32),
+ * 1 = GMRES (one product column per system and step; the default). */
+int mstm_hip_set_inner_solver(mstm_hip_ctx *ctx, int kind);
+
 /* clear_fft_matrix (fft_translation-5.f90:42-70) / table release. */
 int mstm_hip_clear(mstm_hip_ctx *ctx, int clear_tables);
 

@@ -230,6 +230,121 @@ __global__ __launch_bounds__(256) void gemm32_kernel(const GemmJob *__restrict__
     }
 }
 
+// Second variant of the grouped FP32 GEMM: the next 64 x 32 k-chunk is
+// fetched into registers while the current one is multiplied (two LDS
+// buffers, one barrier per chunk), and the LDS keeps each lane's four
+// successive k-pairs contiguous so one 16-byte read feeds four MFMA steps
+// (k = 2s + h stored at h*16 + s; rows padded to 36 floats).
+__global__ __launch_bounds__(256) void gemm32b_kernel(const GemmJob *__restrict__ jobs, int njobs,
+                                                      int *__restrict__ counter, const int *__restrict__ cM,
+                                                      const int64_t *__restrict__ choff,
+                                                      const int64_t *__restrict__ cvoff, const Z32 *__restrict__ H,
+                                                      const Z32 *__restrict__ in0, Z32 *__restrict__ out0,
+                                                      const Z32 *__restrict__ in1, Z32 *__restrict__ out1, int R)
+{
+    typedef float float4v __attribute__((ext_vector_type(4)));
+    __shared__ __align__(16) float As[2][2][64][36];
+    __shared__ __align__(16) float Bs[2][2][64][36];
+    __shared__ int sjob;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wr = wv >> 1, wc = wv & 1;
+    for (;;) {
+        if (tid == 0) sjob = atomicAdd(counter, 1);
+        __syncthreads();
+        const int jb = sjob;
+        __syncthreads();
+        if (jb >= njobs) break;
+        const GemmJob J = jobs[jb];
+        const int M = cM[J.c];
+        const Z32 *Hc = H + choff[J.c] + (int64_t)J.mode * M * M;
+        const int64_t vb = (cvoff[J.c] + (int64_t)J.mode * M) * R;
+        const Z32 *X = (J.op ? in1 : in0) + vb;
+        Z32 *Y = (J.op ? out1 : out0) + vb;
+        const int row0 = J.rt * 64;
+        Z32 ra[8], rb[8];
+        auto fetch = [&](int k0) {
+#pragma unroll
+            for (int it = 0; it < 8; it++) {
+                int rr, kk;
+                if (!J.op) {
+                    kk = tid & 31;
+                    rr = (tid >> 5) + 8 * it;
+                } else {
+                    rr = tid & 63;
+                    kk = (tid >> 6) + 4 * it;
+                }
+                const int gr = row0 + rr, gk = k0 + kk;
+                Z32 v = cmk(0.0f, 0.0f);
+                if (gr < M && gk < M) v = J.op ? Hc[(int64_t)gk * M + gr] : Hc[(int64_t)gr * M + gk];
+                ra[it] = J.op ? cmk(v.re, -v.im) : v;
+            }
+#pragma unroll
+            for (int it = 0; it < 8; it++) {
+                const int cc = tid & 63, kk = (tid >> 6) + 4 * it;
+                const int gk = k0 + kk;
+                rb[it] = (gk < M && cc < R) ? X[(int64_t)gk * R + cc] : cmk(0.0f, 0.0f);
+            }
+        };
+        auto stash = [&](int buf) {
+#pragma unroll
+            for (int it = 0; it < 8; it++) {
+                int rr, kk;
+                if (!J.op) {
+                    kk = tid & 31;
+                    rr = (tid >> 5) + 8 * it;
+                } else {
+                    rr = tid & 63;
+                    kk = (tid >> 6) + 4 * it;
+                }
+                const int ks = (kk & 1) * 16 + (kk >> 1);
+                As[buf][0][rr][ks] = ra[it].re;
+                As[buf][1][rr][ks] = ra[it].im;
+            }
+#pragma unroll
+            for (int it = 0; it < 8; it++) {
+                const int cc = tid & 63, kk = (tid >> 6) + 4 * it;
+                const int ks = (kk & 1) * 16 + (kk >> 1);
+                Bs[buf][0][cc][ks] = rb[it].re;
+                Bs[buf][1][cc][ks] = rb[it].im;
+            }
+        };
+        floatx16 cr, ci;
+#pragma unroll
+        for (int r = 0; r < 16; r++) cr[r] = ci[r] = 0.0f;
+        const int nch = (M + 31) / 32;
+        fetch(0);
+        stash(0);
+        __syncthreads();
+        const int arow = wr * 32 + (lane & 31), bcol = wc * 32 + (lane & 31), h = lane >> 5;
+        for (int ch = 0; ch < nch; ch++) {
+            const int buf = ch & 1;
+            if (ch + 1 < nch) fetch((ch + 1) * 32);
+#pragma unroll
+            for (int sb = 0; sb < 4; sb++) {
+                const float4v a_r = *reinterpret_cast<const float4v *>(&As[buf][0][arow][h * 16 + 4 * sb]);
+                const float4v a_i = *reinterpret_cast<const float4v *>(&As[buf][1][arow][h * 16 + 4 * sb]);
+                const float4v b_r = *reinterpret_cast<const float4v *>(&Bs[buf][0][bcol][h * 16 + 4 * sb]);
+                const float4v b_i = *reinterpret_cast<const float4v *>(&Bs[buf][1][bcol][h * 16 + 4 * sb]);
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    cr = __builtin_amdgcn_mfma_f32_32x32x2f32(a_r[u], b_r[u], cr, 0, 0, 0);
+                    cr = __builtin_amdgcn_mfma_f32_32x32x2f32(-a_i[u], b_i[u], cr, 0, 0, 0);
+                    ci = __builtin_amdgcn_mfma_f32_32x32x2f32(a_r[u], b_i[u], ci, 0, 0, 0);
+                    ci = __builtin_amdgcn_mfma_f32_32x32x2f32(a_i[u], b_r[u], ci, 0, 0, 0);
+                }
+            }
+            if (ch + 1 < nch) stash(buf ^ 1);
+            __syncthreads();
+        }
+        const int col = wc * 32 + (lane & 31);
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int row = row0 + wr * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (row < M && col < R) Y[(int64_t)row * R + col] = cmk(cr[r], ci[r]);
+        }
+    }
+}
+
 // FP64 grouped GEMM Y = H X (op 0 jobs only), VALU, 64x64 tile, 4x4 per thread
 __global__ __launch_bounds__(256) void gemm64_kernel(const GemmJob *__restrict__ jobs, int njobs,
                                                      int *__restrict__ counter, const int *__restrict__ cM,
@@ -710,7 +825,7 @@ struct mstm_hip_batch {
     // graph of GRAPH_K inner iterations
     hipGraphExec_t gexec = nullptr;
     // tuning and statistics
-    int grid32 = 1024, lpt = 1;
+    int grid32 = 1024, lpt = 1, kvar = 0;
     double build_ms = 0, setup_ms = 0, tune_ms = 0, gemm32_ms = 0, gemm64_ms = 0, solve_ms = 0;
     double inner_ms = 0;
     long gemm32_launches = 0, outer_steps = 0, inner_iters = 0;
@@ -777,8 +892,12 @@ static hipError_t gemm32(mstm_hip_batch *b, const Z32 *in0, Z32 *out0, const Z32
                          const GemmJob *jobs)
 {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, b->st, b->d_counter);
-    hipLaunchKernelGGL(gemm32_kernel, dim3(std::min(grid, b->njobs32)), dim3(256), 0, b->st, jobs, b->njobs32,
-                       b->d_counter, b->d_cM, b->d_choff, b->d_cvoff, b->d_H32, in0, out0, in1, out1, b->R);
+    if (b->kvar == 0)
+        hipLaunchKernelGGL(gemm32_kernel, dim3(std::min(grid, b->njobs32)), dim3(256), 0, b->st, jobs, b->njobs32,
+                           b->d_counter, b->d_cM, b->d_choff, b->d_cvoff, b->d_H32, in0, out0, in1, out1, b->R);
+    else
+        hipLaunchKernelGGL(gemm32b_kernel, dim3(std::min(grid, b->njobs32)), dim3(256), 0, b->st, jobs, b->njobs32,
+                           b->d_counter, b->d_cM, b->d_choff, b->d_cvoff, b->d_H32, in0, out0, in1, out1, b->R);
     return hipGetLastError();
 }
 
@@ -907,14 +1026,17 @@ static int build_rhs(mstm_hip_batch *b)
 
 static int autotune(mstm_hip_batch *b)
 {
-    const int grids[] = {256, 512, 768, 1024, 1536, 2048};
+    const int grids[] = {512, 1024, 2048};
     double best = 1e30;
+    int bestv = 0;
     b->tune_table.clear();
+    for (int kv = 0; kv < 2; kv++)
     for (int o = 0; o < 2; o++)
         for (int g : grids) {
+            b->kvar = kv;
             float ms = 0.0f, tot = 0.0f;
             BCHK(gemm32(b, b->p32, b->y1, b->tw, b->y2, g, b->d_jobs32[o]));  // warm-up
-            for (int rep = 0; rep < 3; rep++) {
+            for (int rep = 0; rep < 2; rep++) {
                 BCHK(hipEventRecord(b->ev0, b->st));
                 BCHK(gemm32(b, b->p32, b->y1, b->tw, b->y2, g, b->d_jobs32[o]));
                 BCHK(hipEventRecord(b->ev1, b->st));
@@ -922,16 +1044,18 @@ static int autotune(mstm_hip_batch *b)
                 BCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
                 tot += ms;
             }
-            const double avg = tot / 3.0;
-            b->tune_table.push_back(o);
+            const double avg = tot / 2.0;
+            b->tune_table.push_back(kv * 2 + o);
             b->tune_table.push_back(g);
             b->tune_table.push_back(avg);
             if (avg < best) {
                 best = avg;
                 b->grid32 = g;
                 b->lpt = o;
+                bestv = kv;
             }
         }
+    b->kvar = bestv;
     b->gemm32_ms = best;
     return 0;
 }
@@ -1429,7 +1553,7 @@ int mstm_hip_batch_qeff(mstm_hip_batch *b, double *q)
 const char *mstm_hip_batch_stats_names(void)
 {
     return "setup_ms,build_ms,tune_ms,solve_ms,inner_ms,gemm32_ms,gemm32_flops,gemm32_bytes,grid32,lpt,"
-           "outer_steps,inner_iters,hsize,nrows,njobs32";
+           "outer_steps,inner_iters,hsize,nrows,njobs32,gemm32_variant";
 }
 
 int mstm_hip_batch_stats(const mstm_hip_batch *b, double *out, int nout)
@@ -1437,13 +1561,13 @@ int mstm_hip_batch_stats(const mstm_hip_batch *b, double *out, int nout)
     if (!b || !out) return 0;
     double v[] = {b->setup_ms, b->build_ms, b->tune_ms, b->solve_ms, b->inner_ms, b->gemm32_ms, b->flops32,
                   b->bytes32, (double)b->grid32, (double)b->lpt, (double)b->outer_steps, (double)b->inner_iters,
-                  (double)b->hsize, (double)b->nrows, (double)b->njobs32};
+                  (double)b->hsize, (double)b->nrows, (double)b->njobs32, (double)b->kvar};
     int k = std::min(nout, (int)(sizeof v / sizeof v[0]));
     for (int q = 0; q < k; q++) out[q] = v[q];
     return k;
 }
 
-// autotune table: (order, grid, ms) triples; returns the number of values
+// autotune table: (2*variant + longest-first, grid, ms) triples; returns the number of values
 int mstm_hip_batch_tune_table(const mstm_hip_batch *b, double *out, int nout)
 {
     if (!b || !out) return 0;

@@ -96,6 +96,11 @@ struct mstm_hip_ctx {
     LearnedNet lig;
     double *d_fg = nullptr;  // FGMRES basis V (m+1) and Z (m), complex FP64
     size_t fg_bytes = 0;
+    double *d_gm = nullptr;  // GMRES basis of the inner solver, two systems interleaved
+    size_t gm_bytes = 0;
+    double *d_gh = nullptr;  // GMRES projections on the device (2 passes x 2 systems)
+    double *d_gp = nullptr;  // block partials of the GMRES projections (krylov grid x basis size)
+    int inner = 1;           // inner solver of the mixed refinement: 0 BiCG, 1 GMRES
     // Krylov work vectors (complex FP64, neqns each)
     double *d_kv = nullptr;
     size_t kv_n = 0;
@@ -235,6 +240,8 @@ static void release_spheres(mstm_hip_ctx *c)
     dfree(c->d_kv);
     dfree(c->d_fg);
     c->fg_bytes = 0;
+    dfree(c->d_gm);
+    c->gm_bytes = 0;
     c->kv_n = 0;
     c->nsph = 0;
 }
@@ -265,6 +272,8 @@ int mstm_hip_destroy(mstm_hip_ctx *c)
     hipStreamSynchronize(c->st);
     release_spheres(c);
     dfree(c->lig.params);
+    dfree(c->d_gh);
+    dfree(c->d_gp);
     dfree(c->d_vcc);
     dfree(c->d_partials);
     for (auto &e : c->ev) hipEventDestroy(e);
@@ -712,6 +721,193 @@ static int bicg2_dev(mstm_hip_ctx *c, int ns, int niter, const double *eps, cons
     return 0;
 }
 
+// GMRES(m) (Saad & Schultz, SIAM J. Sci. Stat. Comput. 7, 856 (1986)) for
+// ns = 1 or 2 systems A d_s = b_s at once, from d_s = 0.  Arnoldi vectors
+// are FP64 and orthogonalised by classical Gram-Schmidt applied twice (one
+// projection kernel and one update kernel per pass, no host round trip);
+// the products run at precision prec, one column per system (both systems
+// in one launch while both iterate) -- half the columns of a BiCG step.  A
+// system stops when its residual estimate |g_{j+1}|^2/|b|^2 < eps[s];
+// restarts every m steps from the residual b - A d.  it[s]: Arnoldi steps.
+static int scaled_copy(mstm_hip_ctx *c, double *dst, const double *src, double sr, double si);
+static const int GM_M = 120;
+static int gmres2_dev(mstm_hip_ctx *c, int ns, int niter, const double *eps, const double *b, double *x, int prec,
+                      int use_fft, int *it, double *err)
+{
+    typedef std::complex<double> zc;
+    const int m = std::min(GM_M, multi_vec_max() - 2);
+    const int n = c->neqns;
+    const size_t V = 2 * c->kv_n;
+    const size_t need = sizeof(double) * V * 2 * (m + 1);
+    if (c->gm_bytes < need) {
+        dfree(c->d_gm);
+        HIPCHK(dalloc(&c->d_gm, need));
+        c->gm_bytes = need;
+    }
+    if (!c->d_gh) HIPCHK(dalloc(&c->d_gh, sizeof(double) * 2 * 4 * (GM_M + 2)));
+    if (!c->d_gp) HIPCHK(dalloc(&c->d_gp, sizeof(double) * 2 * (size_t)krylov_grid() * (GM_M + 2)));
+    auto basis = [&](int j, int sys) { return c->d_gm + V * (2 * (size_t)j + sys); };
+    double *W = c->d_kv + V * KV_CAPS;  // w_s at W + s V
+    double *T = c->d_kv + V * KV_ST;    // temporaries t_s at T + s V
+    auto hdev = [&](int pass, int sys) { return c->d_gh + 2 * (size_t)(GM_M + 2) * (2 * sys + pass); };
+    double bb[2] = {0, 0};
+    bool open[2] = {false, false};
+    for (int q = 0; q < ns; q++) {
+        it[q] = 0;
+        err[q] = 0.0;
+        HIPCHK(hipMemsetAsync(x + V * q, 0, sizeof(double) * V, c->st));
+        HIPCHK(launch_zdot(b + V * q, b + V * q, n, 1, pregion(c, q), nullptr, c->st));
+    }
+    int rc = fetch_partials(c, 2 * krylov_grid() * ns);
+    if (rc) return rc;
+    for (int q = 0; q < ns; q++) {
+        double v[2];
+        reduce_region(c, pregion_off(q), 1, v);
+        bb[q] = v[0];
+        open[q] = bb[q] > 0.0;
+        err[q] = open[q] ? 1.0 : 0.0;
+    }
+    int f2[2] = {2, 2};
+    std::vector<zc> H[2], g[2], sn[2];
+    std::vector<double> cs[2];
+    for (int cycle = 0; open[0] || (ns == 2 && open[1]); cycle++) {
+        double beta[2] = {0, 0};
+        if (cycle == 0) {
+            for (int q = 0; q < ns; q++)
+                if (open[q]) {
+                    beta[q] = std::sqrt(bb[q]);
+                    rc = scaled_copy(c, basis(0, q), b + V * q, 1.0 / beta[q], 0.0);
+                    if (rc) return rc;
+                }
+        } else {
+            // r = b - A x = b - x + T H x
+            if (ns == 2 && open[0] && open[1]) rc = matvec(c, 2, x, W, f2, prec, use_fft);
+            else
+                for (int q = 0; q < ns && !rc; q++)
+                    if (open[q]) rc = matvec(c, 1, x + V * q, W + V * q, f2, prec, use_fft);
+            if (rc) return rc;
+            for (int q = 0; q < ns; q++) {
+                if (!open[q]) continue;
+                double *t = T + V * q;
+                HIPCHK(hipMemcpyAsync(t, b + V * q, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+                HIPCHK(launch_zaxpby(-1.0, 0.0, x + V * q, 1.0, 0.0, t, n, c->st));
+                HIPCHK(launch_zaxpby(1.0, 0.0, W + V * q, 1.0, 0.0, t, n, c->st));
+                HIPCHK(launch_zdot(t, t, n, 1, pregion(c, q), nullptr, c->st));
+            }
+            rc = fetch_partials(c, 2 * krylov_grid() * ns);
+            if (rc) return rc;
+            for (int q = 0; q < ns; q++) {
+                if (!open[q]) continue;
+                double v[2];
+                reduce_region(c, pregion_off(q), 1, v);
+                beta[q] = std::sqrt(std::max(v[0], 0.0));
+                err[q] = v[0] / bb[q];
+                if (err[q] < eps[q] || it[q] >= niter || beta[q] == 0.0) {
+                    open[q] = false;
+                    continue;
+                }
+                rc = scaled_copy(c, basis(0, q), T + V * q, 1.0 / beta[q], 0.0);
+                if (rc) return rc;
+            }
+        }
+        bool run[2] = {open[0], ns == 2 && open[1]};
+        int k[2] = {0, 0};
+        for (int q = 0; q < ns; q++) {
+            H[q].assign((size_t)(m + 1) * m, zc(0.0));
+            g[q].assign(m + 1, zc(0.0));
+            sn[q].assign(m, zc(0.0));
+            cs[q].assign(m, 0.0);
+            g[q][0] = beta[q];
+        }
+        for (int j = 0; j < m && (run[0] || run[1]); j++) {
+            // w = A v_j = v_j - T H v_j
+            if (run[0] && run[1]) rc = matvec(c, 2, basis(j, 0), W, f2, prec, use_fft);
+            else rc = matvec(c, 1, basis(j, run[0] ? 0 : 1), W + V * (run[0] ? 0 : 1), f2, prec, use_fft);
+            if (rc) return rc;
+            for (int q = 0; q < ns; q++) {
+                if (!run[q]) continue;
+                double *w = W + V * q;
+                HIPCHK(launch_zaxpby(1.0, 0.0, basis(j, q), -1.0, 0.0, w, n, c->st));
+                // two passes of classical Gram-Schmidt against v_0..v_j
+                for (int pass = 0; pass < 2; pass++) {
+                    HIPCHK(launch_multi_zdot(basis(0, q), (int64_t)(2 * V), j + 1, w, n, c->d_gp, hdev(pass, q),
+                                             c->st));
+                    HIPCHK(launch_multi_axpy(basis(0, q), (int64_t)(2 * V), j + 1, hdev(pass, q), -1.0, w, n,
+                                             c->st));
+                }
+            }
+            // |w|^2 of each system (separate partial regions), then one host round trip
+            for (int q = 0; q < ns; q++)
+                if (run[q]) HIPCHK(launch_zdot(W + V * q, W + V * q, n, 1, pregion(c, 2 + q), nullptr, c->st));
+            std::vector<zc> hh[2][2];
+            for (int q = 0; q < ns; q++) {
+                if (!run[q]) continue;
+                for (int pass = 0; pass < 2; pass++) {
+                    hh[q][pass].resize(j + 1);
+                    HIPCHK(hipMemcpyAsync(hh[q][pass].data(), hdev(pass, q), sizeof(zc) * (j + 1),
+                                          hipMemcpyDeviceToHost, c->st));
+                }
+            }
+            rc = fetch_partials(c, 2 * krylov_grid() * 4);
+            if (rc) return rc;
+            for (int q = 0; q < ns; q++) {
+                if (!run[q]) continue;
+                double v[2];
+                reduce_region(c, pregion_off(2 + q), 1, v);
+                const double hn = std::sqrt(std::max(v[0], 0.0));
+                auto h = [&](int r, int col) -> zc & { return H[q][(size_t)r * m + col]; };
+                for (int r = 0; r <= j; r++) h(r, j) = hh[q][0][r] + hh[q][1][r];
+                h(j + 1, j) = hn;
+                for (int r = 0; r < j; r++) {  // earlier rotations on column j
+                    const zc a = h(r, j), bq = h(r + 1, j);
+                    h(r, j) = cs[q][r] * a + sn[q][r] * bq;
+                    h(r + 1, j) = -std::conj(sn[q][r]) * a + cs[q][r] * bq;
+                }
+                const zc a = h(j, j), bq = h(j + 1, j);
+                const double nu = std::sqrt(std::norm(a) + std::norm(bq));
+                if (std::abs(a) == 0.0) {
+                    cs[q][j] = 0.0;
+                    sn[q][j] = 1.0;
+                } else {
+                    cs[q][j] = std::abs(a) / nu;
+                    sn[q][j] = (a / std::abs(a)) * std::conj(bq) / nu;
+                }
+                h(j, j) = cs[q][j] * a + sn[q][j] * bq;
+                h(j + 1, j) = 0.0;
+                g[q][j + 1] = -std::conj(sn[q][j]) * g[q][j];
+                g[q][j] = cs[q][j] * g[q][j];
+                k[q] = j + 1;
+                it[q]++;
+                if (prec == MSTM_HIP_FP64) c->stats.iters_fp64++;
+                else c->stats.iters_fp32++;
+                err[q] = std::norm(g[q][j + 1]) / bb[q];
+                if (err[q] < eps[q] || it[q] >= niter || hn == 0.0 || nu == 0.0) {
+                    run[q] = false;
+                    if (err[q] < eps[q] || it[q] >= niter) open[q] = false;
+                } else {
+                    rc = scaled_copy(c, basis(j + 1, q), W + V * q, 1.0 / hn, 0.0);
+                    if (rc) return rc;
+                }
+            }
+        }
+        // x += sum_i y_i v_i with R y = g
+        for (int q = 0; q < ns; q++) {
+            if (!k[q]) continue;
+            auto h = [&](int r, int col) -> zc & { return H[q][(size_t)r * m + col]; };
+            std::vector<zc> y(k[q]);
+            for (int r = k[q] - 1; r >= 0; r--) {
+                zc sum = g[q][r];
+                for (int t = r + 1; t < k[q]; t++) sum -= h(r, t) * y[t];
+                y[r] = h(r, r) != zc(0.0) ? sum / h(r, r) : zc(0.0);
+            }
+            HIPCHK(hipMemcpyAsync(hdev(0, q), y.data(), sizeof(zc) * k[q], hipMemcpyHostToDevice, c->st));
+            HIPCHK(launch_multi_axpy(basis(0, q), (int64_t)(2 * V), k[q], hdev(0, q), 1.0, x + V * q, n, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));  // y lives on the host stack
+        }
+    }
+    return 0;
+}
+
 // Mixed precision for ns = 1 or 2 systems: x_s <- x_s + d_s with A d_s = r_s
 // solved by FP32 BiCG, r_s = b_s - A x_s evaluated with the FP64 operator;
 // a system stops on the reference criterion |r|^2/|b|^2 < eps measured in
@@ -785,7 +981,12 @@ static int refine2_dev(mstm_hip_ctx *c, int ns, int niter, double eps, const dou
         int it_in[2] = {0, 0};
         double e_in[2] = {0, 0};
         const int budget = std::max(1, niter - std::max(iters[sidx[0]], nin > 1 ? iters[sidx[1]] : 0));
-        rc = bicg2_dev(c, nin, budget, eps_in, R + V * sidx[0], D + V * sidx[0], MSTM_HIP_FP32, use_fft, it_in, e_in);
+        if (c->inner == 1)
+            rc = gmres2_dev(c, nin, budget, eps_in, R + V * sidx[0], D + V * sidx[0], MSTM_HIP_FP32, use_fft, it_in,
+                            e_in);
+        else
+            rc = bicg2_dev(c, nin, budget, eps_in, R + V * sidx[0], D + V * sidx[0], MSTM_HIP_FP32, use_fft, it_in,
+                           e_in);
         if (rc) return rc;
         for (int q = 0; q < nin; q++) {
             const int s = sidx[q];
@@ -1243,6 +1444,14 @@ int mstm_hip_lig_guess(mstm_hip_ctx *c, double *x0)
     return 0;
 }
 
+int mstm_hip_set_inner_solver(mstm_hip_ctx *c, int kind)
+{
+    if (!c) return MSTM_HIP_EARG;
+    if (kind != 0 && kind != 1) return fail(c, MSTM_HIP_EARG, "set_inner_solver: kind %d (0 BiCG, 1 GMRES)", kind);
+    c->inner = kind;
+    return 0;
+}
+
 int mstm_hip_clear(mstm_hip_ctx *c, int clear_tables)
 {
     if (!c) return MSTM_HIP_EARG;
@@ -1260,7 +1469,7 @@ const char *mstm_hip_stats_names(void)
     return "tables_ms_fp64,tables_ms_fp32,tile_ms_fp64,tile_ms_fp32,tile_launches_fp64,tile_launches_fp32,"
            "matvecs_fp64,matvecs_fp32,solve_ms,iters_fp64,iters_fp32,refinements,table_bytes_fp64,table_bytes_fp32,"
            "fft_setup_ms,fft_conv_ms,fft_launches,npad,ntiles,nblocks,lig_solves,lig_fallbacks,lig_ms,lig_err0,"
-           "lig_errb,fgmres_steps,tile4_ms_fp64,tile4_ms_fp32,tile4_launches_fp64,tile4_launches_fp32";
+           "lig_errb,fgmres_steps,tile4_ms_fp64,tile4_ms_fp32,tile4_launches_fp64,tile4_launches_fp32,inner_solver";
 }
 
 int mstm_hip_stats(const mstm_hip_ctx *c, double *out, int nout)
@@ -1295,7 +1504,8 @@ int mstm_hip_stats(const mstm_hip_ctx *c, double *out, int nout)
                   c->stats.tile4_ms[0],
                   c->stats.tile4_ms[1],
                   (double)c->stats.tile4_launches[0],
-                  (double)c->stats.tile4_launches[1]};
+                  (double)c->stats.tile4_launches[1],
+                  (double)c->inner};
     int k = std::min(nout, (int)(sizeof v / sizeof v[0]));
     for (int q = 0; q < k; q++) out[q] = v[q];
     return k;

@@ -51,3 +51,9 @@ hipError_t launch_bicg_c(double *cp, double *cw, const double *cr, const double 
 hipError_t launch_bicg_init(double *cr, double *cq, double *cw, double *cp, const double *pnp, const double *anp, int n,
                             double *partials, hipStream_t st);
 hipError_t launch_zaxpby(double ar, double ai, const double *x, double br, double bi, double *y, int n, hipStream_t st);
+// block Gram-Schmidt (GMRES): V_j at V + j*stride (doubles), out/h device arrays of nvec complex
+int multi_vec_max();
+hipError_t launch_multi_zdot(const double *V, int64_t stride, int nvec, const double *w, int n, double *partials,
+                             double *out, hipStream_t st);
+hipError_t launch_multi_axpy(const double *V, int64_t stride, int nvec, const double *h, double sign, double *w, int n,
+                             hipStream_t st);

@@ -173,3 +173,81 @@ hipError_t launch_zaxpby(double ar, double ai, const double *x, double br, doubl
     hipLaunchKernelGGL(zaxpby_kernel, dim3(KGRID), dim3(KB), 0, st, cmk(ar, ai), (const Z *)x, cmk(br, bi), (Z *)y, n);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Block Gram-Schmidt pieces of the GMRES inner solver: nvec vectors V_j at
+// V + j*stride (complex), one vector w.
+// multi_zdot: partial[b*nvec + j] = sum over the block's share of conj(V_j) w
+// reduce_cols: out[j] = sum_b partial[b*nvec + j] (fixed order)
+// multi_axpy: w -= sum_j h_j V_j (sign = -1) or w += ... (sign = +1)
+// ---------------------------------------------------------------------------
+#define MV_MAX 128
+__global__ __launch_bounds__(KB) void multi_zdot_kernel(const cplx<double> *__restrict__ V, int64_t stride, int nvec,
+                                                        const cplx<double> *__restrict__ w, int n,
+                                                        cplx<double> *__restrict__ partials)
+{
+    __shared__ cplx<double> red[KB];
+    for (int j = 0; j < nvec; j++) {
+        cplx<double> s = cmk(0.0, 0.0);
+        const cplx<double> *v = V + (int64_t)j * stride;
+        for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) cfma(s, cconj(v[i]), w[i]);
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int h = KB / 2; h > 0; h >>= 1) {
+            if (threadIdx.x < h) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + h]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partials[(int64_t)blockIdx.x * nvec + j] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ void reduce_cols_kernel(const cplx<double> *__restrict__ partials, int nblk, int nvec,
+                                   cplx<double> *__restrict__ out)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nvec) return;
+    cplx<double> s = cmk(0.0, 0.0);
+    for (int b = 0; b < nblk; b++) s = cadd(s, partials[(int64_t)b * nvec + j]);
+    out[j] = s;
+}
+
+__global__ void multi_axpy_kernel(const cplx<double> *__restrict__ V, int64_t stride, int nvec,
+                                  const cplx<double> *__restrict__ h, double sign, cplx<double> *__restrict__ w, int n)
+{
+    __shared__ cplx<double> hs[MV_MAX];
+    for (int j = threadIdx.x; j < nvec; j += blockDim.x) hs[j] = cscale(h[j], sign);
+    __syncthreads();
+    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) {
+        cplx<double> a = w[i];
+        for (int j = 0; j < nvec; j++) cfma(a, hs[j], V[(int64_t)j * stride + i]);
+        w[i] = a;
+    }
+}
+
+int multi_vec_max() { return MV_MAX; }
+
+// out (device, nvec complex) = V^H w; partials needs KGRID*nvec complex
+hipError_t launch_multi_zdot(const double *V, int64_t stride, int nvec, const double *w, int n, double *partials,
+                             double *out, hipStream_t st)
+{
+    if (nvec <= 0) return hipSuccess;
+    hipLaunchKernelGGL(multi_zdot_kernel, dim3(KGRID), dim3(KB), 0, st, reinterpret_cast<const cplx<double> *>(V),
+                       stride / 2, nvec, reinterpret_cast<const cplx<double> *>(w), n,
+                       reinterpret_cast<cplx<double> *>(partials));
+    hipLaunchKernelGGL(reduce_cols_kernel, dim3((nvec + 63) / 64), dim3(64), 0, st,
+                       reinterpret_cast<const cplx<double> *>(partials), KGRID, nvec,
+                       reinterpret_cast<cplx<double> *>(out));
+    return hipGetLastError();
+}
+
+hipError_t launch_multi_axpy(const double *V, int64_t stride, int nvec, const double *h, double sign, double *w, int n,
+                             hipStream_t st)
+{
+    if (nvec <= 0) return hipSuccess;
+    if (nvec > MV_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(multi_axpy_kernel, dim3(KGRID), dim3(KB), 0, st, reinterpret_cast<const cplx<double> *>(V),
+                       stride / 2, nvec, reinterpret_cast<const cplx<double> *>(h), sign,
+                       reinterpret_cast<cplx<double> *>(w), n);
+    return hipGetLastError();
+}

@@ -174,6 +174,10 @@ class HipContext:
         self._chk(self.L.mstm_hip_lig_guess(self.h, _d(x0.view(np.float64))), "lig_guess")
         return x0
 
+    def set_inner_solver(self, kind: int):
+        """Inner solver of the mixed refinement: 0 BiCG, 1 GMRES (default)."""
+        self._chk(self.L.mstm_hip_set_inner_solver(self.h, int(kind)), "set_inner_solver")
+
     def clear(self, clear_tables=True):
         self._chk(self.L.mstm_hip_clear(self.h, int(bool(clear_tables))), "clear")
 

@@ -146,3 +146,24 @@ def test_fixedorsoln_fgmres_reaches_reference_residual(gpu_ctx):
     rp = gpu_ctx.fixedorsoln(eps=1e-14, niter=10000)
     assert rt["istat"] == 0
     assert rel(rt["amnp"], rp["amnp"]) < 1e-5
+
+
+@pytest.mark.parametrize("inner", [0, 1])
+def test_mixed_inner_solvers_reach_reference_residual(gpu_ctx, inner):
+    """Both inner solvers of the mixed refinement (0 BiCG, 1 GMRES) end on the
+    reference criterion in FP64 and, driven to a tight residual, agree with
+    the plain FP64 solution to 1e-5."""
+    from mstm_amd.solver import SOLVE_MIXED
+
+    g = golden("frac_agg")
+    load_cluster(gpu_ctx, g)
+    gpu_ctx.set_inner_solver(inner)
+    try:
+        r = gpu_ctx.fixedorsoln(eps=1e-6, niter=10000, solver=SOLVE_MIXED)
+        assert r["istat"] == 0 and r["maxerr"] < 1e-6
+        rt = gpu_ctx.fixedorsoln(eps=1e-14, niter=10000, solver=SOLVE_MIXED)
+        rp = gpu_ctx.fixedorsoln(eps=1e-14, niter=10000)
+        assert rt["istat"] == 0
+        assert rel(rt["amnp"], rp["amnp"]) < 1e-5
+    finally:
+        gpu_ctx.set_inner_solver(1)
