@@ -130,7 +130,8 @@ def bench_batch(args):
            "max_err": float(r["err"].max()), "outer_steps": st["outer_steps"],
            "dense_operator_GB": {"fp32": st["hsize"] * 8 / 1e9, "fp64": st["hsize"] * 16 / 1e9},
            "qext_mean": float(q[:, :, 0, 0].mean()),
-           "roofline": {"kernel": "gemm32_kernel (grouped complex GEMM, v_mfma_f32_32x32x2f32)", "bound": "mfma",
+           "roofline": {"kernel": ("gemm32b_kernel" if st.get("gemm32_variant", 0) == 1 else "gemm32_kernel")
+                        + " (grouped complex GEMM, v_mfma_f32_32x32x2f32)", "bound": "mfma",
                         "achieved": round(tf, 2), "peak": 157.3, "unit": "TFLOP/s", "frac": round(tf / 157.3, 4),
                         "traffic": None, "flops_per_launch": st["gemm32_flops"],
                         "algorithmic_bytes": st["gemm32_bytes"], "avg_launch_ms": st["gemm32_ms"]},

@@ -128,10 +128,11 @@ const char *mstm_hip_stats_names(void);
  * wavenumber shift.  Parameters: one flat float array, per layer W [out x in]
  * row-major then bias [out], then kappa (re, im); Lnet is the sphere order
  * the output layer covers (8 Lnet (Lnet+2) outputs, both polarizations).
- * MSTM_HIP_SOLVE_LEARNED evaluates the prediction and b in FP64, starts the
- * mixed solver from the better one, and hands over to the plain FP64 solver
- * from b if the mixed solve does not reach eps; other incidence directions
- * start from b.
+ * MSTM_HIP_SOLVE_LEARNED checks the prediction's residual against the zero
+ * start, runs the mixed solver from the better one (every answer is accepted
+ * on its FP64 residual), and hands over to the plain FP64 solver from b if
+ * the mixed solve does not reach eps; other incidence directions start from
+ * zero.  MSTM_HIP_SOLVE_MIXED starts from zero.
  * ------------------------------------------------------------------------- */
 int mstm_hip_lig_nparams(int hidden, int nlayers, int Lnet);
 /* load (params != NULL) or unload the network of the learned solver */

@@ -1017,49 +1017,46 @@ static int lig_predict(mstm_hip_ctx *c)
 }
 
 // Starting vectors of the learned solver for both polarizations: the
-// network's prediction xl_p and the default b_p are evaluated in FP64 (one
-// four-column product) and, per polarization, the one with the smaller
-// residual goes to x_p and its residual to r_p.  used[p]: prediction kept.
+// network's prediction xl_p is checked against the zero start (residual b,
+// |r|^2/|b|^2 = 1) with one two-column FP32 product; the better one goes to
+// x_p and its residual to r_p (the refinement re-evaluates every residual in
+// FP64 before it accepts an answer).  used[p]: prediction kept.
 static int lig_start2(mstm_hip_ctx *c, const double *b, double *x, double *r, int use_fft, bool *used)
 {
     const int n = c->neqns;
     const size_t V = 2 * c->kv_n;
-    double *in = c->d_kv + V * KV_PW, *y = c->d_kv + V * KV_CAPS;  // [xl_0, xb_0, xl_1, xb_1] -> products
-    double *rt = c->d_kv + V * KV_ST;                                // 4 residuals
-    for (int p = 0; p < 2; p++) {
-        HIPCHK(launch_lig_combine(c->nsph, p, c->d_order, c->d_offset, c->d_pos, c->rimed, c->lig.out, c->lig.Lnet,
-                                  c->lig.depth, c->lig.params + c->lig.np - 2, b + V * p, in + V * 2 * p, c->st));
-        HIPCHK(hipMemcpyAsync(in + V * (2 * p + 1), b + V * p, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
-    }
-    int f4[4] = {2, 2, 2, 2};
-    int rc = matvec(c, 4, in, y, f4, MSTM_HIP_FP64, use_fft);
-    if (rc) return rc;
-    for (int q = 0; q < 4; q++) {
-        double *rq = rt + V * q;
-        HIPCHK(hipMemcpyAsync(rq, b + V * (q / 2), sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
-        HIPCHK(launch_zaxpby(-1.0, 0.0, in + V * q, 1.0, 0.0, rq, n, c->st));
-        HIPCHK(launch_zaxpby(1.0, 0.0, y + V * q, 1.0, 0.0, rq, n, c->st));
-        HIPCHK(launch_zdot(rq, rq, n, 1, c->d_partials + (size_t)2 * krylov_grid() * q, nullptr, c->st));
-    }
+    double *in = c->d_kv + V * KV_PW, *y = c->d_kv + V * KV_CAPS;  // [xl_0, xl_1] -> products
     for (int p = 0; p < 2; p++)
-        HIPCHK(launch_zdot(b + V * p, b + V * p, n, 1, c->d_partials + (size_t)2 * krylov_grid() * (4 + p), nullptr,
-                           c->st));
-    rc = fetch_partials(c, 6 * krylov_grid());
+        HIPCHK(launch_lig_combine(c->nsph, p, c->d_order, c->d_offset, c->d_pos, c->rimed, c->lig.out, c->lig.Lnet,
+                                  c->lig.depth, c->lig.params + c->lig.np - 2, b + V * p, in + V * p, c->st));
+    int f2[2] = {2, 2};
+    int rc = matvec(c, 2, in, y, f2, MSTM_HIP_FP32, use_fft);
     if (rc) return rc;
-    double e[6];
-    for (int q = 0; q < 6; q++) {
-        double v[2];
-        reduce_region(c, krylov_grid() * q, 1, v);
-        e[q] = v[0];
-    }
     for (int p = 0; p < 2; p++) {
-        const double bn = e[4 + p] > 0.0 ? e[4 + p] : 1.0;
-        c->stats.lig_err0 = e[2 * p] / bn;  // the second polarization's values remain, as maxerr does
-        c->stats.lig_errb = e[2 * p + 1] / bn;
-        used[p] = e[2 * p] <= e[2 * p + 1];
-        const int k = used[p] ? 2 * p : 2 * p + 1;
-        HIPCHK(hipMemcpyAsync(x + V * p, in + V * k, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
-        HIPCHK(hipMemcpyAsync(r + V * p, rt + V * k, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+        double *rp = r + V * p;
+        HIPCHK(hipMemcpyAsync(rp, b + V * p, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(launch_zaxpby(-1.0, 0.0, in + V * p, 1.0, 0.0, rp, n, c->st));
+        HIPCHK(launch_zaxpby(1.0, 0.0, y + V * p, 1.0, 0.0, rp, n, c->st));
+        HIPCHK(launch_zdot(rp, rp, n, 1, c->d_partials + (size_t)2 * krylov_grid() * p, nullptr, c->st));
+        HIPCHK(launch_zdot(b + V * p, b + V * p, n, 1, c->d_partials + (size_t)2 * krylov_grid() * (2 + p), nullptr,
+                           c->st));
+    }
+    rc = fetch_partials(c, 4 * krylov_grid());
+    if (rc) return rc;
+    for (int p = 0; p < 2; p++) {
+        double el[2], eb[2];
+        reduce_region(c, krylov_grid() * p, 1, el);
+        reduce_region(c, krylov_grid() * (2 + p), 1, eb);
+        const double bn = eb[0] > 0.0 ? eb[0] : 1.0;
+        c->stats.lig_err0 = el[0] / bn;  // the second polarization's values remain, as maxerr does
+        c->stats.lig_errb = 1.0;         // the zero start
+        used[p] = el[0] < eb[0];
+        if (used[p]) {
+            HIPCHK(hipMemcpyAsync(x + V * p, in + V * p, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+        } else {
+            HIPCHK(hipMemsetAsync(x + V * p, 0, sizeof(double) * V, c->st));
+            HIPCHK(hipMemcpyAsync(r + V * p, b + V * p, sizeof(double) * V, hipMemcpyDeviceToDevice, c->st));
+        }
     }
     return 0;
 }
@@ -1294,20 +1291,25 @@ int mstm_hip_fixedorsoln(mstm_hip_ctx *c, double alpha, double sinc, int dir, do
     if (solver == MSTM_HIP_SOLVE_PLAIN) {
         rc = bicg2_dev(c, 2, niter, eps2, b, x, MSTM_HIP_FP64, use_fft, it, err);
     } else if (solver == MSTM_HIP_SOLVE_MIXED) {
-        rc = refine2_dev(c, 2, niter, eps, b, x, use_fft, it, err);
+        // the refinement starts from x = 0 (residual b, no product): the
+        // reference's start x = b has |r|^2/|b|^2 >> 1 on dense clusters
+        HIPCHK(hipMemsetAsync(x, 0, sizeof(double) * 2 * V, c->st));
+        rc = refine2_dev(c, 2, niter, eps, b, x, use_fft, it, err, b);
     } else if (solver == MSTM_HIP_SOLVE_LEARNED) {
-        // learned start (or b where the prediction is worse), mixed refinement
-        // from it; the plain FP64 solver from b takes over for a polarization
-        // whose refinement does not reach eps
+        // learned start (or x = 0 where the prediction is worse), mixed
+        // refinement from it; the plain FP64 solver from b takes over for a
+        // polarization whose refinement does not reach eps
         c->stats.lig_solves += 2;
         bool used[2] = {false, false};
         if (lig_on) {
             rc = lig_start2(c, b, x, r, use_fft, used);
             if (rc) return rc;
+        } else {
+            HIPCHK(hipMemsetAsync(x, 0, sizeof(double) * 2 * V, c->st));
         }
         for (int p = 0; p < 2; p++)
             if (!used[p]) c->stats.lig_fallbacks++;
-        rc = refine2_dev(c, 2, niter, eps, b, x, use_fft, it, err, lig_on ? r : nullptr);
+        rc = refine2_dev(c, 2, niter, eps, b, x, use_fft, it, err, lig_on ? r : b);
         for (int p = 0; p < 2 && !rc; p++)
             if (err[p] >= eps && it[p] < niter) {
                 c->stats.lig_fallbacks++;
