@@ -1308,3 +1308,270 @@ void orc_fft_external_to_external(const orc_cluster *c, int nrhs, const int *con
     free(gn);
     free(gt);
 }
+
+/* ------------------------------------------------------------------ */
+/* Post-solve observables of a single-origin run                        */
+/* (mstm-input-37.f90:1159-1197 and 1503-1535).                         */
+/* ------------------------------------------------------------------ */
+
+/* tranordertest (mstm-modules-33.f90:1896-1942): the smallest degree n at
+ * which the regular translation of a degree-lmax, m = 1 wave over r has
+ * gathered all but eps of its norm (capped at 200, at least lmax).  The
+ * medium index is real here, so xi_l = psi_l(z)/z i^l with real z.  psi
+ * is evaluated once to degree 200 + lmax (the reference re-evaluates it
+ * per n; both are the same functions).                                  */
+int orc_tranorder(double r, double rimed, int lmax, double eps)
+{
+    const int nlim = 200;
+    if (r == 0.0) return lmax;
+    double z = r * rimed;
+    int nw = nlim + lmax;
+    double *psi = malloc(sizeof(double) * (nw + 1));
+    orc_riccati_psi(nw, z, psi);
+    double sum = 0.0;
+    int n;
+    for (n = 1; n <= nlim; n++) {
+        int l = lmax;
+        zc c = fnr(2 * n + 1) * fnr(2 * l + 1) * cpow(CI, n - l);
+        zc a = 0.0, b = 0.0;
+        for (int w = iabs_(n - l); w <= n + l; w++) {
+            double v = orc_cg(n, -1, l, 1, w);
+            zc xw = psi[w] / z * cpow(CI, w);
+            if ((n + l + w) % 2 == 0) a += v * v * xw;
+            else b += v * v * xw;
+        }
+        a *= c;
+        b *= c;
+        sum += creal(a * conj(a) + b * conj(b));
+        if (fabs(1.0 - sum) < eps) break;
+    }
+    free(psi);
+    int nmax = imin(n, nlim);
+    return imax(nmax, lmax);
+}
+
+/* translation orders about origin r0 and the common-origin order T
+ * (mstm-input-37.f90:1159-1197 with set_t_matrix_order = .true.):
+ * T = max(min(max_mie_order, maxt), max_i min(tranorder_i, maxt)).     */
+int orc_translation_orders(const orc_cluster *c, const double *r0, double eps, int maxt, int *torder)
+{
+    int lmx = 0;
+    for (int i = 0; i < c->n; i++) lmx = imax(lmx, c->order[i]);
+    int T = imin(lmx, maxt);
+    for (int i = 0; i < c->n; i++) {
+        double d[3] = {c->pos[3 * i] - r0[0], c->pos[3 * i + 1] - r0[1], c->pos[3 * i + 2] - r0[2]};
+        double rt = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        torder[i] = imin(orc_tranorder(rt, creal(c->rimed), c->order[i], eps), maxt);
+        T = imax(T, torder[i]);
+    }
+    return T;
+}
+
+/* merge_to_common_origin (mstm-scatprops-26.f90:447-559): every sphere's
+ * solution (L/R modes) is carried to r0 by the regular translation of
+ * order min(T, torder_i) and summed; the result is stored in the E/M form
+ * a0(.,.,1) = aL + aR, a0(.,.,2) = aL - aR, layout (0:T+1, T, 2) per rhs.
+ * amnp holds nrhs columns of neqns, amnp0 nrhs columns of 2T(T+2).      */
+void orc_merge(const orc_cluster *c, int T, const int *torder, const double *r0, int nrhs, const zc *amnp, zc *amnp0)
+{
+    int ne = c->neqns, b0 = T * (T + 2);
+    zc ri[2] = {c->rimed, c->rimed};
+    memset(amnp0, 0, sizeof(zc) * 2 * b0 * nrhs);
+    for (int i = 0; i < c->n; i++) {
+        int L = c->order[i], Ti = imin(T, torder[i]), bt = Ti * (Ti + 2);
+        double rt[3] = {r0[0] - c->pos[3 * i], r0[1] - c->pos[3 * i + 1], r0[2] - c->pos[3 * i + 2]};
+        zc *t = malloc(sizeof(zc) * 2 * bt);
+        for (int r = 0; r < nrhs; r++) {
+            memset(t, 0, sizeof(zc) * 2 * bt);
+            orc_pair_translate(1, rt, ri, L, Ti, 0, 0, imax(L, Ti) >= c->switch_order, amnp + (size_t)r * ne + c->offset[i],
+                               t);
+            zc *o = amnp0 + (size_t)r * 2 * b0;
+            for (int n = 1; n <= Ti; n++)
+                for (int m = -n; m <= n; m++) {
+                    zc a1 = t[amnaddr(m, n, Ti) - 1], a2 = t[amnaddr(m, n, Ti) - 1 + bt];
+                    o[amnaddr(m, n, T) - 1] += a1 + a2;
+                    o[amnaddr(m, n, T) - 1 + b0] += a1 - a2;
+                }
+        }
+        free(t);
+    }
+}
+
+/* amplitude_to_scattering_matrix (mstm-scatprops-26.f90:1410-1436);
+ * sm is the Fortran sm(4,4), column-major.                              */
+static void amp_to_sm(const zc *sa, double *sm)
+{
+    zc sp[4][4];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) sp[i][j] = sa[i] * conj(sa[j]);
+#define SM(i, j) sm[((i) - 1) + 4 * ((j) - 1)]
+#define SP(i, j) sp[(i) - 1][(j) - 1]
+    SM(1, 1) = creal(SP(1, 1) + SP(2, 2) + SP(3, 3) + SP(4, 4));
+    SM(1, 2) = creal(-SP(1, 1) + SP(2, 2) - SP(3, 3) + SP(4, 4));
+    SM(2, 1) = creal(-SP(1, 1) + SP(2, 2) + SP(3, 3) - SP(4, 4));
+    SM(2, 2) = creal(SP(1, 1) + SP(2, 2) - SP(3, 3) - SP(4, 4));
+    SM(3, 3) = creal(2.0 * (SP(1, 2) + SP(3, 4)));
+    SM(3, 4) = 2.0 * cimag(SP(2, 1) + SP(4, 3));
+    SM(4, 3) = 2.0 * cimag(SP(1, 2) - SP(3, 4));
+    SM(4, 4) = creal(2.0 * (SP(1, 2) - SP(3, 4)));
+    SM(1, 3) = creal(2.0 * (SP(2, 3) + SP(1, 4)));
+    SM(3, 1) = creal(2.0 * (SP(2, 4) + SP(1, 3)));
+    SM(1, 4) = -2.0 * cimag(SP(2, 3) - SP(1, 4));
+    SM(4, 1) = -2.0 * cimag(SP(4, 2) + SP(1, 3));
+    SM(2, 3) = creal(-2.0 * (SP(2, 3) - SP(1, 4)));
+    SM(3, 2) = creal(-2.0 * (SP(2, 4) - SP(1, 3)));
+    SM(2, 4) = -2.0 * cimag(SP(2, 3) + SP(1, 4));
+    SM(4, 2) = -2.0 * cimag(SP(4, 2) - SP(1, 3));
+#undef SM
+#undef SP
+}
+
+/* scatteringmatrix (mstm-scatprops-26.f90:1498-1573) with rotate_plane =
+ * .false., normalize_s11 = .false.: amplitude matrix from the common-origin
+ * E/M coefficients a0 (0:T+1, T, 2, 2) at (ct, phi) with the normalised
+ * vector harmonics of taufunc (mstm-modules-33.f90:1324-1344), then the
+ * 16 Mueller elements / (4 pi) / medium index.  pi is the reference's
+ * widened real(4) value.                                                */
+void orc_scatmat(int T, const zc *a0, double ct, double phi, double rimed, double *sm, zc *sa_out)
+{
+    int b0 = T * (T + 2);
+    double *dc = malloc(sizeof(double) * 3 * (T + 1) * (T + 1));
+    orc_rotcoef(ct, 1, T, dc);
+#define DR(k, mn) dc[(k) + 1 + 3 * (mn)]
+    zc ephi = cos(phi) + CI * sin(phi);
+    zc *ephim = malloc(sizeof(zc) * (2 * T + 1));
+    ephim[T] = 1.0;
+    for (int m = 1; m <= T; m++) {
+        ephim[T + m] = ephi * ephim[T + m - 1];
+        ephim[T - m] = conj(ephim[T + m]);
+    }
+    zc sa[4] = {0, 0, 0, 0};
+    for (int n = 1; n <= T; n++) {
+        zc cin = cpow(-CI, n);
+        double fnm = sqrt((double)(2 * n + 1) / 2.0) / 4.0;
+        for (int m = -n; m <= n; m++) {
+            int mn = n * (n + 1) + m;
+            double tau[2] = {-fnm * (-DR(-1, mn) + DR(1, mn)), -fnm * (DR(-1, mn) + DR(1, mn))};
+            int ad = amnaddr(m, n, T) - 1;
+            for (int p = 0; p < 2; p++) {
+                zc a = a0[ad + p * b0], b = -a0[ad + p * b0 + 2 * b0];
+                sa[0] += cin * tau[1 - p] * b * ephim[T + m];
+                sa[1] += CI * cin * tau[p] * a * ephim[T + m];
+                sa[2] += CI * cin * tau[p] * b * ephim[T + m];
+                sa[3] += cin * tau[1 - p] * a * ephim[T + m];
+            }
+        }
+    }
+#undef DR
+    double qsca = 1.0 / PI_R4;
+    for (int k = 0; k < 4; k++) sa[k] *= 4.0 / sqrt(2.0 * qsca);
+    amp_to_sm(sa, sm);
+    for (int k = 0; k < 16; k++) sm[k] = sm[k] / (4.0 * PI_R4) / rimed;
+    if (sa_out)
+        for (int k = 0; k < 4; k++) sa_out[k] = sa[k];
+    free(dc);
+    free(ephim);
+}
+
+/* fosmexpansion (mstm-scatprops-26.f90:1820-1951): generalized-spherical-
+ * function expansion of the azimuthally averaged scattering matrix.  Each
+ * output is a Fortran real(8) (4,4,0:2T) array.  s00(1,1,1)/(3 s00(1,1,0))
+ * is the asymmetry parameter; 4 pi s00(1,1,0) is the scattering cross
+ * section in units of k^-2.                                             */
+void orc_fosm(int T, const zc *amnp, double rimed, double *s00, double *s02, double *sp22, double *sm22)
+{
+    int b0 = T * (T + 2), W = 2 * T + 1, ld = 2 * T + 5;
+    zc *A1 = calloc((size_t)ld * (T + 1) * 2, sizeof(zc)), *A2 = calloc((size_t)ld * (T + 1) * 2, sizeof(zc));
+#define AA(A, m, n, p) A[((m) + T + 2) + ld * ((n) + (T + 1) * (p))]
+    for (int n = 1; n <= T; n++)
+        for (int p = 0; p < 2; p++)
+            for (int m = -n; m <= n; m++) {
+                AA(A1, m, n, p) = amnp[amnaddr(m, n, T) - 1 + p * b0];
+                AA(A2, m, n, p) = amnp[amnaddr(m, n, T) - 1 + p * b0 + 2 * b0];
+            }
+    size_t nall = (size_t)16 * W;
+    memset(s00, 0, sizeof(double) * nall);
+    memset(s02, 0, sizeof(double) * nall);
+    memset(sp22, 0, sizeof(double) * nall);
+    memset(sm22, 0, sizeof(double) * nall);
+#define S(a, i, j, w) a[((i) - 1) + 4 * (((j) - 1) + 4 * (w))]
+    double *cm1p1 = malloc(sizeof(double) * (W + 1)), *cm1m1 = malloc(sizeof(double) * (W + 1));
+    double *cmm = malloc(sizeof(double) * (W + 1)), *cmm2 = malloc(sizeof(double) * (W + 1)),
+           *cmp2 = malloc(sizeof(double) * (W + 1));
+    for (int n = 1; n <= T; n++)
+        for (int l = 1; l <= T; l++) {
+            int wmin = iabs_(n - l), wmax = n + l;
+            zc fnl = sqrt((double)((2 * n + 1) * (2 * l + 1))) * cpow(CI, l - n);
+            for (int w = wmin; w <= wmax; w++) {
+                cm1p1[w] = orc_cg(n, -1, l, 1, w);
+                cm1m1[w] = orc_cg(n, -1, l, -1, w);
+            }
+            int mm = imin(n, l + 2);
+            for (int m = -mm; m <= mm; m++) {
+                double m1m = sgn1(iabs_(m));
+                for (int w = wmin; w <= wmax; w++) {
+                    cmm[w] = (iabs_(m) <= l) ? orc_cg(n, -m, l, m, w) : 0.0;
+                    cmm2[w] = (iabs_(m - 2) <= l) ? orc_cg(n, -m, l, m - 2, w) : 0.0;
+                    cmp2[w] = (iabs_(m + 2) <= l) ? orc_cg(n, -m, l, m + 2, w) : 0.0;
+                }
+                for (int p = 0; p < 2; p++)
+                    for (int q = 0; q < 2; q++) {
+                        double m1mq = sgn1(iabs_(m + q + 1));
+                        double m1mnpl = sgn1(iabs_(m + n + p + 1 + l));
+                        zc a1122 = AA(A1, m, n, p) * conj(AA(A1, m, l, q)) + AA(A2, m, n, p) * conj(AA(A2, m, l, q));
+                        zc a2112 = AA(A2, m, n, p) * conj(AA(A1, m, l, q)) - AA(A1, m, n, p) * conj(AA(A2, m, l, q));
+                        zc a1p2 = (AA(A1, m, n, p) + CI * AA(A2, m, n, p)) *
+                                  conj(AA(A1, m - 2, l, q) - CI * AA(A2, m - 2, l, q));
+                        zc a1m2 = (AA(A1, m, n, p) - CI * AA(A2, m, n, p)) *
+                                  conj(AA(A1, m + 2, l, q) + CI * AA(A2, m + 2, l, q));
+                        for (int w = wmin; w <= wmax; w++) {
+                            double m1w = sgn1(w);
+                            int even = ((n + l + w + p + q + 2) % 2) == 0;
+                            if (even) S(s00, 1, 1, w) += creal(-(m1m * fnl * a1122 * cm1p1[w] * cmm[w]) / 2.0);
+                            else S(s00, 4, 4, w) += creal(CI / 2.0 * m1m * fnl * a2112 * cm1p1[w] * cmm[w]);
+                            if (w < 2) continue;
+                            if (even) {
+                                S(s02, 2, 1, w) += creal(-(m1mq * a1122 * fnl * cm1m1[w] * cmm[w]) / 2.0);
+                                S(s02, 1, 2, w) += creal(-(m1m * a1p2 * fnl * cm1p1[w] * cmm2[w]) / 4.0);
+                                S(s02, 1, 2, w) += creal(-(m1m * a1m2 * fnl * cm1p1[w] * cmp2[w]) / 4.0);
+                            } else {
+                                S(s02, 3, 4, w) += cimag(-CI / 2.0 * m1mq * a2112 * fnl * cm1m1[w] * cmm[w]);
+                                S(s02, 4, 3, w) += cimag(m1m * a1p2 * fnl * cm1p1[w] * cmm2[w]) / 4.0;
+                                S(s02, 4, 3, w) -= cimag(m1m * a1m2 * fnl * cm1p1[w] * cmp2[w]) / 4.0;
+                            }
+                            S(sm22, 2, 2, w) += creal(-(m1mnpl * m1w * a1p2 * fnl * cm1m1[w] * cmm2[w]) / 8.0);
+                            S(sp22, 2, 2, w) += creal(-(m1mq * a1p2 * fnl * cm1m1[w] * cmm2[w]) / 8.0);
+                            S(sm22, 2, 2, w) += creal(-(m1mq * a1m2 * fnl * cm1m1[w] * cmp2[w]) / 8.0);
+                            S(sp22, 2, 2, w) += creal(-(m1mnpl * m1w * a1m2 * fnl * cm1m1[w] * cmp2[w]) / 8.0);
+                        }
+                    }
+            }
+        }
+    /* sign patch and scale of the reference (:1930-1948) */
+    for (int w = 0; w < W; w++) {
+        for (int i = 3; i <= 4; i++)
+            for (int j = 1; j <= i; j++) {
+                S(s00, j, i, w) = -S(s00, j, i, w);
+                S(s02, j, i, w) = -S(s02, j, i, w);
+                S(sm22, j, i, w) = -S(sm22, j, i, w);
+                S(sp22, j, i, w) = -S(sp22, j, i, w);
+            }
+        S(sm22, 3, 3, w) = -S(sm22, 2, 2, w);
+        S(sp22, 3, 3, w) = S(sp22, 2, 2, w);
+    }
+    for (size_t k = 0; k < nall; k++) {
+        s00[k] *= 0.5 / rimed;
+        s02[k] *= 0.5 / rimed;
+        sm22[k] *= 0.5 / rimed;
+        sp22[k] *= 0.5 / rimed;
+    }
+#undef S
+#undef AA
+    free(A1);
+    free(A2);
+    free(cm1p1);
+    free(cm1m1);
+    free(cmm);
+    free(cmm2);
+    free(cmp2);
+}

@@ -75,6 +75,14 @@ void orc_node_gather(const orc_cluster *c, const zc *anode, zc *asphere);
 void orc_node_convolution(const orc_cluster *c, const zc *anode, zc *gnode);
 int orc_local_pairs(const orc_cluster *c, int *pairs);
 
+/* post-solve observables (single-origin expansion) */
+int orc_tranorder(double r, double rimed, int lmax, double eps);
+int orc_translation_orders(const orc_cluster *c, const double *r0, double eps, int maxt, int *torder);
+void orc_merge(const orc_cluster *c, int T, const int *torder, const double *r0, int nrhs, const zc *amnp,
+               zc *amnp0);
+void orc_scatmat(int T, const zc *a0, double ct, double phi, double rimed, double *sm, zc *sa_out);
+void orc_fosm(int T, const zc *amnp, double rimed, double *s00, double *s02, double *sp22, double *sm22);
+
 #ifdef __cplusplus
 }
 #endif

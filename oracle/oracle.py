@@ -62,6 +62,11 @@ def lib():
         L.orc_pair_translate.argtypes = [C.c_int, dp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]
         L.orc_mie_order.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]
         L.orc_mie_coef.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, vp, dp, dp]
+        L.orc_tranorder.argtypes = [C.c_double, C.c_double, C.c_int, C.c_double]
+        L.orc_translation_orders.argtypes = [vp, dp, C.c_double, C.c_int, ip]
+        L.orc_merge.argtypes = [vp, C.c_int, ip, dp, C.c_int, vp, vp]
+        L.orc_scatmat.argtypes = [C.c_int, vp, C.c_double, C.c_double, C.c_double, dp, vp]
+        L.orc_fosm.argtypes = [C.c_int, vp, C.c_double, dp, dp, dp, dp]
         L.orc_cg.restype = C.c_double
         L.orc_cg.argtypes = [C.c_int] * 5
         L.orc_bcof.restype = C.c_double
@@ -174,11 +179,80 @@ class OracleCluster:
         self.fft = dict(cell_dim=cd.copy(), d_cell=float(d[0]), node_order=int(no[0]))
         return self.fft
 
+    def translation_orders(self, origin=None, eps=1e-5, maxt=100):
+        """(T, torder) about origin (default: mean sphere position)."""
+        r0 = np.ascontiguousarray(self.pos.mean(axis=0) if origin is None else origin, np.float64)
+        t = np.zeros(self.n, np.int32)
+        T = lib().orc_translation_orders(self.h, _d(r0), eps, maxt, _i(t))
+        return int(T), t
+
+    def merge(self, T, torder, origin, amnp):
+        """merge_to_common_origin of the (nrhs, neqns) solution: (nrhs, 2, T(T+2)) E/M."""
+        a = np.ascontiguousarray(amnp, np.complex128)
+        nrhs = a.shape[0]
+        out = np.zeros((nrhs, 2, T * (T + 2)), np.complex128)
+        t = np.ascontiguousarray(torder, np.int32)
+        r0 = np.ascontiguousarray(origin, np.float64)
+        lib().orc_merge(self.h, T, _i(t), _d(r0), nrhs, a.ctypes.data, out.ctypes.data)
+        return out
+
     def local_pairs(self):
         n = lib().orc_local_pairs(self.h, None)
         p = np.zeros((n, 2), np.int32)
         lib().orc_local_pairs(self.h, _i(p))
         return p
+
+
+# the reference's pi: a real(4) literal widened to real(8) (mstm-modules-33.f90:18)
+PI_R4 = float(np.float32(np.pi))
+
+
+def scattering_angles(alpha=0.0):
+    """Default scattering-matrix grid of a single-origin run without
+    azimuthal averaging (mstm-input-37.f90:1339-1351, 3036-3046):
+    theta = -180..180 deg in 1 deg steps; negative theta lies in the
+    half-plane phi = alpha + pi.  Returns (theta_deg, cos theta, phi)."""
+    th = np.arange(-180, 181)
+    ct = np.cos(PI_R4 / 180.0 * th.astype(np.float64))
+    ct[ct == 1.0] = 0.9999999
+    ct[ct == -1.0] = -0.9999999
+    phi = np.where(th < 0, alpha + PI_R4, alpha)
+    return th, ct, phi
+
+
+def scatmat(T, a0, ct, phi, rimed=1.0):
+    """16 Mueller elements (Fortran sm(4,4) column-major) at each (ct, phi)."""
+    a0 = np.ascontiguousarray(a0, np.complex128)
+    ct = np.atleast_1d(ct)
+    phi = np.broadcast_to(np.atleast_1d(phi), ct.shape)
+    out = np.zeros((len(ct), 16))
+    row = np.zeros(16)
+    for k in range(len(ct)):
+        lib().orc_scatmat(T, a0.ctypes.data, float(ct[k]), float(phi[k]), rimed, _d(row), None)
+        out[k] = row
+    return out
+
+
+def fosm(T, a0, rimed=1.0):
+    """fosmexpansion: s00, s02, sp22, sm22 as (2T+1, 4, 4) arrays [w, j-1, i-1] = s(i, j, w)."""
+    a0 = np.ascontiguousarray(a0, np.complex128)
+    outs = [np.zeros((2 * T + 1, 4, 4)) for _ in range(4)]
+    lib().orc_fosm(T, a0.ctypes.data, rimed, *[_d(o) for o in outs])
+    return outs
+
+
+def observables(cl: "OracleCluster", amnp, alpha=0.0, eps=1e-5, maxt=100):
+    """The single-origin post-processing of main_calling_program on a solved
+    cluster: orders, merged coefficients, default-grid scattering matrix,
+    s00 expansion and the asymmetry parameter g = s00(1,1,1)/(3 s00(1,1,0))."""
+    r0 = cl.pos.mean(axis=0)
+    T, tord = cl.translation_orders(r0, eps, maxt)
+    a0 = cl.merge(T, tord, r0, amnp)
+    th, ct, phi = scattering_angles(alpha)
+    sm = scatmat(T, a0, ct, phi)
+    s00 = fosm(T, a0)[0]
+    return dict(T=T, torder=tord, origin=r0, amnp0=a0, theta=th, scatmat=sm, s00=s00,
+                g=s00[1, 0, 0] / (3.0 * s00[0, 0, 0]))
 
 
 def rotcoef(cbe, kmax, nmax):
@@ -290,9 +364,9 @@ def run_reference(pos, radius, ri, mode, ain=None, outdir=None, timeout=None, **
         if name.startswith("out_") and name.endswith(".bin"):
             key = name[4:-4]
             raw = open(os.path.join(outdir, name), "rb").read()
-            if key in ("order", "offset"):
+            if key in ("order", "offset", "torder"):
                 res[key] = np.frombuffer(raw, np.int32).copy()
-            elif key in ("qeff", "qtot", "rotcoef"):
+            elif key in ("qeff", "qtot", "rotcoef", "scatmat", "s00", "s02", "sp22", "sm22"):
                 res[key] = np.frombuffer(raw, np.float64).copy()
             else:
                 res[key] = np.frombuffer(raw, np.complex128).copy()

@@ -12,6 +12,11 @@
 !                   external_to_external_expansion of the same RHS
 !   mode='solve'    fixedorsoln (BiCG) + qtotcalc
 !   mode='special'  rotcoef / axialtrancoefrecurrence / gentranmatrix samples
+! With observe=.true., 'solve' continues as a single-origin run of
+! main_calling_program does (mstm-input-37.f90:1159-1197, 1503-1535):
+! translation orders about the mean sphere position (tranordertest),
+! merge_to_common_origin of both polarizations, scatteringmatrix on the
+! default 1-degree grid theta = -180..180 and fosmexpansion.
 ! All outputs are raw little-endian stream files <outprefix>_<name>.bin plus
 ! a small text summary <outprefix>_info.txt.
 !
@@ -33,12 +38,13 @@
       integer :: nsphere,niter,node_order_in,nbr_model,i,n,ios,istat,maxiter,ncol,reps,r,p
       real(8) :: mie_eps,eps,cell_vf,alpha,sinc,vr,maxerr,t0,t1,rmed
       real(8) :: line(6),tmin(3),tmax(3),xg
-      logical :: use_fft,store_tables
+      logical :: use_fft,store_tables,observe
+      real(8) :: teps
       complex(8), allocatable :: ain(:,:),aout(:,:),gout(:,:),amnp(:,:),pmnp0(:,:),pmnpan(:)
       real(8), allocatable :: qeff(:,:,:),qvabs(:,:)
       real(8) :: qtot(3,3)
       namelist /job/ mode,posfile,nsphere,mie_eps,use_fft,cell_vf,node_order_in,nbr_model, &
-         eps,niter,alpha,sinc,infile,outprefix,reps,store_tables,rmed
+         eps,niter,alpha,sinc,infile,outprefix,reps,store_tables,rmed,observe,teps
 
       mode='setup'
       posfile='cluster.pos'
@@ -57,6 +63,8 @@
       reps=1
       store_tables=.false.
       rmed=1.d0
+      observe=.false.
+      teps=1.d-5
       call get_command_argument(1,jobfile)
       open(10,file=trim(jobfile),status='old')
       read(10,nml=job)
@@ -178,6 +186,7 @@
          call dump_z('amnp',amnp,2*number_eqns)
          call dump_d('qeff',qeff,9*nsphere)
          call dump_d('qtot',qtot,9)
+         if(observe) call observables()
       elseif(trim(mode).eq.'special') then
          call special_dumps()
       endif
@@ -213,6 +222,63 @@
          write(30) iv
          close(30)
          end subroutine dump_i
+!
+! post-solve observables of a single-origin run (see the header)
+!
+         subroutine observables()
+         integer :: j,nt,nsm
+         real(8) :: r0(3),rt,ct,ph
+         real(8), allocatable :: sm(:,:),s00(:,:,:),s02(:,:,:),sp22(:,:,:),sm22(:,:,:)
+         complex(8) :: rimed(2),sa(4)
+         complex(8), allocatable :: a0(:,:)
+         r0=0.d0
+         do j=1,nsphere
+            r0=r0+sphere_position(:,j)
+         enddo
+         r0=r0/dble(nsphere)
+         cluster_origin=r0
+         if(allocated(translation_order)) deallocate(translation_order)
+         allocate(translation_order(nsphere))
+         translation_order(1:nsphere)=sphere_order(1:nsphere)
+         t_matrix_order=min(max_mie_order,max_t_matrix_order)
+         do j=1,nsphere
+            call exteriorrefindex(j,rimed)
+            rt=sqrt(sum((sphere_position(:,j)-r0(:))**2))
+            call tranordertest(rt,rimed(1),sphere_order(j),teps,translation_order(j))
+            translation_order(j)=min(translation_order(j),max_t_matrix_order)
+            t_matrix_order=max(t_matrix_order,translation_order(j))
+         enddo
+         nt=t_matrix_order
+         allocate(a0(2*nt*(nt+2),2))
+         a0=(0.d0,0.d0)
+         do j=1,2
+            call merge_to_common_origin(nt,amnp(:,j),a0(:,j),origin_position=r0,merge_procs=.true.)
+         enddo
+         allocate(sm(16,-180:180))
+         do j=-180,180
+            ct=cos(pi/180.d0*dble(j))
+            if(ct.eq.1.d0) ct=0.9999999d0
+            if(ct.eq.-1.d0) ct=-0.9999999d0
+            if(j.lt.0) then
+               ph=alpha+pi
+            else
+               ph=alpha
+            endif
+            call scatteringmatrix(a0,nt,ct,ph,sa,sm(:,j),rotate_plane=.false.,normalize_s11=.false.)
+         enddo
+         allocate(s00(4,4,0:2*nt),s02(4,4,0:2*nt),sp22(4,4,0:2*nt),sm22(4,4,0:2*nt))
+         call fosmexpansion(nt,a0,s00,s02,sp22,sm22)
+         write(20,'(a,i10)') 't_matrix_order ',nt
+         write(20,'(a,3es25.17)') 'cluster_origin ',r0
+         call dump_i('torder',translation_order,nsphere)
+         call dump_z('amnp0',a0,size(a0))
+         nsm=size(sm)
+         call dump_d('scatmat',sm,nsm)
+         call dump_d('s00',s00,size(s00))
+         call dump_d('s02',s02,size(s02))
+         call dump_d('sp22',sp22,size(sp22))
+         call dump_d('sm22',sm22,size(sm22))
+         end subroutine observables
 !
 ! special functions at fixed sample points: Wigner d, axial coefficients
 ! (both types, unequal orders) and the dense translation matrix.

@@ -18,6 +18,11 @@ Cases
                 (see oracle/ref_harness.f90 for the reference allocation bug)
   frac_agg      examples/frac_agg.pos (100 touching monomers) scaled to
                 x = 0.5, m = 1.75+0.4i: matvec + solve
+  sparse100     100 spheres x=1, m=1.5+0.01i, seeded RSA in a sphere at
+                fv 0.002 (common-origin order 49): solve + observables only
+Every solve also runs the single-origin post-processing (translation orders,
+merge_to_common_origin, scatteringmatrix on the default grid,
+fosmexpansion); those outputs are stored under obs_* keys.
 """
 from __future__ import annotations
 
@@ -60,11 +65,29 @@ def case(name, pos, radius, ri, fft=False, solve=True, seed=0):
     else:
         res["hout"] = mv["hout"].reshape(2, neqns)
     if solve:
-        sv = O.run_reference(pos, radius, ri, "solve", use_fft=fft)
-        res.update(amnp=sv["amnp"].reshape(2, neqns), qeff=sv["qeff"].reshape(-1, 3, 3), qtot=sv["qtot"].reshape(3, 3),
-                   iterations=int(sv["iterations"]), maxerr=sv["maxerr"], istat=int(sv["istat"]))
+        res.update(solve_case(pos, radius, ri, neqns, fft))
     np.savez_compressed(os.path.join(OUT, name + ".npz"), **res)
     print(name, "neqns", neqns, "orders", sorted(set(res["order"].tolist())))
+
+
+def solve_case(pos, radius, ri, neqns, fft=False):
+    sv = O.run_reference(pos, radius, ri, "solve", use_fft=fft, observe=True)
+    T = int(sv["t_matrix_order"])
+    return dict(amnp=sv["amnp"].reshape(2, neqns), qeff=sv["qeff"].reshape(-1, 3, 3), qtot=sv["qtot"].reshape(3, 3),
+                iterations=int(sv["iterations"]), maxerr=sv["maxerr"], istat=int(sv["istat"]),
+                obs_T=T, obs_torder=sv["torder"], obs_origin=np.array(sv["cluster_origin"]),
+                obs_amnp0=sv["amnp0"].reshape(2, 2, T * (T + 2)), obs_scatmat=sv["scatmat"].reshape(361, 16),
+                **{"obs_" + k: sv[k].reshape(2 * T + 1, 4, 4) for k in ("s00", "s02", "sp22", "sm22")})
+
+
+def observe_case(name, pos, radius, ri):
+    ri = np.asarray(ri, np.complex128)
+    setup = O.run_reference(pos, radius, ri, "setup")
+    neqns = int(setup["neqns"])
+    res = dict(pos=pos, radius=radius, ri=ri, order=setup["order"], offset=setup["offset"])
+    res.update(solve_case(pos, radius, ri, neqns))
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **res)
+    print(name, "neqns", neqns, "T", res["obs_T"])
 
 
 def main():
@@ -77,6 +100,8 @@ def main():
     case("fft_cyl250", cyl.pos[:n], cyl.radius[:n], np.full(n, 1.6 + 0.0123j), fft=True, seed=13)
     agg = G.read_pos(os.path.join(REF_EXAMPLES, "frac_agg.pos"))
     case("frac_agg", agg.pos * 0.5, agg.radius * 0.5, np.full(agg.n, 1.75 + 0.4j), seed=14)
+    c = G.rsa_in_sphere(100, 1.0, 0.002, seed=7, ri=1.5 + 0.01j)
+    observe_case("sparse100", c.pos, c.radius, c.ri)
 
 
 if __name__ == "__main__":
