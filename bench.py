@@ -212,6 +212,15 @@ def bench_fft(args):
     ctx.close()
 
 
+def observe(ctx):
+    """Single-origin observables of the solution the last fixedorsoln left on the device."""
+    ob = ctx.observables(None)
+    st = ctx.stats()
+    return dict(T=ob["T"], g=ob["g"], s11=ob["scatmat"][:, 0].copy(), s12=ob["scatmat"][:, 4].copy(),
+                ms=dict(orders=round(st["obs_orders_ms"], 3), merge=round(st["obs_merge_ms"], 3),
+                        scatmat=round(st["obs_scatmat_ms"], 3), g=round(st["obs_moments_ms"], 3)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -282,6 +291,7 @@ def main():
     st_lig = ctx.stats()
     lig_s = float(np.mean(times))
     q_lig = qtotcalc(cl.radius, res["qeff"])
+    ob_lig = observe(ctx)
 
     # ---- mixed path without the network --------------------------------------
     ctx.clear(clear_tables=False)
@@ -293,6 +303,7 @@ def main():
     st_mixed = ctx.stats()
     mixed_s = float(np.mean(mtimes))
     q_mixed = qtotcalc(cl.radius, rm["qeff"])
+    ob_mixed = observe(ctx)
     print(f"[bench] mixed (no network): {mixed_s:.3f} s, iters {rm['iterations']}", file=sys.stderr, flush=True)
 
     # ---- plain FP64 path (once) ----------------------------------------------
@@ -304,6 +315,7 @@ def main():
         plain_s = time.perf_counter() - t0
         st_plain = ctx.stats()
         q_plain = qtotcalc(cl.radius, rp["qeff"])
+        ob_plain = observe(ctx)
         n4 = st_plain.get("tile4_launches_fp64", 0)
         plain = dict(s_per_solve=plain_s, iterations_last_pol=rp["iterations"], maxerr=rp["maxerr"],
                      qext=q_plain[0, 0], qabs=q_plain[0, 1],
@@ -319,12 +331,22 @@ def main():
         ctx.clear(clear_tables=False)
         rt = ctx.fixedorsoln(eps=1e-10, niter=10000, solver=SOLVE_PLAIN)
         q_ref = qtotcalc(cl.radius, rt["qeff"])
+        ob_ref = observe(ctx)
         qerr = {}
-        for name, q in (("learned", q_lig), ("mixed", q_mixed)) + ((("plain", q_plain),) if plain else ()):
+        runs = (("learned", q_lig, ob_lig), ("mixed", q_mixed, ob_mixed)) + \
+            ((("plain", q_plain, ob_plain),) if plain else ())
+        for name, q, ob in runs:
             qerr[name] = dict(qext=abs(q[0, 0] - q_ref[0, 0]) / abs(q_ref[0, 0]),
                               qsca=abs(q[0, 2] - q_ref[0, 2]) / abs(q_ref[0, 2]),
-                              qabs_abs=abs(q[0, 1] - q_ref[0, 1]))
-        qerr["reference"] = dict(eps=1e-10, qext=q_ref[0, 0], qabs=q_ref[0, 1], qsca=q_ref[0, 2])
+                              qabs_abs=abs(q[0, 1] - q_ref[0, 1]),
+                              g_abs=abs(ob["g"] - ob_ref["g"]),
+                              s11_max_rel=float(np.max(np.abs(ob["s11"] - ob_ref["s11"]) / np.abs(ob_ref["s11"]))),
+                              s12_max_rel_s11=float(np.max(np.abs(ob["s12"] - ob_ref["s12"]) / np.abs(ob_ref["s11"]))))
+        qerr["reference"] = dict(eps=1e-10, qext=q_ref[0, 0], qabs=q_ref[0, 1], qsca=q_ref[0, 2], g=ob_ref["g"],
+                                 s11_forward=ob_ref["s11"][180], s11_back=ob_ref["s11"][360],
+                                 t_matrix_order=ob_ref["T"])
+        qerr["note"] = ("g from the s00 moments of the merged single-origin expansion; S11/S12 on the default "
+                        "grid theta = -180..180 deg (361 directions); errors against the FP64 solve at 1e-10")
         print(f"[bench] tight FP64 reference done", file=sys.stderr, flush=True)
 
     # ---- roofline of the dominant kernel (FP32 tile kernel) ------------------
@@ -401,6 +423,9 @@ def main():
                       refinements=st_mixed["refinements"] / len(mtimes),
                       speedup_vs_plain_fp64=round(plain["s_per_solve"] / mixed_s, 3) if plain else None),
         "q_error_vs_tight_fp64": qerr,
+        "observables": dict(t_matrix_order=ob_lig["T"], g=ob_lig["g"], ms=ob_lig["ms"],
+                            note="post-solve, on the GPU: translation orders (host), merge to the common origin, "
+                                 "scattering matrix at 361 directions, s00 moments"),
         "matvec": matvec,
         "setup_s": dict(geometry=round(t_geom, 2), tables_fp64_fp32=round(t_tables, 3),
                         tables_fp64_ms=st0["tables_ms_fp64"], tables_fp32_ms=st0["tables_ms_fp32"],

@@ -108,6 +108,50 @@ int mstm_hip_fixedorsoln(mstm_hip_ctx *ctx, double alpha, double sinc, int dir, 
  * 1 = GMRES (one product column per system and step; the default). */
 int mstm_hip_set_inner_solver(mstm_hip_ctx *ctx, int kind);
 
+/* ---------------------------------------------------------------------------
+ * Post-solve observables of a single-origin expansion (a cluster without
+ * plane boundaries, single_origin_expansion = .true., the reference default;
+ * mstm-input-37.f90:1159-1197 and 1503-1535).  All sums are FP64 on the GPU.
+ * ------------------------------------------------------------------------- */
+
+/* Translation orders of the spheres about the origin r0 and the common-origin
+ * order T, as main_calling_program sets them (mstm-input-37.f90:1159-1197)
+ * with tranordertest (mstm-modules-33.f90:1896-1942):
+ *   torder_i = min(tranordertest(|r_i - r0|, m_medium, L_i, eps), maxt),
+ *   T = max(min(max_mie_order, maxt), max_i torder_i).
+ * origin NULL: the mean sphere position (cluster_origin).  eps is
+ * translation_epsilon (reference default 1e-5), maxt max_t_matrix_order
+ * (default 100; at most 127 here).  torder (nsphere ints) and tmo may be
+ * NULL.  The orders are kept for mstm_hip_merge_to_common_origin. */
+int mstm_hip_translation_orders(mstm_hip_ctx *ctx, const double *origin, double eps, int maxt, int *torder,
+                                int *tmo);
+
+/* merge_to_common_origin (mstm-scatprops-26.f90:447-559) of both columns of
+ * a solution: each sphere's expansion is carried to r0 by the regular
+ * rotate-axial-rotate translation of order min(T, torder_i) and summed.
+ * amnp: neqns x 2 complex (host), or NULL for the solution the last
+ * mstm_hip_fixedorsoln left on the device.  amnp0 (may be NULL): 2 columns of
+ * 2 T(T+2) complex in the reference's E/M layout (0:T+1, T, 2) per column
+ * (a0(.,.,1) = aL + aR, a0(.,.,2) = aL - aR).  The merged coefficients stay
+ * on the device for the two calls below. */
+int mstm_hip_merge_to_common_origin(mstm_hip_ctx *ctx, const double *amnp, double *amnp0);
+
+/* scatteringmatrix (mstm-scatprops-26.f90:1498-1573; rotate_plane = .false.,
+ * normalize_s11 = .false.) at nang directions (cos theta, phi), i.e. the loop
+ * of scattering_matrix_calculation (mstm-input-37.f90:3036-3062).  amnp0 NULL:
+ * the last merge (T ignored); else the caller's common-origin coefficients
+ * of order T in the layout above.  sm: nang x 16, the Fortran sm(4,4)
+ * (column-major) of each direction. */
+int mstm_hip_scattering_matrix(mstm_hip_ctx *ctx, int T, const double *amnp0, int nang, const double *ct,
+                               const double *phi, double *sm);
+
+/* s00(1,1,0) and s00(1,1,1) of fosmexpansion (mstm-scatprops-26.f90:1820-1951)
+ * and the asymmetry parameter g = s00(1,1,1) / (3 s00(1,1,0)) (the first
+ * Legendre moment of S11, mstm-scatprops-26.f90:1669).  4 pi s00(1,1,0) is the
+ * scattering cross section (k^-2 units).  amnp0/T as for the scattering
+ * matrix.  s00 (2 doubles) or g may be NULL. */
+int mstm_hip_asymmetry(mstm_hip_ctx *ctx, int T, const double *amnp0, double *s00, double *g);
+
 /* clear_fft_matrix (fft_translation-5.f90:42-70) / table release. */
 int mstm_hip_clear(mstm_hip_ctx *ctx, int clear_tables);
 

@@ -112,6 +112,28 @@ struct mstm_hip_ctx {
     size_t ev_used = 0;
     bool timing = true;
     Stats stats;
+    // post-solve observables of a single-origin expansion
+    struct {
+        bool x_valid = false;  // KV_X holds the last fixedorsoln solution
+        int T = 0;             // common-origin order of the last translation_orders call
+        double r0[3] = {0, 0, 0};
+        std::vector<int> torder;
+        int *d_torder = nullptr;
+        double *d_vccm = nullptr;
+        int vccm_T = 0, vccm_LS = 0;
+        double *d_part = nullptr;
+        size_t part_bytes = 0;
+        double *d_a0 = nullptr;  // merged E/M coefficients, 2 columns x 2 x T(T+2) complex
+        size_t a0_bytes = 0;
+        int a0_T = 0;
+        double *d_in = nullptr;  // caller-supplied common-origin coefficients
+        size_t in_bytes = 0;
+        double *d_ang = nullptr, *d_sm = nullptr;
+        size_t ang_cap = 0;
+        double *d_cg = nullptr, *d_mom = nullptr;
+        int cg_T = 0;
+        double orders_ms = 0, merge_ms = 0, scatmat_ms = 0, moments_ms = 0;
+    } obs;
 };
 
 static int fail(mstm_hip_ctx *c, int code, const char *fmt, ...)
@@ -244,6 +266,26 @@ static void release_spheres(mstm_hip_ctx *c)
     c->gm_bytes = 0;
     c->kv_n = 0;
     c->nsph = 0;
+    c->obs.x_valid = false;
+    c->obs.T = 0;
+    c->obs.a0_T = 0;
+    c->obs.torder.clear();
+    dfree(c->obs.d_torder);
+}
+
+static void release_observables(mstm_hip_ctx *c)
+{
+    dfree(c->obs.d_torder);
+    dfree(c->obs.d_vccm);
+    dfree(c->obs.d_part);
+    dfree(c->obs.d_a0);
+    dfree(c->obs.d_in);
+    dfree(c->obs.d_ang);
+    dfree(c->obs.d_sm);
+    dfree(c->obs.d_cg);
+    dfree(c->obs.d_mom);
+    c->obs.vccm_T = c->obs.vccm_LS = c->obs.cg_T = 0;
+    c->obs.part_bytes = c->obs.a0_bytes = c->obs.in_bytes = c->obs.ang_cap = 0;
 }
 
 // ------------------------------------------------------------------ C ABI
@@ -271,6 +313,7 @@ int mstm_hip_destroy(mstm_hip_ctx *c)
     hipSetDevice(c->device);
     hipStreamSynchronize(c->st);
     release_spheres(c);
+    release_observables(c);
     dfree(c->lig.params);
     dfree(c->d_gh);
     dfree(c->d_gp);
@@ -1239,6 +1282,7 @@ int mstm_hip_cbicg(mstm_hip_ctx *c, int niter, double eps, const double *pnp, do
     if (!c || !c->nsph) return fail(c, MSTM_HIP_ESTATE, "cbicg: no spheres");
     if (niter <= 0) return fail(c, MSTM_HIP_EUNSUP, "cbicg: niter <= 0 (order-of-scattering) not supported");
     HIPCHK(hipSetDevice(c->device));
+    c->obs.x_valid = false;  // the solution slot is reused
     const size_t V = 2 * (size_t)c->neqns;
     double *db = c->d_kv + V * KV_B, *dx = c->d_kv + V * KV_X;
     HIPCHK(hipMemcpyAsync(db, pnp, sizeof(double) * V, hipMemcpyHostToDevice, c->st));
@@ -1264,6 +1308,7 @@ int mstm_hip_fixedorsoln(mstm_hip_ctx *c, double alpha, double sinc, int dir, do
     if (solver == MSTM_HIP_SOLVE_LEARNED && !c->lig.ready)
         return fail(c, MSTM_HIP_ESTATE, "fixedorsoln: the learned solver needs a network (mstm_hip_set_learned)");
     HIPCHK(hipSetDevice(c->device));
+    c->obs.x_valid = false;
     const int n = c->neqns;
     const size_t V = 2 * (size_t)n;
     double *pmnp = c->d_kv + V * KV_PM, *x = c->d_kv + V * KV_X, *b = c->d_kv + V * KV_B, *r = c->d_kv + V * KV_R;
@@ -1339,6 +1384,7 @@ int mstm_hip_fixedorsoln(mstm_hip_ctx *c, double alpha, double sinc, int dir, do
     HIPCHK(hipStreamSynchronize(c->st));
     c->stats.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     HIPCHK(hipMemcpyAsync(amnp, x, sizeof(double) * 2 * V, hipMemcpyDeviceToHost, c->st));
+    c->obs.x_valid = true;
     if (qeff) {
         double *dq = nullptr;
         HIPCHK(dalloc(&dq, sizeof(double) * 9 * c->nsph));
@@ -1399,6 +1445,7 @@ int mstm_hip_lig_samples(mstm_hip_ctx *c, const double *amnp, int Lnet, float *t
     if (!c || !c->nsph) return fail(c, MSTM_HIP_ESTATE, "lig_samples: no spheres");
     if (!amnp || !target || !base || Lnet <= 0 || Lnet > MSTM_MAX_L) return fail(c, MSTM_HIP_EARG, "lig_samples: args");
     HIPCHK(hipSetDevice(c->device));
+    c->obs.x_valid = false;  // the solution slot is reused
     const int n = c->neqns;
     const size_t V = 2 * (size_t)n, row = (size_t)8 * Lnet * (Lnet + 2);
     double *pmnp = c->d_kv + V * KV_PM, *x = c->d_kv + V * KV_X, *b = c->d_kv + V * KV_B;
@@ -1427,6 +1474,7 @@ int mstm_hip_lig_guess(mstm_hip_ctx *c, double *x0)
     if (!c->lig.ready) return fail(c, MSTM_HIP_ESTATE, "lig_guess: no network (mstm_hip_set_learned)");
     if (!x0) return fail(c, MSTM_HIP_EARG, "lig_guess: null output");
     HIPCHK(hipSetDevice(c->device));
+    c->obs.x_valid = false;  // the solution slot is reused
     const int n = c->neqns;
     const size_t V = 2 * (size_t)n;
     double *pmnp = c->d_kv + V * KV_PM, *x = c->d_kv + V * KV_X, *b = c->d_kv + V * KV_B;
@@ -1471,7 +1519,8 @@ const char *mstm_hip_stats_names(void)
     return "tables_ms_fp64,tables_ms_fp32,tile_ms_fp64,tile_ms_fp32,tile_launches_fp64,tile_launches_fp32,"
            "matvecs_fp64,matvecs_fp32,solve_ms,iters_fp64,iters_fp32,refinements,table_bytes_fp64,table_bytes_fp32,"
            "fft_setup_ms,fft_conv_ms,fft_launches,npad,ntiles,nblocks,lig_solves,lig_fallbacks,lig_ms,lig_err0,"
-           "lig_errb,fgmres_steps,tile4_ms_fp64,tile4_ms_fp32,tile4_launches_fp64,tile4_launches_fp32,inner_solver";
+           "lig_errb,fgmres_steps,tile4_ms_fp64,tile4_ms_fp32,tile4_launches_fp64,tile4_launches_fp32,inner_solver,"
+           "obs_orders_ms,obs_merge_ms,obs_scatmat_ms,obs_moments_ms,obs_T";
 }
 
 int mstm_hip_stats(const mstm_hip_ctx *c, double *out, int nout)
@@ -1507,10 +1556,271 @@ int mstm_hip_stats(const mstm_hip_ctx *c, double *out, int nout)
                   c->stats.tile4_ms[1],
                   (double)c->stats.tile4_launches[0],
                   (double)c->stats.tile4_launches[1],
-                  (double)c->inner};
+                  (double)c->inner,
+                  c->obs.orders_ms,
+                  c->obs.merge_ms,
+                  c->obs.scatmat_ms,
+                  c->obs.moments_ms,
+                  (double)c->obs.T};
     int k = std::min(nout, (int)(sizeof v / sizeof v[0]));
     for (int q = 0; q < k; q++) out[q] = v[q];
     return k;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------- observables (host)
+// tranordertest (mstm-modules-33.f90:1896-1942) for a real medium index:
+// the smallest n (capped at 200, at least lmax) at which the regular
+// translation over r of a degree-lmax, m = 1 wave holds all but eps of its
+// norm.  vc2 holds C(n,-1;lmax,1|w,0)^2 at n*(2 lmax+1) + w - |n-lmax|.
+static int tran_order(double r, double rimed, int lmax, double eps, const std::vector<double> &vc2,
+                      std::vector<double> &jw)
+{
+    const int nlim = 200;
+    if (r == 0.0) return lmax;
+    const double z = r * rimed;
+    sph_bessel_j(nlim + lmax, z, jw.data());
+    double sum = 0.0;
+    int n;
+    for (n = 1; n <= nlim; n++) {
+        const int l = lmax, wmin = std::abs(n - l);
+        std::complex<double> a = 0.0, b = 0.0;
+        for (int w = wmin; w <= n + l; w++) {
+            static const std::complex<double> ipow[4] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
+            std::complex<double> xw = jw[w] * ipow[w & 3];
+            double v2 = vc2[(size_t)n * (2 * l + 1) + (w - wmin)];
+            if (((n + l + w) & 1) == 0) a += v2 * xw;
+            else b += v2 * xw;
+        }
+        static const std::complex<double> ipw[4] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
+        std::complex<double> cc = std::sqrt((double)(2 * n + 1)) * std::sqrt((double)(2 * l + 1)) * ipw[((n - l) % 4 + 4) % 4];
+        a *= cc;
+        b *= cc;
+        sum += std::norm(a) + std::norm(b);
+        if (std::fabs(1.0 - sum) < eps) break;
+    }
+    return std::max(std::min(n, nlim), lmax);
+}
+
+// vcc(n, l, m, w) of axialtrancoefinit for the merge seeds: n <= T, l <= LS;
+// entries are filled where the seeds read them (n <= LS, or l one of the
+// sphere orders).  Layout ((n (LS+1) + l)(2LS+1) + m+LS)(2LS+1) + w-|n-l|.
+static int ensure_vccm(mstm_hip_ctx *c, int T, int LS)
+{
+    if (c->obs.d_vccm && c->obs.vccm_T == T && c->obs.vccm_LS == LS) return 0;
+    dfree(c->obs.d_vccm);
+    const int W = 2 * LS + 1;
+    std::vector<double> h((size_t)(T + 1) * (LS + 1) * W * W, 0.0);
+    std::vector<char> have(LS + 1, 0);
+    for (int L : c->order) have[L] = 1;
+    for (int n = 1; n <= T; n++)
+        for (int l = 1; l <= LS; l++) {
+            if (n > LS && !have[l]) continue;
+            const long double c1 = sqrtl((long double)(2 * n + 1)) * sqrtl((long double)(2 * l + 1));
+            for (int m = -std::min(n, l); m <= std::min(n, l); m++)
+                for (int w = std::abs(n - l); w <= n + l; w++) {
+                    const long double c2 = -c1 * ((std::abs(m) & 1) ? -1.0L : 1.0L);
+                    const int k4 = (n - l + w) & 3;
+                    const long double sg = (k4 == 0 || k4 == 1) ? 1.0L : -1.0L;
+                    h[(((size_t)n * (LS + 1) + l) * W + (m + LS)) * W + (w - std::abs(n - l))] =
+                        (double)(c2 * clebsch(n, -m, l, m, w) * clebsch(n, -1, l, 1, w) * sg);
+                }
+        }
+    HIPCHK(dalloc(&c->obs.d_vccm, sizeof(double) * h.size()));
+    HIPCHK(hipMemcpy(c->obs.d_vccm, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    c->obs.vccm_T = T;
+    c->obs.vccm_LS = LS;
+    return 0;
+}
+
+template <typename P> static int grow(mstm_hip_ctx *c, P **p, size_t &cap, size_t bytes)
+{
+    if (*p && cap >= bytes) return 0;
+    dfree(*p);
+    cap = 0;
+    HIPCHK(dalloc(p, bytes));
+    cap = bytes;
+    return 0;
+}
+
+// common-origin coefficients for the scattering matrix / moments: the last
+// merge (a0 == NULL) or the caller's (T, a0) copied to the device
+static int observe_source(mstm_hip_ctx *c, int T, const double *a0, const double **src, int *Tout)
+{
+    if (!a0) {
+        if (!c->obs.a0_T) return fail(c, MSTM_HIP_ESTATE, "no merged coefficients (mstm_hip_merge_to_common_origin)");
+        *src = c->obs.d_a0;
+        *Tout = c->obs.a0_T;
+        return 0;
+    }
+    if (T < 1 || T > 127) return fail(c, MSTM_HIP_EUNSUP, "common-origin order %d outside 1..127", T);
+    const size_t bytes = sizeof(double) * 2 * 4 * (size_t)T * (T + 2);
+    if (int rc = grow(c, &c->obs.d_in, c->obs.in_bytes, bytes)) return rc;
+    HIPCHK(hipMemcpyAsync(c->obs.d_in, a0, bytes, hipMemcpyHostToDevice, c->st));
+    *src = c->obs.d_in;
+    *Tout = T;
+    return 0;
+}
+
+extern "C" {
+
+int mstm_hip_translation_orders(mstm_hip_ctx *c, const double *origin, double eps, int maxt, int *torder, int *tmo)
+{
+    if (!c || !c->nsph) return fail(c, MSTM_HIP_ESTATE, "translation_orders: no spheres");
+    if (maxt < 1 || maxt > 127) return fail(c, MSTM_HIP_EUNSUP, "translation_orders: maxt %d outside 1..127", maxt);
+    if (!(eps > 0.0)) return fail(c, MSTM_HIP_EARG, "translation_orders: eps must be > 0");
+    HIPCHK(hipSetDevice(c->device));
+    auto t0 = std::chrono::steady_clock::now();
+    const int n = c->nsph;
+    double r0[3] = {0, 0, 0};
+    if (origin) {
+        for (int d = 0; d < 3; d++) r0[d] = origin[d];
+    } else {  // cluster_origin: mean sphere position (mstm-input-37.f90:1166-1174)
+        for (int i = 0; i < n; i++)
+            for (int d = 0; d < 3; d++) r0[d] += c->pos[3 * i + d];
+        for (int d = 0; d < 3; d++) r0[d] /= (double)n;
+    }
+    std::vector<std::vector<double>> vc2(c->Lmax + 1);
+    std::vector<double> jw(200 + c->Lmax + 1);
+    c->obs.torder.assign(n, 0);
+    int T = std::min(c->Lmax, maxt);
+    for (int i = 0; i < n; i++) {
+        const int L = c->order[i];
+        if (vc2[L].empty()) {
+            vc2[L].assign((size_t)201 * (2 * L + 1), 0.0);
+            for (int nn = 1; nn <= 200; nn++)
+                for (int w = std::abs(nn - L); w <= nn + L; w++) {
+                    const double v = (double)clebsch(nn, -1, L, 1, w);
+                    vc2[L][(size_t)nn * (2 * L + 1) + (w - std::abs(nn - L))] = v * v;
+                }
+        }
+        const double dx = c->pos[3 * i] - r0[0], dy = c->pos[3 * i + 1] - r0[1], dz = c->pos[3 * i + 2] - r0[2];
+        const double rt = std::sqrt(dx * dx + dy * dy + dz * dz);
+        c->obs.torder[i] = std::min(tran_order(rt, c->rimed, L, eps, vc2[L], jw), maxt);
+        T = std::max(T, c->obs.torder[i]);
+    }
+    dfree(c->obs.d_torder);
+    HIPCHK(dalloc(&c->obs.d_torder, sizeof(int) * n));
+    HIPCHK(hipMemcpy(c->obs.d_torder, c->obs.torder.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    for (int d = 0; d < 3; d++) c->obs.r0[d] = r0[d];
+    c->obs.T = T;
+    c->obs.a0_T = 0;
+    if (torder) std::memcpy(torder, c->obs.torder.data(), sizeof(int) * n);
+    if (tmo) *tmo = T;
+    c->obs.orders_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+int mstm_hip_merge_to_common_origin(mstm_hip_ctx *c, const double *amnp, double *amnp0)
+{
+    if (!c || !c->nsph) return fail(c, MSTM_HIP_ESTATE, "merge: no spheres");
+    if (!c->obs.T) return fail(c, MSTM_HIP_ESTATE, "merge: no translation orders (mstm_hip_translation_orders)");
+    if (c->Lmax > MSTM_MAX_L) return fail(c, MSTM_HIP_EUNSUP, "merge: sphere order %d > %d", c->Lmax, MSTM_MAX_L);
+    if (!amnp && !c->obs.x_valid) return fail(c, MSTM_HIP_ESTATE, "merge: no solution on the device (run fixedorsoln)");
+    HIPCHK(hipSetDevice(c->device));
+    auto t0 = std::chrono::steady_clock::now();
+    const int n = c->neqns, T = c->obs.T, b0 = T * (T + 2);
+    const size_t V = 2 * (size_t)n;
+    const double *src = c->d_kv + V * KV_X;
+    if (amnp) {
+        double *st = c->d_kv + V * KV_ST;
+        HIPCHK(hipMemcpyAsync(st, amnp, sizeof(double) * 2 * V, hipMemcpyHostToDevice, c->st));
+        src = st;
+    }
+    if (int rc = ensure_vccm(c, T, c->Lmax)) return rc;
+    const size_t per_group = sizeof(double) * 8 * (size_t)b0;
+    int G = std::min(c->nsph, 1024);
+    G = (int)std::max<size_t>(1, std::min<size_t>((size_t)G, ((size_t)1 << 30) / per_group));
+    if (int rc = grow(c, &c->obs.d_part, c->obs.part_bytes, per_group * G)) return rc;
+    if (int rc = grow(c, &c->obs.d_a0, c->obs.a0_bytes, sizeof(double) * 8 * (size_t)b0)) return rc;
+    HIPCHK(launch_merge(src, n, c->d_order, c->d_offset, c->obs.d_torder, c->d_pos, c->nsph, T, c->Lmax, c->obs.r0,
+                        c->rimed, c->obs.d_vccm, c->obs.d_part, G, c->obs.d_a0, c->st));
+    c->obs.a0_T = T;
+    if (amnp0) HIPCHK(hipMemcpyAsync(amnp0, c->obs.d_a0, sizeof(double) * 8 * (size_t)b0, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->obs.merge_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+int mstm_hip_scattering_matrix(mstm_hip_ctx *c, int T, const double *amnp0, int nang, const double *ct,
+                               const double *phi, double *sm)
+{
+    if (!c) return fail(c, MSTM_HIP_EARG, "scattering_matrix: null context");
+    if (nang < 1 || !ct || !phi || !sm) return fail(c, MSTM_HIP_EARG, "scattering_matrix: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    auto t0 = std::chrono::steady_clock::now();
+    const double *src = nullptr;
+    int Tu = 0;
+    if (int rc = observe_source(c, T, amnp0, &src, &Tu)) return rc;
+    if ((size_t)nang > c->obs.ang_cap) {
+        dfree(c->obs.d_ang);
+        dfree(c->obs.d_sm);
+        c->obs.ang_cap = 0;
+        HIPCHK(dalloc(&c->obs.d_ang, sizeof(double) * 2 * nang));
+        HIPCHK(dalloc(&c->obs.d_sm, sizeof(double) * 16 * nang));
+        c->obs.ang_cap = nang;
+    }
+    HIPCHK(hipMemcpyAsync(c->obs.d_ang, ct, sizeof(double) * nang, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->obs.d_ang + nang, phi, sizeof(double) * nang, hipMemcpyHostToDevice, c->st));
+    HIPCHK(launch_scatmat(src, Tu, nang, c->obs.d_ang, c->obs.d_ang + nang, c->rimed, c->obs.d_sm, c->st));
+    HIPCHK(hipMemcpyAsync(sm, c->obs.d_sm, sizeof(double) * 16 * nang, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->obs.scatmat_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+int mstm_hip_asymmetry(mstm_hip_ctx *c, int T, const double *amnp0, double *s00, double *g)
+{
+    if (!c) return fail(c, MSTM_HIP_EARG, "asymmetry: null context");
+    if (!s00 && !g) return fail(c, MSTM_HIP_EARG, "asymmetry: no output");
+    HIPCHK(hipSetDevice(c->device));
+    auto t0 = std::chrono::steady_clock::now();
+    const double *src = nullptr;
+    int Tu = 0;
+    if (int rc = observe_source(c, T, amnp0, &src, &Tu)) return rc;
+    if (c->obs.cg_T != Tu) {
+        dfree(c->obs.d_cg);
+        dfree(c->obs.d_mom);
+        c->obs.cg_T = 0;
+        const int W = 2 * Tu + 1;
+        std::vector<double> h((size_t)(Tu + 1) * 3 * W * 2 + (size_t)(Tu + 1) * 3 * 2, 0.0);
+        double *cgm = h.data(), *cg1 = h.data() + (size_t)(Tu + 1) * 3 * W * 2;
+        for (int n = 1; n <= Tu; n++)
+            for (int li = 0; li < 3; li++) {
+                const int l = n - 1 + li;
+                if (l < 1 || l > Tu) continue;
+                for (int w = 0; w <= 1; w++) {
+                    cg1[(n * 3 + li) * 2 + w] = (double)clebsch(n, -1, l, 1, w);
+                    for (int m = -std::min(n, l); m <= std::min(n, l); m++)
+                        cgm[((size_t)(n * 3 + li) * W + m + Tu) * 2 + w] = (double)clebsch(n, -m, l, m, w);
+                }
+            }
+        HIPCHK(dalloc(&c->obs.d_cg, sizeof(double) * h.size()));
+        HIPCHK(hipMemcpy(c->obs.d_cg, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+        HIPCHK(dalloc(&c->obs.d_mom, sizeof(double) * 2 * s00_moment_blocks(Tu)));
+        c->obs.cg_T = Tu;
+    }
+    const int nb = s00_moment_blocks(Tu);
+    const double *cgm = c->obs.d_cg, *cg1 = c->obs.d_cg + (size_t)(Tu + 1) * 3 * (2 * Tu + 1) * 2;
+    HIPCHK(launch_s00_moments(src, Tu, cgm, cg1, c->obs.d_mom, c->st));
+    std::vector<double> part(2 * nb);
+    HIPCHK(hipMemcpyAsync(part.data(), c->obs.d_mom, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    double s[2] = {0, 0};
+    for (int b = 0; b < nb; b++) {
+        s[0] += part[2 * b];
+        s[1] += part[2 * b + 1];
+    }
+    // sign patch and scale of fosmexpansion (mstm-scatprops-26.f90:1930-1948): s00(1,1) keeps its sign
+    for (int w = 0; w < 2; w++) s[w] *= 0.5 / c->rimed;
+    if (s00) {
+        s00[0] = s[0];
+        s00[1] = s[1];
+    }
+    if (g) *g = s[1] / (3.0 * s[0]);
+    c->obs.moments_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
 }
 
 }  // extern "C"

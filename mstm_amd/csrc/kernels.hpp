@@ -57,3 +57,13 @@ hipError_t launch_multi_zdot(const double *V, int64_t stride, int nvec, const do
                              double *out, hipStream_t st);
 hipError_t launch_multi_axpy(const double *V, int64_t stride, int nvec, const double *h, double sign, double *w, int n,
                              hipStream_t st);
+
+// observe.hip (complex FP64 throughout)
+hipError_t launch_merge(const double *amnp, int64_t ldc, const int *order, const int *offset, const int *torder,
+                        const double *pos, int nsph, int T, int LS, const double *r0, double rimed, const double *vccm,
+                        double *part, int G, double *a0, hipStream_t st);
+hipError_t launch_scatmat(const double *a0, int T, int nang, const double *ct, const double *phi, double rimed,
+                          double *sm, hipStream_t st);
+int s00_moment_blocks(int T);
+hipError_t launch_s00_moments(const double *a0, int T, const double *cgm, const double *cg1, double *out,
+                              hipStream_t st);

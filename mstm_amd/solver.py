@@ -22,6 +22,23 @@ def _i(a):
     return a.ctypes.data_as(C.POINTER(C.c_int))
 
 
+# the reference's pi is a real(4) literal widened to real(8) (mstm-modules-33.f90:18)
+PI_R4 = float(np.float32(np.pi))
+
+
+def scattering_angles(alpha: float = 0.0):
+    """Default scattering-matrix directions of a single-origin run without
+    azimuthal averaging (mstm-input-37.f90:1339-1351 and 3036-3046): theta =
+    -180..180 deg in 1 deg steps, cos theta = +-1 moved to +-0.9999999, and
+    negative theta in the half-plane phi = alpha + pi.  Returns (theta_deg,
+    cos theta, phi)."""
+    th = np.arange(-180, 181)
+    ct = np.cos(PI_R4 / 180.0 * th.astype(np.float64))
+    ct[ct == 1.0] = 0.9999999
+    ct[ct == -1.0] = -0.9999999
+    return th, ct, np.where(th < 0, alpha + PI_R4, alpha)
+
+
 class HipContext:
     """One GPU context holding one cluster (spheredata/mie module state)."""
 
@@ -133,6 +150,66 @@ class HipContext:
                                               _d(amnp.view(np.float64)), _d(qeff), _d(maxerr), _i(maxiter),
                                               _i(istat)), "fixedorsoln")
         return dict(amnp=amnp, qeff=qeff, maxerr=float(maxerr[0]), iterations=int(maxiter[0]), istat=int(istat[0]))
+
+    # -- post-solve observables of a single-origin expansion ------------------
+    def translation_orders(self, origin=None, eps: float = 1e-5, maxt: int = 100):
+        """(T, torder): translation orders about origin (default: the mean
+        sphere position) as main_calling_program sets them
+        (mstm-input-37.f90:1159-1197, tranordertest)."""
+        t = np.zeros(self.nsphere, np.int32)
+        T = np.zeros(1, np.int32)
+        o = None if origin is None else np.ascontiguousarray(origin, np.float64)
+        self._chk(self.L.mstm_hip_translation_orders(self.h, None if o is None else _d(o), float(eps), int(maxt),
+                                                     _i(t), _i(T)), "translation_orders")
+        self.t_matrix_order = int(T[0])
+        return self.t_matrix_order, t
+
+    def merge_to_common_origin(self, amnp=None, copy: bool = True):
+        """merge_to_common_origin (mstm-scatprops-26.f90:447-559) of both
+        columns of a solution (None: the last fixedorsoln solution, still on
+        the device).  Returns the E/M common-origin coefficients as a
+        (2 columns, 2, T(T+2)) complex array, or None with copy=False (they
+        stay on the device for scattering_matrix / asymmetry)."""
+        T = self.t_matrix_order
+        a = None if amnp is None else np.ascontiguousarray(amnp, np.complex128)
+        out = np.zeros((2, 2, T * (T + 2)), np.complex128) if copy else None
+        self._chk(self.L.mstm_hip_merge_to_common_origin(
+            self.h, None if a is None else _d(a.view(np.float64)),
+            None if out is None else _d(out.view(np.float64))), "merge_to_common_origin")
+        return out
+
+    def scattering_matrix(self, ct, phi, a0=None, T: int = 0):
+        """scatteringmatrix (mstm-scatprops-26.f90:1498-1573) at each
+        (cos theta, phi): (nang, 16) Mueller elements, Fortran sm(4,4)
+        column-major per direction (S11 = [:, 0], S12 = [:, 4])."""
+        ct = np.ascontiguousarray(np.atleast_1d(ct), np.float64)
+        phi = np.ascontiguousarray(np.broadcast_to(np.atleast_1d(phi), ct.shape), np.float64)
+        out = np.zeros((len(ct), 16))
+        a = None if a0 is None else np.ascontiguousarray(a0, np.complex128)
+        self._chk(self.L.mstm_hip_scattering_matrix(self.h, int(T), None if a is None else _d(a.view(np.float64)),
+                                                    len(ct), _d(ct), _d(phi), _d(out)), "scattering_matrix")
+        return out
+
+    def asymmetry(self, a0=None, T: int = 0):
+        """(s00(1,1,0:1), g): first two Legendre moments of S11 of
+        fosmexpansion and g = s00(1,1,1) / (3 s00(1,1,0))."""
+        s = np.zeros(2)
+        g = np.zeros(1)
+        a = None if a0 is None else np.ascontiguousarray(a0, np.complex128)
+        self._chk(self.L.mstm_hip_asymmetry(self.h, int(T), None if a is None else _d(a.view(np.float64)), _d(s),
+                                            _d(g)), "asymmetry")
+        return s, float(g[0])
+
+    def observables(self, amnp=None, alpha: float = 0.0, eps: float = 1e-5, maxt: int = 100, copy: bool = False):
+        """The single-origin post-processing of a solved cluster on the GPU:
+        translation orders, merge, scattering matrix on the default grid
+        (theta = -180..180 deg, scattering_angles) and g."""
+        T, tord = self.translation_orders(None, eps, maxt)
+        a0 = self.merge_to_common_origin(amnp, copy=copy)
+        th, ct, phi = scattering_angles(alpha)
+        sm = self.scattering_matrix(ct, phi)
+        s00, g = self.asymmetry()
+        return dict(T=T, torder=tord, amnp0=a0, theta=th, scatmat=sm, s00=s00, g=g)
 
     # -- learned initial guess (no reference counterpart) ---------------------
     def set_learned(self, params=None, hidden: int = 128, nlayers: int = 3, lnet: int = 4):
