@@ -365,7 +365,8 @@ def main():
     traffic = None
     pmc = os.path.join(HERE, "profiles", "tile_fp32_cfg5_pmc.json")
     if os.path.exists(pmc):
-        traffic = json.load(open(pmc)).get("hbm_bytes_per_launch")
+        rec = json.load(open(pmc)).get("by_columns", {}).get(str(ncol))
+        traffic = rec["hbm_bytes_per_launch"] if rec else None
     roof = dict(kernel=f"direct_tile_kernel<float,{L},{ncol}>", bound="hbm", achieved=round(ach32, 1),
                 peak=HBM_PEAK_GBPS, unit="GB/s", frac=round(ach32 / HBM_PEAK_GBPS, 4), traffic=traffic,
                 algorithmic_bytes=alg32, columns=ncol, avg_launch_ms=round(t32, 4),

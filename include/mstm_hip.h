@@ -126,6 +126,11 @@ int mstm_hip_set_inner_solver(mstm_hip_ctx *ctx, int kind);
 int mstm_hip_translation_orders(mstm_hip_ctx *ctx, const double *origin, double eps, int maxt, int *torder,
                                 int *tmo);
 
+/* The same state from a host that computed translation_order, t_matrix_order
+ * and cluster_origin itself (mstm-input-37.f90:1159-1197): torder holds
+ * nsphere orders (1..127), T the common-origin order (1..127). */
+int mstm_hip_set_translation_orders(mstm_hip_ctx *ctx, const double *origin, const int *torder, int T);
+
 /* merge_to_common_origin (mstm-scatprops-26.f90:447-559) of both columns of
  * a solution: each sphere's expansion is carried to r0 by the regular
  * rotate-axial-rotate translation of order min(T, torder_i) and summed.

@@ -1712,6 +1712,25 @@ int mstm_hip_translation_orders(mstm_hip_ctx *c, const double *origin, double ep
     return 0;
 }
 
+int mstm_hip_set_translation_orders(mstm_hip_ctx *c, const double *origin, const int *torder, int T)
+{
+    if (!c || !c->nsph) return fail(c, MSTM_HIP_ESTATE, "set_translation_orders: no spheres");
+    if (!origin || !torder) return fail(c, MSTM_HIP_EARG, "set_translation_orders: null argument");
+    if (T < 1 || T > 127) return fail(c, MSTM_HIP_EUNSUP, "set_translation_orders: T %d outside 1..127", T);
+    for (int i = 0; i < c->nsph; i++)
+        if (torder[i] < 1 || torder[i] > 127)
+            return fail(c, MSTM_HIP_EARG, "set_translation_orders: order %d of sphere %d", torder[i], i + 1);
+    HIPCHK(hipSetDevice(c->device));
+    c->obs.torder.assign(torder, torder + c->nsph);
+    dfree(c->obs.d_torder);
+    HIPCHK(dalloc(&c->obs.d_torder, sizeof(int) * c->nsph));
+    HIPCHK(hipMemcpy(c->obs.d_torder, torder, sizeof(int) * c->nsph, hipMemcpyHostToDevice));
+    for (int d = 0; d < 3; d++) c->obs.r0[d] = origin[d];
+    c->obs.T = T;
+    c->obs.a0_T = 0;
+    return 0;
+}
+
 int mstm_hip_merge_to_common_origin(mstm_hip_ctx *c, const double *amnp, double *amnp0)
 {
     if (!c || !c->nsph) return fail(c, MSTM_HIP_ESTATE, "merge: no spheres");

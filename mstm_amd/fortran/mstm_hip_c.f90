@@ -96,6 +96,35 @@
          real(c_float) :: params(*)
          integer(c_int), value :: nparams,hidden,nlayers,lnet
          end function mstm_hip_set_learned
+         integer(c_int) function mstm_hip_set_translation_orders(ctx,origin,torder,t) &
+            bind(c,name='mstm_hip_set_translation_orders')
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double) :: origin(3)
+         integer(c_int) :: torder(*)
+         integer(c_int), value :: t
+         end function mstm_hip_set_translation_orders
+         integer(c_int) function mstm_hip_merge_to_common_origin(ctx,amnp,amnp0) &
+            bind(c,name='mstm_hip_merge_to_common_origin')
+         import :: c_ptr, c_int, c_double_complex
+         type(c_ptr), value :: ctx
+         complex(c_double_complex) :: amnp(*),amnp0(*)
+         end function mstm_hip_merge_to_common_origin
+         integer(c_int) function mstm_hip_scattering_matrix(ctx,t,amnp0,nang,ct,phi,sm) &
+            bind(c,name='mstm_hip_scattering_matrix')
+         import :: c_ptr, c_int, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: t,nang
+         complex(c_double_complex) :: amnp0(*)
+         real(c_double) :: ct(*),phi(*),sm(*)
+         end function mstm_hip_scattering_matrix
+         integer(c_int) function mstm_hip_asymmetry(ctx,t,amnp0,s00,g) bind(c,name='mstm_hip_asymmetry')
+         import :: c_ptr, c_int, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: t
+         complex(c_double_complex) :: amnp0(*)
+         real(c_double) :: s00(2),g
+         end function mstm_hip_asymmetry
          integer(c_int) function mstm_hip_clear(ctx,clear_tables) bind(c,name='mstm_hip_clear')
          import :: c_ptr, c_int
          type(c_ptr), value :: ctx

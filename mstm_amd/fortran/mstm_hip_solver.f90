@@ -8,6 +8,11 @@
 !   hip_cbicg               <- cbicg               (mstm-solver-8.f90:654-655)
 !   hip_fixedorsoln         <- fixedorsoln         (mstm-solver-8.f90:350-352)
 !   hip_sphereinteraction   <- sphereinteraction   (mstm-scatprops-26.f90:25-27)
+!   hip_merge_to_common_origin <- the two merge_to_common_origin calls of
+!                              main_calling_program (mstm-input-37.f90:1510-1515)
+!   hip_scatteringmatrix    <- the scatteringmatrix loop of
+!                              scattering_matrix_calculation (mstm-input-37.f90:3036-3062)
+!   hip_asymmetry           <- s00(1,1,0:1) of fosmexpansion (mstm-scatprops-26.f90:1820)
 ! The GPU context is created on first use and refilled when
 ! initialize_solver / initial_run is true (geometry, orders and an_mie are
 ! taken from the MSTM modules, so the Mie data are the host's own).
@@ -187,5 +192,49 @@
          call hip_check(mstm_hip_sphereinteraction(hip_ctx,int(nrhs,c_int),ain,aout,ct,mm,ufft, &
             int(hip_precision,c_int)),'sphereinteraction')
          end subroutine hip_sphereinteraction
+!
+! merge_to_common_origin (mstm-scatprops-26.f90:447-559) of both
+! polarizations in one call, with the host's translation_order (spheredata)
+! and the common-origin order nodrt (t_matrix_order)
+!
+         subroutine hip_merge_to_common_origin(nodrt,amnp,amnp0,origin_position)
+         integer :: nodrt
+         real(8), optional :: origin_position(3)
+         complex(8) :: amnp(number_eqns,2),amnp0(2*nodrt*(nodrt+2),2)
+         real(c_double) :: r0(3)
+         integer(c_int), allocatable :: tord(:)
+         if(.not.hip_loaded) call hip_load_state()
+         r0=0.d0
+         if(present(origin_position)) r0=origin_position
+         allocate(tord(number_spheres))
+         tord=translation_order(1:number_spheres)
+         call hip_check(mstm_hip_set_translation_orders(hip_ctx,r0,tord,int(nodrt,c_int)), &
+            'set_translation_orders')
+         call hip_check(mstm_hip_merge_to_common_origin(hip_ctx,amnp,amnp0),'merge_to_common_origin')
+         deallocate(tord)
+         end subroutine hip_merge_to_common_origin
+!
+! scatteringmatrix (mstm-scatprops-26.f90:1498-1573; rotate_plane = .false.,
+! normalize_s11 = .false.) at nang directions (ct, phi): sm(16,nang)
+!
+         subroutine hip_scatteringmatrix(amn0,nodrt,nang,ct,phi,sm)
+         integer :: nodrt,nang
+         real(8) :: ct(nang),phi(nang),sm(16,nang)
+         complex(8) :: amn0(0:nodrt+1,nodrt,2,2)
+         if(.not.hip_loaded) call hip_load_state()
+         call hip_check(mstm_hip_scattering_matrix(hip_ctx,int(nodrt,c_int),amn0,int(nang,c_int),ct,phi,sm), &
+            'scattering_matrix')
+         end subroutine hip_scatteringmatrix
+!
+! s00(1,1,0:1) of fosmexpansion (mstm-scatprops-26.f90:1820-1951) and the
+! asymmetry parameter g = s00(1,1,1)/(3 s00(1,1,0))
+!
+         subroutine hip_asymmetry(amn0,nodrt,s00,g)
+         integer :: nodrt
+         real(8) :: s00(0:1),g
+         complex(8) :: amn0(0:nodrt+1,nodrt,2,2)
+         if(.not.hip_loaded) call hip_load_state()
+         call hip_check(mstm_hip_asymmetry(hip_ctx,int(nodrt,c_int),amn0,s00,g),'asymmetry')
+         end subroutine hip_asymmetry
 
       end module mstm_hip_solver

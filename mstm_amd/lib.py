@@ -46,6 +46,7 @@ _SIGNATURES = {
                                        _dp, _dp, _dp, _ip, _ip]),
     "mstm_hip_clear": (C.c_int, [_vp, C.c_int]),
     "mstm_hip_translation_orders": (C.c_int, [_vp, _dp, C.c_double, C.c_int, _ip, _ip]),
+    "mstm_hip_set_translation_orders": (C.c_int, [_vp, _dp, _ip, C.c_int]),
     "mstm_hip_merge_to_common_origin": (C.c_int, [_vp, _dp, _dp]),
     "mstm_hip_scattering_matrix": (C.c_int, [_vp, C.c_int, _dp, C.c_int, _dp, _dp, _dp]),
     "mstm_hip_asymmetry": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),

@@ -164,6 +164,13 @@ class HipContext:
         self.t_matrix_order = int(T[0])
         return self.t_matrix_order, t
 
+    def set_translation_orders(self, origin, torder, T: int):
+        """Use a host's own translation_order / t_matrix_order / cluster_origin."""
+        o = np.ascontiguousarray(origin, np.float64)
+        t = np.ascontiguousarray(torder, np.int32)
+        self._chk(self.L.mstm_hip_set_translation_orders(self.h, _d(o), _i(t), int(T)), "set_translation_orders")
+        self.t_matrix_order = int(T)
+
     def merge_to_common_origin(self, amnp=None, copy: bool = True):
         """merge_to_common_origin (mstm-scatprops-26.f90:447-559) of both
         columns of a solution (None: the last fixedorsoln solution, still on

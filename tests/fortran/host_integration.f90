@@ -79,9 +79,58 @@
       d1=maxval(abs(x_r-x_g))/maxval(abs(x_r))
       write(*,'(a,2i6,es12.4)') 'cbicg iterations(ref,gpu) rel_err: ',iref,igpu,d1
       if(iref.ne.igpu.or.d1.gt.1.d-10) bad=1
+      ! single-origin observables of the reference solution: orders about the
+      ! mean position as main_calling_program sets them, then the reference
+      ! merge / scatteringmatrix / fosmexpansion against the hip_* forms
+      call observables_check()
       if(bad.ne.0) then
          write(*,'(a)') 'HOST INTEGRATION FAILED'
          call exit(1)
       endif
       write(*,'(a)') 'HOST INTEGRATION OK'
+      contains
+         subroutine observables_check()
+         integer :: j,nt
+         real(8) :: r0(3),rt,ct(361),ph(361),gg,s2(0:1),e1,e2,e3
+         real(8), allocatable :: sm_r(:,:),sm_g(:,:),s00(:,:,:),s02(:,:,:),sp22(:,:,:),sm22(:,:,:)
+         complex(8) :: rimed(2),sa(4)
+         complex(8), allocatable :: a0_r(:,:),a0_g(:,:)
+         r0=0.d0
+         do j=1,n
+            r0=r0+sphere_position(:,j)
+         enddo
+         r0=r0/dble(n)
+         if(allocated(translation_order)) deallocate(translation_order)
+         allocate(translation_order(n))
+         t_matrix_order=min(max_mie_order,max_t_matrix_order)
+         do j=1,n
+            call exteriorrefindex(j,rimed)
+            rt=sqrt(sum((sphere_position(:,j)-r0(:))**2))
+            call tranordertest(rt,rimed(1),sphere_order(j),1.d-5,translation_order(j))
+            translation_order(j)=min(translation_order(j),max_t_matrix_order)
+            t_matrix_order=max(t_matrix_order,translation_order(j))
+         enddo
+         nt=t_matrix_order
+         allocate(a0_r(2*nt*(nt+2),2),a0_g(2*nt*(nt+2),2),sm_r(16,361),sm_g(16,361))
+         do j=1,2
+            call merge_to_common_origin(nt,amnp_r(:,j),a0_r(:,j),origin_position=r0,merge_procs=.true.)
+         enddo
+         call hip_merge_to_common_origin(nt,amnp_r,a0_g,origin_position=r0)
+         e1=maxval(abs(a0_r-a0_g))/maxval(abs(a0_r))
+         do j=1,361
+            ct(j)=cos(pi/180.d0*dble(j-181))
+            if(ct(j).eq.1.d0) ct(j)=0.9999999d0
+            ph(j)=0.d0
+            if(j.lt.181) ph(j)=pi
+            call scatteringmatrix(a0_r,nt,ct(j),ph(j),sa,sm_r(:,j),rotate_plane=.false.,normalize_s11=.false.)
+         enddo
+         call hip_scatteringmatrix(a0_r,nt,361,ct,ph,sm_g)
+         e2=maxval(abs(sm_r-sm_g))/maxval(abs(sm_r))
+         allocate(s00(4,4,0:2*nt),s02(4,4,0:2*nt),sp22(4,4,0:2*nt),sm22(4,4,0:2*nt))
+         call fosmexpansion(nt,a0_r,s00,s02,sp22,sm22)
+         call hip_asymmetry(a0_r,nt,s2,gg)
+         e3=max(abs(s2(0)-s00(1,1,0))/abs(s00(1,1,0)),abs(s2(1)-s00(1,1,1))/abs(s00(1,1,0)))
+         write(*,'(a,i4,3es12.4,f10.6)') 'observables T rel_err(merge,scatmat,s00) g: ',nt,e1,e2,e3,gg
+         if(e1.gt.1.d-12.or.e2.gt.1.d-12.or.e3.gt.1.d-12) bad=1
+         end subroutine observables_check
       end program host_integration
