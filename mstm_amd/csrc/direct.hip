@@ -14,7 +14,15 @@
 //     tile, so it accumulates in registers and writes one partial sum per
 //     (target, partner block): no atomics, deterministic order;
 //   * pair tables are laid out in HBM in exactly this consumption order and
-//     interleaved over the 16 slots, so each table load is 16 x 16 B.
+//     interleaved over the 16 slots, so each table load is 16 x 16 B;
+//   * when a step's table group (16 slots) fits in 64 KB of LDS, all waves
+//     of the tile copy it there once per step and read it from LDS: every
+//     table byte then leaves the L2 once per tile instead of once per lane
+//     that needs it (4 lanes x 2 or 4 waves);
+//   * the back rotation sums d^t(m,k) z_m(t) over the m blocks of a step
+//     and applies the output phase exp(-i sigma k phi) once per
+//     coefficient (with LDS-staged tables, whose freed registers hold the
+//     extra accumulators); larger orders fold the phase into every term.
 // The reduce kernel sums the partials and applies the Mie/con_tran post-op
 // of sphereinteraction (mstm-scatprops-26.f90:158-168).
 #include <utility>
@@ -63,15 +71,73 @@ __device__ __forceinline__ void load_sec(const T *__restrict__ p0, T *r)
     }
 }
 
+// LDS staging of a step's table group (see the file header)
+template <typename T, int L> constexpr bool tile_staged()
+{
+    return (int64_t)slot_len(L, L, 16 / (int)sizeof(T)) * MSTM_SLOTS * (int64_t)sizeof(T) <= 65536;
+}
+// two-column FP32 tiles up to order 4 also software-pipeline the staging:
+// step s+1's group is loaded into registers while step s computes
+template <typename T, int L, int NR> constexpr bool tile_pipelined() { return sizeof(T) == 4 && L <= 4 && NR == 2; }
 // occupancy target: FP32 tiles up to order 4 fit 2 waves/SIMD without
-// spilling; larger states use the full register file (AGPRs) instead
-template <typename T, int L> constexpr int tile_waves() { return (sizeof(T) == 4 && L <= 4) ? 2 : 1; }
+// spilling (the pipelined form holds its prefetch in the full register file
+// at 1 wave/SIMD instead); larger states use the full register file
+template <typename T, int L, int NR> constexpr int tile_waves()
+{
+    return tile_pipelined<T, L, NR>() ? 1 : ((sizeof(T) == 4 && L <= 4) ? 2 : 1);
+}
+// register prefetch of a table group for the pipelined form
+template <typename T, int L, int NR> struct GroupPrefetch {
+    static constexpr int VW = 16 / sizeof(T);
+    static constexpr int NV = slot_len(L, L, VW) * MSTM_SLOTS / VW;
+    static constexpr int NT = 64 * NR;
+    static constexpr int PF = tile_pipelined<T, L, NR>() ? (NV + NT - 1) / NT : 1;
+    typedef typename vec_of<T, VW>::type V;
+    V r[PF];
+    __device__ __forceinline__ void load(const T *__restrict__ tab, int tile, int s)
+    {
+        const V *g = reinterpret_cast<const V *>(tab + ((int64_t)tile * MSTM_SLOTS + s) * (int64_t)slot_len(L, L, VW) *
+                                                           MSTM_SLOTS);
+#pragma unroll
+        for (int q = 0; q < PF; q++) {
+            const int i = threadIdx.x + q * NT;
+            if (i < NV) r[q] = g[i];
+        }
+    }
+    __device__ __forceinline__ void store(T *__restrict__ grp) const
+    {
+        V *d = reinterpret_cast<V *>(grp);
+#pragma unroll
+        for (int q = 0; q < PF; q++) {
+            const int i = threadIdx.x + q * NT;
+            if (i < NV) d[i] = r[q];
+        }
+    }
+};
+template <typename T, int L> constexpr int tile_lds_elems()
+{
+    return tile_staged<T, L>() ? slot_len(L, L, 16 / (int)sizeof(T)) * MSTM_SLOTS : 1;
+}
+// copy table group s of a tile into LDS, all threads of the workgroup
+template <typename T, int L, int NR>
+__device__ __forceinline__ void stage_group(const T *__restrict__ tab, int tile, int s, T *__restrict__ grp)
+{
+    constexpr int VW = 16 / sizeof(T);
+    constexpr int NV = slot_len(L, L, VW) * MSTM_SLOTS / VW;
+    typedef typename vec_of<T, VW>::type V;
+    const V *g = reinterpret_cast<const V *>(tab + ((int64_t)tile * MSTM_SLOTS + s) * (int64_t)slot_len(L, L, VW) *
+                                                       MSTM_SLOTS);
+    V *d = reinterpret_cast<V *>(grp);
+    for (int q = threadIdx.x; q < NV; q += 64 * NR) d[q] = g[q];
+}
 // One wave's work on a tile; DIR = 0 forward (I <- J), 1 reverse (J <- I);
 // the wave serves right-hand sides 2*rpair and 2*rpair+1 of NR.
 template <typename T, int L, int DIR, int NR>
 __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, int tile, const cplx<T> *__restrict__ xin,
-                                          cplx<T> *__restrict__ part, int npad, int rpair)
+                                          cplx<T> *__restrict__ part, int npad, int rpair, T *__restrict__ grp)
 {
+    constexpr bool STAGED = tile_staged<T, L>();
+    constexpr bool PIPE = tile_pipelined<T, L, NR>();
     constexpr int VW = 16 / sizeof(T);
     constexpr int NB = L * (L + 2);
     constexpr int SL = slot_len(L, L, VW);
@@ -85,13 +151,11 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
 #pragma unroll
     for (int q = 0; q < NB; q++) W[q] = cmk((T)0, (T)0);
 
+    GroupPrefetch<T, L, NR> pre;
+    if constexpr (PIPE) pre.load(tab, tile, 0);
+
 #pragma unroll 1
     for (int s = 0; s < S; s++) {
-        // all waves of the tile read the same table group in step s: keep them
-        // in step so that the group is fetched from HBM once and the other
-        // waves hit in L2
-        __syncthreads();
-        if (diag && s == 0) continue;
         int slot, src;
         if (!DIR) {
             slot = kslot;
@@ -100,7 +164,30 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
             slot = (kslot - s + S) & (S - 1);
             src = bt.x * S + slot;
         }
-        const T *p0 = tab + ((int64_t)tile * S + s) * (int64_t)SL * S + slot * VW;
+        const cplx<T> *xs = xin + ((int64_t)(rhs * npad + src) * 2 + mode) * NB;
+        // all waves of the tile read the same table group in step s: keep them
+        // in step so that the group is fetched from HBM once (and, staged,
+        // copied to LDS once)
+        __syncthreads();
+        cplx<T> xv[PIPE ? NB : 1];
+        if constexpr (PIPE) {
+            pre.store(grp);
+            // this step's source coefficients are loaded before the next
+            // group's prefetch, so that waiting for them never waits for it
+            if (!(diag && s == 0)) {
+#pragma unroll
+                for (int n = 1; n <= L; n++)
+#pragma unroll
+                    for (int k = -n; k <= n; k++) xv[nk(n, k)] = xs[amn(DIR ? -k : k, n, L)];
+            }
+            if (s + 1 < S) pre.load(tab, tile, s + 1);
+            __syncthreads();
+        } else if constexpr (STAGED) {
+            stage_group<T, L, NR>(tab, tile, s, grp);
+            __syncthreads();
+        }
+        if (diag && s == 0) continue;
+        const T *p0 = STAGED ? grp + slot * VW : tab + ((int64_t)tile * S + s) * (int64_t)SL * S + slot * VW;
         // exp(i sigma k phi), sigma = +1 forward, -1 reverse (im of :6690)
         T pv[2];
         load_sec<T, phi_off(L, L, VW), 2>(p0, pv);
@@ -113,20 +200,21 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
             ph[L - k] = cconj(ph[L + k]);
         }
         // source coefficients: m-flip for the reverse direction, then phase
-        const cplx<T> *xs = xin + ((int64_t)(rhs * npad + src) * 2 + mode) * NB;
         cplx<T> xp[NB];
 #pragma unroll
         for (int n = 1; n <= L; n++)
 #pragma unroll
             for (int k = -n; k <= n; k++) {
                 cplx<T> v;
-                if (!DIR) v = xs[amn(k, n, L)];
-                else {
-                    v = xs[amn(-k, n, L)];
-                    if (k & 1) v = cmk(-v.re, -v.im);
-                }
+                if constexpr (PIPE) v = xv[nk(n, k)];
+                else v = xs[amn(DIR ? -k : k, n, L)];
+                if (DIR && (k & 1)) v = cmk(-v.re, -v.im);
                 xp[nk(n, k)] = cmul(ph[L + k], v);
             }
+        // this step's back-rotated sums before the output phase (staged form)
+        cplx<T> U[STAGED ? NB : 1];
+#pragma unroll
+        for (int q = 0; q < (STAGED ? NB : 1); q++) U[q] = cmk((T)0, (T)0);
         static_for<2 * L + 1>([&](auto bc) {
             constexpr int b = decltype(bc)::value;
             constexpr int m = mblock_m(b);
@@ -165,8 +253,8 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
                 }
                 z[t] = acc;
             }
-            // back rotation with the output phase folded in:
-            // W(t,k) += d^t(m,k) exp(-i sigma k phi) z(t)
+            // back rotation: staged, U(t,k) += d^t(m,k) z(t); otherwise with
+            // the output phase folded in, W(t,k) += d^t(m,k) exp(-i sigma k phi) z(t)
 #pragma unroll
             for (int t = m1; t <= L; t++) {
                 int ro = 0;
@@ -174,11 +262,22 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
                 for (int q = m1; q < t; q++) ro += 2 * q + 1;
 #pragma unroll
                 for (int k = -t; k <= t; k++) {
-                    cplx<T> f = cscale(ph[L - k], dr[ro + k + t]);
-                    cfma(W[nk(t, k)], f, z[t - m1]);
+                    if constexpr (STAGED) {
+                        crfma(U[nk(t, k)], dr[ro + k + t], z[t - m1]);
+                    } else {
+                        cplx<T> f = cscale(ph[L - k], dr[ro + k + t]);
+                        cfma(W[nk(t, k)], f, z[t - m1]);
+                    }
                 }
             }
         });
+        if constexpr (STAGED) {
+            // output phase once per coefficient: W(t,k) += exp(-i sigma k phi) U(t,k)
+#pragma unroll
+            for (int t = 1; t <= L; t++)
+#pragma unroll
+                for (int k = -t; k <= t; k++) cfma(W[nk(t, k)], ph[L - k], U[nk(t, k)]);
+        }
     }
     // partial for (target, partner block); m-flip back for the reverse wave
     const int partner = DIR ? bt.x : bt.y;
@@ -202,17 +301,33 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
 // sides 0-1 and waves 2/3 sides 2-3 from the same table stream, so every
 // table byte read from HBM feeds four columns.
 template <typename T, int L, int NR>
-__global__ __launch_bounds__(64 * NR) __attribute__((amdgpu_waves_per_eu(tile_waves<T, L>()))) void direct_tile_kernel(
+__global__ __launch_bounds__(64 * NR) __attribute__((amdgpu_waves_per_eu(tile_waves<T, L, NR>()))) void direct_tile_kernel(
     const T *__restrict__ tab, const int2 *__restrict__ tiles, const cplx<T> *__restrict__ xin,
     cplx<T> *__restrict__ part, int npad)
 {
     const int tile = blockIdx.x;
     const int2 bt = tiles[tile];
     const int w = threadIdx.x >> 6;
-    if (!(w & 1)) tile_body<T, L, 0, NR>(tab, bt, tile, xin, part, npad, w >> 1);
-    else if (bt.x != bt.y) tile_body<T, L, 1, NR>(tab, bt, tile, xin, part, npad, w >> 1);
+    __shared__ __attribute__((aligned(16))) T grp[tile_lds_elems<T, L>()];
+    if (!(w & 1)) tile_body<T, L, 0, NR>(tab, bt, tile, xin, part, npad, w >> 1, grp);
+    else if (bt.x != bt.y) tile_body<T, L, 1, NR>(tab, bt, tile, xin, part, npad, w >> 1, grp);
     else
-        for (int s = 0; s < MSTM_SLOTS; s++) __syncthreads();  // idle reverse wave of a diagonal tile: same barriers
+    {
+        // idle reverse wave of a diagonal tile: its share of the staging and the same barriers
+        GroupPrefetch<T, L, NR> pre;
+        if constexpr (tile_pipelined<T, L, NR>()) pre.load(tab, tile, 0);
+        for (int s = 0; s < MSTM_SLOTS; s++) {
+            __syncthreads();
+            if constexpr (tile_pipelined<T, L, NR>()) {
+                pre.store(grp);
+                if (s + 1 < MSTM_SLOTS) pre.load(tab, tile, s + 1);
+                __syncthreads();
+            } else if constexpr (tile_staged<T, L>()) {
+                stage_group<T, L, NR>(tab, tile, s, grp);
+                __syncthreads();
+            }
+        }
+    }
 }
 
 template <typename T>

@@ -197,19 +197,27 @@ __global__ __launch_bounds__(KB) void multi_zdot_kernel(const cplx<double> *__re
             if (threadIdx.x < h) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + h]);
             __syncthreads();
         }
-        if (threadIdx.x == 0) partials[(int64_t)blockIdx.x * nvec + j] = red[0];
+        if (threadIdx.x == 0) partials[(int64_t)j * gridDim.x + blockIdx.x] = red[0];
         __syncthreads();
     }
 }
 
-__global__ void reduce_cols_kernel(const cplx<double> *__restrict__ partials, int nblk, int nvec,
-                                   cplx<double> *__restrict__ out)
+// out[j] = sum over the nblk block partials of vector j (stored j-major);
+// one workgroup per vector, fixed reduction order
+__global__ __launch_bounds__(KB) void reduce_cols_kernel(const cplx<double> *__restrict__ partials, int nblk,
+                                                         cplx<double> *__restrict__ out)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nvec) return;
+    __shared__ cplx<double> red[KB];
+    const int j = blockIdx.x;
     cplx<double> s = cmk(0.0, 0.0);
-    for (int b = 0; b < nblk; b++) s = cadd(s, partials[(int64_t)b * nvec + j]);
-    out[j] = s;
+    for (int b = threadIdx.x; b < nblk; b += KB) s = cadd(s, partials[(int64_t)j * nblk + b]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = KB / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] = red[0];
 }
 
 __global__ void multi_axpy_kernel(const cplx<double> *__restrict__ V, int64_t stride, int nvec,
@@ -235,9 +243,8 @@ hipError_t launch_multi_zdot(const double *V, int64_t stride, int nvec, const do
     hipLaunchKernelGGL(multi_zdot_kernel, dim3(KGRID), dim3(KB), 0, st, reinterpret_cast<const cplx<double> *>(V),
                        stride / 2, nvec, reinterpret_cast<const cplx<double> *>(w), n,
                        reinterpret_cast<cplx<double> *>(partials));
-    hipLaunchKernelGGL(reduce_cols_kernel, dim3((nvec + 63) / 64), dim3(64), 0, st,
-                       reinterpret_cast<const cplx<double> *>(partials), KGRID, nvec,
-                       reinterpret_cast<cplx<double> *>(out));
+    hipLaunchKernelGGL(reduce_cols_kernel, dim3(nvec), dim3(KB), 0, st, reinterpret_cast<const cplx<double> *>(partials),
+                       KGRID, reinterpret_cast<cplx<double> *>(out));
     return hipGetLastError();
 }
 

@@ -47,6 +47,22 @@ def test_direct_matvec_fp64(gpu_ctx, case):
     assert rel(h[0], g["hout"][0]) < FP64_TOL
 
 
+@pytest.mark.parametrize("case,prec", [("frac_agg", 0), ("frac_agg", 1), ("fft_cyl250", 0), ("fft_cyl250", 1),
+                                       ("cfg1_direct", 0), ("cfg1_direct", 1)])
+def test_direct_matvec_four_columns(gpu_ctx, case, prec):
+    """Four product columns in one launch (both polarizations' A p and A^H w,
+    the BiCG form; orders 3, 4, 5 in FP64 and FP32) against the reference's
+    two-column products."""
+    g = golden(case)
+    load_cluster(gpu_ctx, g)
+    ref = g["aout_direct"] if "aout_direct" in g else g["aout"]
+    ain4 = np.concatenate([g["ain"], g["ain"][::-1]])
+    out = gpu_ctx.sphereinteraction(ain4, [False, True, True, False], [True] * 4, precision=prec)
+    tol = FP64_TOL if prec == 0 else FP32_TOL
+    for r, rr in enumerate([0, 1, 1, 0]):
+        assert rel(out[r], ref[rr]) < tol, (case, prec, r)
+
+
 @pytest.mark.parametrize("case", ["cfg1_direct", "frac_agg"])
 def test_direct_matvec_fp32(gpu_ctx, case):
     g = golden(case)
