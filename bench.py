@@ -188,11 +188,29 @@ def bench_fft(args):
 
     for _ in range(args.warmup):
         solve_all(SOLVE_MIXED, True)
+    ctx.clear(clear_tables=False)  # counters only
     times = []
     for _ in range(args.steps):
         t, q_mix, it_mix = solve_all(SOLVE_MIXED, True)
         times.append(t)
     t_mix = float(np.mean(times))
+    st = ctx.stats()
+    # the spectral contraction streams the node-kernel spectrum once per
+    # two-column product: K (2 modes x nblk^2 x M complex FP32) + the
+    # transformed node fields in and out (2 columns x 2 modes x nblk x M)
+    M = 8 * int(np.prod(grid["cell_dim"]))
+    LG = int(grid["node_order"])
+    nblk = LG * (LG + 2)
+    nc = max(1.0, st.get("fft_contract_launches_fp32", 0.0))
+    t_c = st.get("fft_contract_ms_fp32", 0.0) / nc
+    alg_c = 2 * nblk * nblk * M * 8 + 2 * (2 * 2 * nblk * M * 8)
+    roof = None
+    if t_c > 0:
+        ach = alg_c / (t_c * 1e-3) / 1e9
+        roof = dict(kernel="contract_kernel<float> (node-kernel spectrum x node fields, per frequency)", bound="hbm",
+                    achieved=round(ach, 1), peak=HBM_PEAK_GBPS, unit="GB/s", frac=round(ach / HBM_PEAK_GBPS, 4),
+                    traffic=None, algorithmic_bytes=alg_c, avg_launch_ms=round(t_c, 4), launches=int(nc),
+                    flops_per_launch=2 * 2 * nblk * nblk * M * 8)
     t_plain, q_plain, it_plain = solve_all(SOLVE_PLAIN, True)
     ctx.setup_direct(FP64)
     t_dir, q_dir, it_dir = solve_all(SOLVE_PLAIN, False)
@@ -208,7 +226,7 @@ def bench_fft(args):
            "q_error": {"mixed_fft_vs_plain_fft": rel(q_mix[:, 0, :2], q_plain[:, 0, :2]),
                        "mixed_fft_vs_direct": rel(q_mix[:, 0, :2], q_dir[:, 0, :2]),
                        "plain_fft_vs_direct": rel(q_plain[:, 0, :2], q_dir[:, 0, :2])},
-           "roofline": None, "cpu_baseline": None})
+           "roofline": roof, "cpu_baseline": None})
     ctx.close()
 
 

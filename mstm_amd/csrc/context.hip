@@ -1511,6 +1511,7 @@ int mstm_hip_clear(mstm_hip_ctx *c, int clear_tables)
         if (c->fft) fft_clear(c->fft);
     }
     c->stats = Stats();
+    if (c->fft) fft_reset_stats(c->fft);
     return 0;
 }
 
@@ -1520,7 +1521,8 @@ const char *mstm_hip_stats_names(void)
            "matvecs_fp64,matvecs_fp32,solve_ms,iters_fp64,iters_fp32,refinements,table_bytes_fp64,table_bytes_fp32,"
            "fft_setup_ms,fft_conv_ms,fft_launches,npad,ntiles,nblocks,lig_solves,lig_fallbacks,lig_ms,lig_err0,"
            "lig_errb,fgmres_steps,tile4_ms_fp64,tile4_ms_fp32,tile4_launches_fp64,tile4_launches_fp32,inner_solver,"
-           "obs_orders_ms,obs_merge_ms,obs_scatmat_ms,obs_moments_ms,obs_T";
+           "obs_orders_ms,obs_merge_ms,obs_scatmat_ms,obs_moments_ms,obs_T,fft_contract_ms_fp32,"
+           "fft_contract_launches_fp32,fft_contract_ms_fp64,fft_contract_launches_fp64";
 }
 
 int mstm_hip_stats(const mstm_hip_ctx *c, double *out, int nout)
@@ -1561,7 +1563,11 @@ int mstm_hip_stats(const mstm_hip_ctx *c, double *out, int nout)
                   c->obs.merge_ms,
                   c->obs.scatmat_ms,
                   c->obs.moments_ms,
-                  (double)c->obs.T};
+                  (double)c->obs.T,
+                  c->fft ? fft_stat(c->fft, 3) : 0.0,
+                  c->fft ? fft_stat(c->fft, 4) : 0.0,
+                  c->fft ? fft_stat(c->fft, 5) : 0.0,
+                  c->fft ? fft_stat(c->fft, 6) : 0.0};
     int k = std::min(nout, (int)(sizeof v / sizeof v[0]));
     for (int q = 0; q < k; q++) out[q] = v[q];
     return k;

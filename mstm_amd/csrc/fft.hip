@@ -542,7 +542,27 @@ struct FftOp {
          *out[2] = {};
     double setup_ms = 0, conv_ms = 0;
     long launches = 0;
+    // HIP events around the spectral contraction, per precision (the stage
+    // whose HBM rate the benchmark reports)
+    std::vector<hipEvent_t> cev;
+    std::vector<int> cev_prec;
+    size_t cev_used = 0;
+    double contract_ms[2] = {0, 0};
+    long contract_launches[2] = {0, 0};
 };
+
+static void collect_contract(FftOp *f)
+{
+    for (size_t i = 0; i + 1 < f->cev_used; i += 2) {
+        float ms = 0.f;
+        if (hipEventSynchronize(f->cev[i + 1]) == hipSuccess &&
+            hipEventElapsedTime(&ms, f->cev[i], f->cev[i + 1]) == hipSuccess) {
+            f->contract_ms[f->cev_prec[i / 2]] += ms;
+            f->contract_launches[f->cev_prec[i / 2]]++;
+        }
+    }
+    f->cev_used = 0;
+}
 
 static int ferr(FftOp *f, int code, const char *msg)
 {
@@ -688,10 +708,29 @@ void fft_bind(FftOp *f, const FftGeometry &g)
 
 int fft_required_vcc(const FftOp *f) { return f ? std::max(f->LG, f->Lp) : 0; }
 const char *fft_last_error(const FftOp *f) { return f ? f->err.c_str() : "no FFT operator"; }
-double fft_stat(const FftOp *f, int which)
+double fft_stat(FftOp *f, int which)
 {
     if (!f) return 0.0;
-    return which == 0 ? f->setup_ms : which == 1 ? f->conv_ms : (double)f->launches;
+    if (which >= 3) collect_contract(f);
+    switch (which) {
+    case 0: return f->setup_ms;
+    case 1: return f->conv_ms;
+    case 2: return (double)f->launches;
+    case 3: return f->contract_ms[1];
+    case 4: return (double)f->contract_launches[1];
+    case 5: return f->contract_ms[0];
+    case 6: return (double)f->contract_launches[0];
+    default: return 0.0;
+    }
+}
+
+void fft_reset_stats(FftOp *f)
+{
+    if (!f) return;
+    collect_contract(f);
+    f->contract_ms[0] = f->contract_ms[1] = 0.0;
+    f->contract_launches[0] = f->contract_launches[1] = 0;
+    f->launches = 0;
 }
 
 static void free_prec(FftOp *f, int p)
@@ -720,6 +759,8 @@ void fft_clear(FftOp *f)
 void fft_destroy(FftOp *f)
 {
     if (!f) return;
+    collect_contract(f);
+    for (auto &e : f->cev) hipEventDestroy(e);
     fft_clear(f);
     ffree(f->d_cstart);
     ffree(f->d_cmem);
@@ -988,8 +1029,19 @@ template <typename T> static int matvec_t(FftOp *f, int nrhs, const double *ain,
     FCHK(fft_axis<T>(B, A, batch, d2, d3, 0, 2 * N.x, N.x, 2 * N.x, -1, st));
     // per-frequency contraction with the node kernel spectrum
     dim3 cg_((f->M + 127) / 128, 2, (nblk + 7) / 8);
+    if (f->cev_used + 2 > 512) collect_contract(f);
+    while (f->cev.size() < f->cev_used + 2) {
+        hipEvent_t e;
+        FCHK(hipEventCreate(&e));
+        f->cev.push_back(e);
+    }
+    f->cev_prec.resize(f->cev.size() / 2 + 1);
+    f->cev_prec[f->cev_used / 2] = prec;
+    FCHK(hipEventRecord(f->cev[f->cev_used], st));
     hipLaunchKernelGGL(contract_kernel<T>, cg_, dim3(128), 0, st, f->K, A, B, f->M, nblk);
     FCHK(hipGetLastError());
+    FCHK(hipEventRecord(f->cev[f->cev_used + 1], st));
+    f->cev_used += 2;
     // inverse: x, y, z keeping the first N per axis
     FCHK(fft_axis<T>(B, A, batch, d3, d2, 0, 2 * N.x, 2 * N.x, N.x, 1, st));
     FCHK(fft_axis<T>(A, B, batch, d2, d1, 1, 2 * N.y, 2 * N.y, N.y, 1, st));
