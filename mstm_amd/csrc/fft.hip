@@ -32,90 +32,130 @@
 // opposite phase).  Source element (rhs r, mode p, coefficient e) is read at
 // src + src_off[q] + r*rs + p*ms + e*es; the result for slot q is written
 // contiguously to dst[q][r][p][NB(target)].
+// One 128-thread workgroup per slot serves its four columns (r, p); each
+// stage is spread over the threads and kept in LDS (dynamic size, from the
+// orders):
+//   xp(n,k)  = e^{i k phi} a(k,n)                          n <= Ls
+//   y_m(n)   = sum_k d^n_{m,k} xp(n,k)                      (rotation)
+//   z_m(t)   = sum_s A_m(t,s) y_m(s)                        (axial)
+//   W(t,k)   = e^{-i k phi} sum_m d^t_{m,k} z_m(t)          (back rotation)
 // ===========================================================================
+#define LT_THREADS 128
+
 template <typename T, int DIR>
-__global__ __launch_bounds__(64) void list_translate_kernel(const T *__restrict__ tab, int nslots, int LT, int LS,
-                                                            const cplx<T> *__restrict__ src,
-                                                            const int64_t *__restrict__ src_off, int64_t rs,
-                                                            int64_t ms, int64_t es,
-                                                            const unsigned char *__restrict__ ident,
-                                                            cplx<T> *__restrict__ dst)
+__global__ __launch_bounds__(LT_THREADS) void list_translate_kernel(const T *__restrict__ tab, int nslots, int LT,
+                                                                    int LS, const cplx<T> *__restrict__ src,
+                                                                    const int64_t *__restrict__ src_off, int64_t rs,
+                                                                    int64_t ms, int64_t es,
+                                                                    const unsigned char *__restrict__ ident,
+                                                                    cplx<T> *__restrict__ dst)
 {
     constexpr int VW = 16 / sizeof(T);
-    const int q = blockIdx.x * MSTM_SLOTS + (threadIdx.x >> 2);
+    const int q = blockIdx.x, tid = threadIdx.x;
     if (q >= nslots) return;
-    const int r = (threadIdx.x >> 1) & 1, p = threadIdx.x & 1;
     const int Ls = DIR ? LT : LS, Lt = DIR ? LS : LT;
-    const int NBT = Lt * (Lt + 2);
+    const int NBS = Ls * (Ls + 2), NBT = Lt * (Lt + 2);
+    const int nmin = imn(LT, LS), nmax = imx(LT, LS), NM = 2 * nmin + 1;
     const int SL = slot_len(LT, LS, VW);
     const T *p0 = tab + (int64_t)(q / MSTM_SLOTS) * SL * MSTM_SLOTS + (q % MSTM_SLOTS) * VW;
     auto E = [&](int e) -> T { return p0[(int64_t)(e / VW) * MSTM_SLOTS * VW + e % VW]; };
-    const cplx<T> *xs = src + src_off[q] + r * rs + p * ms;
-    cplx<T> *out = dst + ((int64_t)q * 4 + r * 2 + p) * NBT;
+    const cplx<T> *xs = src + src_off[q];
+    cplx<T> *out = dst + (int64_t)q * 4 * NBT;
+    auto srcv = [&](int c, int n, int k) -> cplx<T> {
+        return xs[(c >> 1) * rs + (c & 1) * ms + (int64_t)amn(k, n, Ls) * es];
+    };
     if (ident[q]) {  // zero translation of regular waves: truncated copy (mtransfer)
-        for (int t = 1; t <= Lt; t++)
-            for (int k = -t; k <= t; k++)
-                out[amn(k, t, Lt)] = (t <= Ls) ? xs[(int64_t)amn(k, t, Ls) * es] : cmk((T)0, (T)0);
+        for (int e = tid; e < 4 * NBT; e += LT_THREADS) {
+            const int c = e / NBT, w = e - c * NBT;
+            int t = (int)sqrt((double)(w + 1));
+            while (t * (t + 2) < w + 1) t++;
+            while ((t - 1) * (t + 1) >= w + 1) t--;
+            const int k = w + 1 - t * (t + 1);
+            out[(int64_t)c * NBT + amn(k, t, Lt)] = (t <= Ls) ? srcv(c, t, k) : cmk((T)0, (T)0);
+        }
         return;
     }
-    const int nmin = imn(LT, LS), nmax = imx(LT, LS);
-    const int po = phi_off(LT, LS, VW);
-    cplx<T> e1 = cmk(E(po), DIR ? -E(po + 1) : E(po + 1));
-    cplx<T> ph[2 * LMAX_LIST + 1];
-    ph[LMAX_LIST] = cmk((T)1, (T)0);
-    for (int k = 1; k <= nmax; k++) {
-        ph[LMAX_LIST + k] = cmul(ph[LMAX_LIST + k - 1], e1);
-        ph[LMAX_LIST - k] = cconj(ph[LMAX_LIST + k]);
+    extern __shared__ __attribute__((aligned(16))) unsigned char lt_smem[];
+    cplx<T> *xp = reinterpret_cast<cplx<T> *>(lt_smem);  // [c][nk(n,k)], NBS each
+    cplx<T> *ys = xp + 4 * NBS;                          // [c][m+nmin][n], n = 0..Ls
+    cplx<T> *zs = ys + 4 * NM * (Ls + 1);                // [c][m+nmin][t], t = 0..Lt
+    cplx<T> *ph = zs + 4 * NM * (Lt + 1);                // e^{i k phi}, k = -nmax..nmax
+    const cplx<T> e1 = cmk(E(phi_off(LT, LS, VW)), DIR ? -E(phi_off(LT, LS, VW) + 1) : E(phi_off(LT, LS, VW) + 1));
+    for (int k = tid - nmax; k <= nmax; k += LT_THREADS) {
+        cplx<T> r = cmk((T)1, (T)0), b = k < 0 ? cconj(e1) : e1;
+        for (int u = 0; u < iab(k); u++) r = cmul(r, b);
+        ph[k + nmax] = r;
     }
-    cplx<T> xp[LMAX_LIST * (LMAX_LIST + 2)], W[LMAX_LIST * (LMAX_LIST + 2)];
-    for (int n = 1; n <= Ls; n++)
-        for (int k = -n; k <= n; k++) {
-            cplx<T> v;
-            if (!DIR) v = xs[(int64_t)amn(k, n, Ls) * es];
-            else {
-                v = xs[(int64_t)amn(-k, n, Ls) * es];
-                if (k & 1) v = cmk(-v.re, -v.im);
-            }
-            xp[nk(n, k)] = cmul(ph[LMAX_LIST + k], v);
+    __syncthreads();
+    // phase-shifted source, m-flipped for the reverse direction
+    for (int e = tid; e < 4 * NBS; e += LT_THREADS) {
+        const int c = e / NBS, w = e - c * NBS;
+        int n = (int)sqrt((double)(w + 1));
+        while (n * (n + 2) < w + 1) n++;
+        while ((n - 1) * (n + 1) >= w + 1) n--;
+        const int k = w + 1 - n * (n + 1);
+        cplx<T> v;
+        if (!DIR) v = srcv(c, n, k);
+        else {
+            v = srcv(c, n, -k);
+            if (k & 1) v = cmk(-v.re, -v.im);
         }
-    for (int e = 0; e < NBT; e++) W[e] = cmk((T)0, (T)0);
-    cplx<T> y[LMAX_LIST + 1], z[LMAX_LIST + 1];
-    for (int b = 0; b < 2 * nmin + 1; b++) {
-        const int m = mblock_m(b), m1 = imx(1, iab(m));
-        const int e0 = sec_off(b, LT, LS, VW);
-        for (int n = m1; n <= Ls; n++) {
-            int ro = 0;
-            for (int qq = m1; qq < n; qq++) ro += 2 * qq + 1;
-            cplx<T> acc = cmk((T)0, (T)0);
-            for (int k = -n; k <= n; k++) crfma(acc, E(e0 + ro + k + n), xp[nk(n, k)]);
-            y[n] = acc;
-        }
-        const int ea = e0 + pad_to(rot_len(m, LT, LS), VW) + p * pad_to(ax_len(m, LT, LS), VW);
+        xp[c * NBS + w] = cmul(ph[k + nmax], v);
+    }
+    __syncthreads();
+    // rotation rows of block m start at sec_off(b); degree n at + sum_{q=m1}^{n-1} (2q+1) = n^2 - m1^2
+    auto bofm = [](int m) { return m == 0 ? 0 : (m < 0 ? 2 * (-m) - 1 : 2 * m); };
+    // forward rotation y_m(n), n = m1..Ls
+    for (int e = tid; e < 4 * NM * Ls; e += LT_THREADS) {
+        const int c = e / (NM * Ls), rem = e - c * NM * Ls, mi = rem / Ls, n = rem - mi * Ls + 1;
+        const int m = mi - nmin, m1 = imx(1, iab(m));
+        if (n < m1) continue;
+        const int e0 = sec_off(bofm(m), LT, LS, VW) + n * n - m1 * m1;
+        cplx<T> acc = cmk((T)0, (T)0);
+        for (int k = -n; k <= n; k++) crfma(acc, E(e0 + k + n), xp[c * NBS + nk(n, k)]);
+        ys[(c * NM + mi) * (Ls + 1) + n] = acc;
+    }
+    __syncthreads();
+    // axial z_m(t), t = m1..Lt (mode-specific matrix, transposed for DIR = 1)
+    for (int e = tid; e < 4 * NM * Lt; e += LT_THREADS) {
+        const int c = e / (NM * Lt), rem = e - c * NM * Lt, mi = rem / Lt, t = rem - mi * Lt + 1;
+        const int m = mi - nmin, m1 = imx(1, iab(m)), p = c & 1;
+        if (t < m1) continue;
+        const int ea = sec_off(bofm(m), LT, LS, VW) + pad_to(rot_len(m, LT, LS), VW) + p * pad_to(ax_len(m, LT, LS), VW);
         const int NR = LT - m1 + 1;
-        for (int t = m1; t <= Lt; t++) {
-            cplx<T> acc = cmk((T)0, (T)0);
-            for (int s = m1; s <= Ls; s++) {
-                int e = DIR ? ea + 2 * ((s - m1) + NR * (t - m1)) : ea + 2 * ((t - m1) + NR * (s - m1));
-                cfma(acc, cmk(E(e), E(e + 1)), y[s]);
-            }
-            z[t] = acc;
+        cplx<T> acc = cmk((T)0, (T)0);
+        for (int s = m1; s <= Ls; s++) {
+            const int ex = DIR ? ea + 2 * ((s - m1) + NR * (t - m1)) : ea + 2 * ((t - m1) + NR * (s - m1));
+            cfma(acc, cmk(E(ex), E(ex + 1)), ys[(c * NM + mi) * (Ls + 1) + s]);
         }
-        for (int t = m1; t <= Lt; t++) {
-            int ro = 0;
-            for (int qq = m1; qq < t; qq++) ro += 2 * qq + 1;
-            for (int k = -t; k <= t; k++) cfma(W[nk(t, k)], cscale(ph[LMAX_LIST - k], E(e0 + ro + k + t)), z[t]);
-        }
+        zs[(c * NM + mi) * (Lt + 1) + t] = acc;
     }
-    for (int t = 1; t <= Lt; t++)
-        for (int k = -t; k <= t; k++) {
-            cplx<T> v;
-            if (!DIR) v = W[nk(t, k)];
-            else {
-                v = W[nk(t, -k)];
-                if (k & 1) v = cmk(-v.re, -v.im);
-            }
-            out[amn(k, t, Lt)] = v;
+    __syncthreads();
+    // back rotation with the output phase; m-flip back for the reverse direction
+    for (int e = tid; e < 4 * NBT; e += LT_THREADS) {
+        const int c = e / NBT, w = e - c * NBT;
+        int t = (int)sqrt((double)(w + 1));
+        while (t * (t + 2) < w + 1) t++;
+        while ((t - 1) * (t + 1) >= w + 1) t--;
+        const int k = w + 1 - t * (t + 1);
+        const int kk = DIR ? -k : k;  // W(t, kk) lands at (k, t)
+        cplx<T> acc = cmk((T)0, (T)0);
+        const int mm = imn(t, nmin);
+        for (int m = -mm; m <= mm; m++) {
+            const int m1 = imx(1, iab(m));
+            const int e0 = sec_off(bofm(m), LT, LS, VW) + t * t - m1 * m1;
+            crfma(acc, E(e0 + kk + t), zs[(c * NM + m + nmin) * (Lt + 1) + t]);
         }
+        cplx<T> v = cmul(ph[-kk + nmax], acc);
+        if (DIR && (k & 1)) v = cmk(-v.re, -v.im);
+        out[(int64_t)c * NBT + amn(k, t, Lt)] = v;
+    }
+}
+
+template <typename T> static size_t list_translate_lds(int LT, int LS)
+{
+    const int nmin = imn(LT, LS), nmax = imx(LT, LS), NM = 2 * nmin + 1, L1 = imx(LT, LS) + 1;
+    return sizeof(cplx<T>) * (4 * (size_t)nmax * (nmax + 2) + 2 * 4 * (size_t)NM * L1 + (2 * nmax + 1));
 }
 
 // anode[r][p][n][cell] = sum over spheres of the cell of asph[i][r][p][n]
@@ -167,32 +207,57 @@ struct FftPlan {
     int n, nstage, radix[16];
 };
 
-template <typename T>
-__device__ __forceinline__ void dft_small(cplx<T> *v, int R, int sign)
+// one radix-R butterfly of a Stockham stage (R = 2..5), twiddled inputs;
+// R is a compile-time constant so the work arrays stay in registers
+template <typename T, int R>
+__device__ __forceinline__ void stage_radix(const cplx<T> *__restrict__ A, cplx<T> *__restrict__ B, int p, int jj,
+                                            int n, int nr, int Ns, int sign)
 {
-    cplx<T> o[5];
-    for (int k = 0; k < R; k++) {
-        cplx<T> s = cmk((T)0, (T)0);
-        for (int r = 0; r < R; r++) {
+    const int kk = jj % Ns;
+    cplx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        cplx<T> x = A[p * n + jj + r * nr];
+        if (kk != 0 && r != 0) {
             T sn, cs;
-            int kr = (k * r) % R;
             if (sizeof(T) == 8) {
-                double a = 2.0 * sign * (double)kr / (double)R;
                 double sd, cd;
-                sincospi(a, &sd, &cd);
+                sincospi(2.0 * sign * (double)(kk * r) / (double)(Ns * R), &sd, &cd);
                 sn = (T)sd;
                 cs = (T)cd;
             } else {
-                float a = 2.0f * sign * (float)kr / (float)R, sf, cf;
-                sincospif(a, &sf, &cf);
+                float sf, cf;
+                sincospif(2.0f * sign * (float)(kk * r) / (float)(Ns * R), &sf, &cf);
+                sn = (T)sf;
+                cs = (T)cf;
+            }
+            x = cmul(x, cmk(cs, sn));
+        }
+        v[r] = x;
+    }
+    const int idxD = (jj / Ns) * Ns * R + kk;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        cplx<T> s = cmk((T)0, (T)0);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            T sn, cs;
+            const int kr = (k * r) % R;
+            if (sizeof(T) == 8) {
+                double sd, cd;
+                sincospi(2.0 * sign * (double)kr / (double)R, &sd, &cd);
+                sn = (T)sd;
+                cs = (T)cd;
+            } else {
+                float sf, cf;
+                sincospif(2.0f * sign * (float)kr / (float)R, &sf, &cf);
                 sn = (T)sf;
                 cs = (T)cf;
             }
             cfma(s, cmk(cs, sn), v[r]);
         }
-        o[k] = s;
+        B[p * n + idxD + k * Ns] = s;
     }
-    for (int k = 0; k < R; k++) v[k] = o[k];
 }
 
 template <typename T>
@@ -240,31 +305,13 @@ __global__ void fft_pass_kernel(const cplx<T> *__restrict__ in, cplx<T> *__restr
         const int R = plan.radix[s];
         const int nr = n / R;
         for (int w = tid; w < P * nr; w += bs) {
-            int p = w / nr, jj = w % nr;
-            int kk = jj % Ns;
-            cplx<T> v[5];
-            for (int r = 0; r < R; r++) {
-                cplx<T> x = A[p * n + jj + r * nr];
-                if (kk != 0 && r != 0) {
-                    T sn, cs;
-                    if (sizeof(T) == 8) {
-                        double sd, cd;
-                        sincospi(2.0 * sign * (double)(kk * r) / (double)(Ns * R), &sd, &cd);
-                        sn = (T)sd;
-                        cs = (T)cd;
-                    } else {
-                        float sf, cf;
-                        sincospif(2.0f * sign * (float)(kk * r) / (float)(Ns * R), &sf, &cf);
-                        sn = (T)sf;
-                        cs = (T)cf;
-                    }
-                    x = cmul(x, cmk(cs, sn));
-                }
-                v[r] = x;
+            const int p = w / nr, jj = w % nr;
+            switch (R) {
+            case 2: stage_radix<T, 2>(A, B, p, jj, n, nr, Ns, sign); break;
+            case 3: stage_radix<T, 3>(A, B, p, jj, n, nr, Ns, sign); break;
+            case 4: stage_radix<T, 4>(A, B, p, jj, n, nr, Ns, sign); break;
+            default: stage_radix<T, 5>(A, B, p, jj, n, nr, Ns, sign); break;
             }
-            dft_small(v, R, sign);
-            int idxD = (jj / Ns) * Ns * R + kk;
-            for (int r = 0; r < R; r++) B[p * n + idxD + r * Ns] = v[r];
         }
         __syncthreads();
         cplx<T> *t = A;
@@ -365,9 +412,13 @@ __global__ void contract_kernel(const cplx<float> *__restrict__ K, const cplx<T>
             }
         }
     }
-    for (int l = 0; l < lend; l++) {
-        g[((int64_t)(0 * 2 + p) * nblk + l0 + l) * M + k] = acc[0][l];
-        g[((int64_t)(1 * 2 + p) * nblk + l0 + l) * M + k] = acc[1][l];
+    // unrolled with a guard so that acc stays in registers
+#pragma unroll
+    for (int l = 0; l < LC; l++) {
+        if (l < lend) {
+            g[((int64_t)(0 * 2 + p) * nblk + l0 + l) * M + k] = acc[0][l];
+            g[((int64_t)(1 * 2 + p) * nblk + l0 + l) * M + k] = acc[1][l];
+        }
     }
 }
 
@@ -1012,8 +1063,9 @@ template <typename T> static int matvec_t(FftOp *f, int nrhs, const double *ain,
                         f->g.d_an, st));
     const int64_t rs_sph = (int64_t)f->npad * 2 * NBP;
     // sphere -> node (J translation, :573-626)
-    dim3 lg((n + MSTM_SLOTS - 1) / MSTM_SLOTS), lb(64);
-    hipLaunchKernelGGL((list_translate_kernel<T, 0>), lg, lb, 0, st, jt, n, LG, Lp, xin, f->d_off_sph, rs_sph,
+    dim3 lg(n), lb(LT_THREADS);
+    const size_t ljb = list_translate_lds<T>(LG, Lp), lhb = list_translate_lds<T>(Lp, Lp);
+    hipLaunchKernelGGL((list_translate_kernel<T, 0>), lg, lb, ljb, st, jt, n, LG, Lp, xin, f->d_off_sph, rs_sph,
                        (int64_t)NBP, (int64_t)1, f->d_ident_j, asph);
     FCHK(hipGetLastError());
     int64_t nn4 = (int64_t)4 * nblk * f->ncells;
@@ -1048,16 +1100,16 @@ template <typename T> static int matvec_t(FftOp *f, int nrhs, const double *ain,
     FCHK(fft_axis<T>(B, A, batch, d1, d0, 2, 2 * N.z, 2 * N.z, N.z, 1, st));
     // node -> sphere (reverse J translation) from gnode[r][p][l][cell] in A
     const int64_t nc = f->ncells;
-    hipLaunchKernelGGL((list_translate_kernel<T, 1>), lg, lb, 0, st, jt, n, LG, Lp, A, f->d_off_node, 2 * nblk * nc,
+    hipLaunchKernelGGL((list_translate_kernel<T, 1>), lg, lb, ljb, st, jt, n, LG, Lp, A, f->d_off_node, 2 * nblk * nc,
                        nblk * nc, nc, f->d_ident_j, far);
     FCHK(hipGetLastError());
     // near-field pairs, both directions (local_sphere_to_sphere_expansion)
     if (f->npairs) {
-        dim3 pg((f->npairs + MSTM_SLOTS - 1) / MSTM_SLOTS);
-        hipLaunchKernelGGL((list_translate_kernel<T, 0>), pg, lb, 0, st, ht, f->npairs, Lp, Lp, xin, f->d_off_pj,
+        dim3 pg(f->npairs);
+        hipLaunchKernelGGL((list_translate_kernel<T, 0>), pg, lb, lhb, st, ht, f->npairs, Lp, Lp, xin, f->d_off_pj,
                            rs_sph, (int64_t)NBP, (int64_t)1, f->d_ident_h, nf);
         FCHK(hipGetLastError());
-        hipLaunchKernelGGL((list_translate_kernel<T, 1>), pg, lb, 0, st, ht, f->npairs, Lp, Lp, xin, f->d_off_pi,
+        hipLaunchKernelGGL((list_translate_kernel<T, 1>), pg, lb, lhb, st, ht, f->npairs, Lp, Lp, xin, f->d_off_pi,
                            rs_sph, (int64_t)NBP, (int64_t)1, f->d_ident_h, nr);
         FCHK(hipGetLastError());
     }
