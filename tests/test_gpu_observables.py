@@ -106,3 +106,27 @@ def test_observables_5000_sphere_cluster_match_oracle(gpu_ctx):
     ring = gpu_ctx.scattering_matrix(CT.ravel(), PH.ravel())[:, 0].reshape(CT.shape).mean(axis=1)
     g_quad = float(np.sum(w * x * ring) / np.sum(w * ring))
     assert abs(ob["g"] - g_quad) < 1e-10
+
+
+def test_sphere_at_the_common_origin(gpu_ctx):
+    """A symmetric cluster whose centre sphere sits exactly at the mean
+    position: its expansion is carried over by the zero-translation copy
+    (mtransfer) in both the GPU merge and the oracle."""
+    from mstm_amd.solver import scattering_angles
+    from oracle import oracle as O
+
+    d = 4.5
+    pos = np.array([[0.0, 0.0, 0.0], [d, 0, 0], [-d, 0, 0], [0, d, 0], [0, -d, 0], [0, 0, d], [0, 0, -d]])
+    radius = np.full(len(pos), 1.5)
+    ri = np.full(len(pos), 1.45 + 0.02j)
+    gpu_ctx.set_spheres(pos, radius, ri, 1.0, 1e-6)
+    r = gpu_ctx.fixedorsoln(eps=1e-10, niter=10000)
+    T, tord = gpu_ctx.translation_orders()
+    oc = O.OracleCluster(pos, radius, ri)
+    To, to = oc.translation_orders(np.zeros(3), 1e-5, 100)
+    assert T == To and np.array_equal(tord, to) and tord[0] == gpu_ctx.sphere_order[0]
+    a0 = gpu_ctx.merge_to_common_origin(None)
+    ref = oc.merge(T, to, np.zeros(3), r["amnp"])
+    assert rel(a0, ref) < TOL
+    th, ct, phi = scattering_angles(0.0)
+    assert rel(gpu_ctx.scattering_matrix(ct, phi), O.scatmat(T, ref, ct, phi)) < TOL
