@@ -20,6 +20,11 @@ Cases
                 x = 0.5, m = 1.75+0.4i: matvec + solve
   sparse100     100 spheres x=1, m=1.5+0.01i, seeded RSA in a sphere at
                 fv 0.002 (common-origin order 49): solve + observables only
+  poly30        30 spheres of mixed size (x = 0.4 .. 2.4, orders 2..6) on
+                seeded RSA centres (placed at x = 2.4, fv 0.10, then shrunk,
+                so they stay disjoint), m = 1.5+0.02i: direct matvec, bare H,
+                solve, observables; and the same cluster through the FFT
+                operator (poly30_fft)
 Every solve also runs the single-origin post-processing (translation orders,
 merge_to_common_origin, scatteringmatrix on the default grid,
 fosmexpansion); those outputs are stored under obs_* keys.
@@ -102,6 +107,10 @@ def main():
     case("frac_agg", agg.pos * 0.5, agg.radius * 0.5, np.full(agg.n, 1.75 + 0.4j), seed=14)
     c = G.rsa_in_sphere(100, 1.0, 0.002, seed=7, ri=1.5 + 0.01j)
     observe_case("sparse100", c.pos, c.radius, c.ri)
+    c = G.rsa_in_sphere(30, 2.4, 0.10, seed=8, ri=1.5 + 0.02j)
+    radius = np.round(np.random.default_rng(8).uniform(0.4, 2.4, c.n), 6)
+    case("poly30", c.pos, radius, c.ri, seed=15)
+    case("poly30_fft", c.pos, radius, c.ri, fft=True, seed=16)
 
 
 if __name__ == "__main__":

@@ -24,7 +24,7 @@ def load_cluster(ctx, g):
     ctx.set_spheres(g["pos"], g["radius"], g["ri"], 1.0, 1e-6)
 
 
-@pytest.mark.parametrize("case", ["cfg1_direct", "dense_l2", "frac_agg"])
+@pytest.mark.parametrize("case", ["cfg1_direct", "dense_l2", "frac_agg", "poly30"])
 def test_mie_orders_and_coefficients(gpu_ctx, case):
     g = golden(case)
     load_cluster(gpu_ctx, g)
@@ -34,7 +34,7 @@ def test_mie_orders_and_coefficients(gpu_ctx, case):
     assert rel(gpu_ctx.an_mie(), g["anmie"]) < 1e-12
 
 
-@pytest.mark.parametrize("case", ["cfg1_direct", "dense_l2", "frac_agg"])
+@pytest.mark.parametrize("case", ["cfg1_direct", "dense_l2", "frac_agg", "poly30"])
 def test_direct_matvec_fp64(gpu_ctx, case):
     g = golden(case)
     load_cluster(gpu_ctx, g)
@@ -48,7 +48,7 @@ def test_direct_matvec_fp64(gpu_ctx, case):
 
 
 @pytest.mark.parametrize("case,prec", [("frac_agg", 0), ("frac_agg", 1), ("fft_cyl250", 0), ("fft_cyl250", 1),
-                                       ("cfg1_direct", 0), ("cfg1_direct", 1)])
+                                       ("cfg1_direct", 0), ("cfg1_direct", 1), ("poly30", 0), ("poly30", 1)])
 def test_direct_matvec_four_columns(gpu_ctx, case, prec):
     """Four product columns in one launch (both polarizations' A p and A^H w,
     the BiCG form; orders 3, 4, 5 in FP64 and FP32) against the reference's
@@ -63,7 +63,7 @@ def test_direct_matvec_four_columns(gpu_ctx, case, prec):
         assert rel(out[r], ref[rr]) < tol, (case, prec, r)
 
 
-@pytest.mark.parametrize("case", ["cfg1_direct", "frac_agg"])
+@pytest.mark.parametrize("case", ["cfg1_direct", "frac_agg", "poly30"])
 def test_direct_matvec_fp32(gpu_ctx, case):
     g = golden(case)
     load_cluster(gpu_ctx, g)
@@ -74,7 +74,7 @@ def test_direct_matvec_fp32(gpu_ctx, case):
         assert rel(out[r], g["aout"][r]) < FP32_TOL, (case, r)
 
 
-@pytest.mark.parametrize("case", ["cfg1_direct", "frac_agg"])
+@pytest.mark.parametrize("case", ["cfg1_direct", "frac_agg", "poly30"])
 def test_fixedorsoln_plain_matches_reference(gpu_ctx, case):
     g = golden(case)
     load_cluster(gpu_ctx, g)

@@ -1,6 +1,6 @@
 """GPU parity of the FFT node-grid operator against the reference's FFT
-path (golden fft_cyl250: the first 250 spheres of the reference fixture
-examples/random_in_cylinder_1000.pos).
+path (goldens fft_cyl250: the first 250 spheres of the reference fixture
+examples/random_in_cylinder_1000.pos; poly30_fft: 30 spheres of orders 3..6).
 
 The reference keeps the node-kernel spectrum in complex(4)
 (fft_translation-5.f90:31), so FFT-path results are compared at the FP32
@@ -18,9 +18,9 @@ def rel(a, b):
     return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(np.asarray(b)).max())
 
 
-@pytest.fixture(scope="module")
-def fft_case(gpu_ctx):
-    g = golden("fft_cyl250")
+@pytest.fixture(scope="module", params=["fft_cyl250", "poly30_fft"])
+def fft_case(gpu_ctx, request):
+    g = golden(request.param)
     gpu_ctx.set_spheres(g["pos"], g["radius"], g["ri"], 1.0, 1e-6)
     info = gpu_ctx.node_selection(0.0, -1, 2)
     return g, info

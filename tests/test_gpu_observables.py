@@ -16,7 +16,7 @@ from conftest import golden
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["cfg1_direct", "dense_l2", "fft_cyl250", "frac_agg", "sparse100"]
+CASES = ["cfg1_direct", "dense_l2", "fft_cyl250", "frac_agg", "sparse100", "poly30", "poly30_fft"]
 TOL = 1e-12
 
 

@@ -226,7 +226,7 @@ def test_bicg_equals_dense_solve():
 
 
 # ------------------------------------------------------------ reference goldens
-@pytest.mark.parametrize("case", ["cfg1_direct", "dense_l2", "frac_agg"])
+@pytest.mark.parametrize("case", ["cfg1_direct", "dense_l2", "frac_agg", "poly30"])
 def test_oracle_matches_reference_direct(case):
     g = golden(case)
     oc = O.OracleCluster(g["pos"], g["radius"], g["ri"])

@@ -1,7 +1,7 @@
 """Pins of the oracle's post-solve observables (CPU, no GPU needed).
 
 The restatements of tranordertest, merge_to_common_origin, scatteringmatrix
-and fosmexpansion are checked against the reference's own outputs for five
+and fosmexpansion are checked against the reference's own outputs for seven
 clusters (tests/golden/, obs_* keys, written by tests/golden/make_golden.py
 from the unmodified Fortran) and against what the physics fixes without the
 reference:
@@ -19,7 +19,7 @@ from numpy.polynomial.legendre import leggauss, legval
 from conftest import golden
 from oracle import oracle as O
 
-CASES = ["cfg1_direct", "dense_l2", "fft_cyl250", "frac_agg", "sparse100"]
+CASES = ["cfg1_direct", "dense_l2", "fft_cyl250", "frac_agg", "sparse100", "poly30", "poly30_fft"]
 
 
 def rel(a, b):
