@@ -397,8 +397,8 @@ def main():
                 tflops=round(npairs * tile_flops_per_pair(L, ncol) / (t32 * 1e-3) / 1e12, 2),
                 flop_per_byte=round(npairs * tile_flops_per_pair(L, ncol) / alg32, 2),
                 note=("algorithmic intensity below the FP32 ridge (157.3 TF / 8 TB/s = 19.7 flop/B), so quoted "
-                      "against HBM; tables are staged in LDS once per tile step (HBM traffic 1.17x algorithmic), "
-                      "VALU issues about 41% of cycles at one wave per SIMD (packed FP32 is disabled on this pool; "
+                      "against HBM; tables are staged in LDS once per tile step (HBM traffic 1.15x algorithmic), "
+                      "VALU issues about 48% of cycles at one wave per SIMD (packed FP32 is disabled on this pool; "
                       "SQ counters in profiles/direct_tile_cfg5_sq_counters.txt)"))
     matvec = dict(fp32_tile_ms=round(t32, 4), fp32_columns=ncol, fp32_hbm_GBps=round(ach32, 1))
     if st_lig.get("tile_launches_fp32", 0) > 0:

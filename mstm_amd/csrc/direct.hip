@@ -19,6 +19,9 @@
 //     of the tile copy it there once per step and read it from LDS: every
 //     table byte then leaves the L2 once per tile instead of once per lane
 //     that needs it (4 lanes x 2 or 4 waves);
+//   * the pipelined form (two-column FP32, order <= 4) also copies the
+//     tile's two source blocks into LDS once per tile, instead of every lane
+//     reloading its source sphere from L2 in every step;
 //   * the back rotation sums d^t(m,k) z_m(t) over the m blocks of a step
 //     and applies the output phase exp(-i sigma k phi) once per
 //     coefficient (with LDS-staged tables, whose freed registers hold the
@@ -118,6 +121,12 @@ template <typename T, int L> constexpr int tile_lds_elems()
 {
     return tile_staged<T, L>() ? slot_len(L, L, 16 / (int)sizeof(T)) * MSTM_SLOTS : 1;
 }
+// pipelined form: the tile's two source blocks (16 spheres x 2 columns x 2
+// modes x NB complex each) also sit in LDS for the whole tile
+template <typename T, int L, int NR> constexpr int tile_src_elems()
+{
+    return tile_pipelined<T, L, NR>() ? 2 * MSTM_SLOTS * 4 * L * (L + 2) * 2 : 1;
+}
 // copy table group s of a tile into LDS, all threads of the workgroup
 template <typename T, int L, int NR>
 __device__ __forceinline__ void stage_group(const T *__restrict__ tab, int tile, int s, T *__restrict__ grp)
@@ -134,7 +143,8 @@ __device__ __forceinline__ void stage_group(const T *__restrict__ tab, int tile,
 // the wave serves right-hand sides 2*rpair and 2*rpair+1 of NR.
 template <typename T, int L, int DIR, int NR>
 __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, int tile, const cplx<T> *__restrict__ xin,
-                                          cplx<T> *__restrict__ part, int npad, int rpair, T *__restrict__ grp)
+                                          cplx<T> *__restrict__ part, int npad, int rpair, T *__restrict__ grp,
+                                          const cplx<T> *__restrict__ srcl)
 {
     constexpr bool STAGED = tile_staged<T, L>();
     constexpr bool PIPE = tile_pipelined<T, L, NR>();
@@ -165,6 +175,8 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
             src = bt.x * S + slot;
         }
         const cplx<T> *xs = xin + ((int64_t)(rhs * npad + src) * 2 + mode) * NB;
+        // staged copy: block 0 = bt.y (forward sources), 1 = bt.x (reverse sources)
+        const cplx<T> *xl = srcl + (((DIR * S + (src % S)) * 2 + (rhs & 1)) * 2 + mode) * NB;
         // all waves of the tile read the same table group in step s: keep them
         // in step so that the group is fetched from HBM once (and, staged,
         // copied to LDS once)
@@ -178,7 +190,7 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
 #pragma unroll
                 for (int n = 1; n <= L; n++)
 #pragma unroll
-                    for (int k = -n; k <= n; k++) xv[nk(n, k)] = xs[amn(DIR ? -k : k, n, L)];
+                    for (int k = -n; k <= n; k++) xv[nk(n, k)] = xl[amn(DIR ? -k : k, n, L)];
             }
             if (s + 1 < S) pre.load(tab, tile, s + 1);
             __syncthreads();
@@ -309,8 +321,22 @@ __global__ __launch_bounds__(64 * NR) __attribute__((amdgpu_waves_per_eu(tile_wa
     const int2 bt = tiles[tile];
     const int w = threadIdx.x >> 6;
     __shared__ __attribute__((aligned(16))) T grp[tile_lds_elems<T, L>()];
-    if (!(w & 1)) tile_body<T, L, 0, NR>(tab, bt, tile, xin, part, npad, w >> 1, grp);
-    else if (bt.x != bt.y) tile_body<T, L, 1, NR>(tab, bt, tile, xin, part, npad, w >> 1, grp);
+    __shared__ __attribute__((aligned(16))) T srcs[tile_src_elems<T, L, NR>()];
+    cplx<T> *srcl = reinterpret_cast<cplx<T> *>(srcs);
+    if constexpr (tile_pipelined<T, L, NR>()) {
+        // the two source blocks of the tile, all threads; visible after the
+        // first step's barrier
+        constexpr int NB = L * (L + 2), BLK = MSTM_SLOTS * 2 * NB;  // one column r: 16 spheres x 2 modes
+        for (int e = threadIdx.x; e < 2 * 2 * BLK; e += 64 * NR) {
+            const int b = e / (2 * BLK), rem = e - b * 2 * BLK, r = rem / BLK, o = rem - r * BLK;
+            const int blk = b == 0 ? bt.y : bt.x;
+            const int i = o / (2 * NB), mo = o - i * 2 * NB;  // sphere in block, (mode, coefficient)
+            srcl[((b * MSTM_SLOTS + i) * 2 + r) * 2 * NB + mo] =
+                xin[((int64_t)(r * npad + blk * MSTM_SLOTS + i) * 2) * NB + mo];
+        }
+    }
+    if (!(w & 1)) tile_body<T, L, 0, NR>(tab, bt, tile, xin, part, npad, w >> 1, grp, srcl);
+    else if (bt.x != bt.y) tile_body<T, L, 1, NR>(tab, bt, tile, xin, part, npad, w >> 1, grp, srcl);
     else
     {
         // idle reverse wave of a diagonal tile: its share of the staging and the same barriers
