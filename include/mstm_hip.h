@@ -108,6 +108,12 @@ int mstm_hip_fixedorsoln(mstm_hip_ctx *ctx, double alpha, double sinc, int dir, 
  * 1 = GMRES (one product column per system and step; the default). */
 int mstm_hip_set_inner_solver(mstm_hip_ctx *ctx, int kind);
 
+/* Largest number of FP64 refinement cycles of MSTM_HIP_SOLVE_MIXED and
+ * _LEARNED (no counterpart in MSTM; default 6, one normally suffices).  A
+ * polarization still above eps after them is handed to the plain FP64
+ * solver from b; 0 hands it over at once, after the first FP64 residual. */
+int mstm_hip_set_refinement_cycles(mstm_hip_ctx *ctx, int cycles);
+
 /* ---------------------------------------------------------------------------
  * Post-solve observables of a single-origin expansion (a cluster without
  * plane boundaries, single_origin_expansion = .true., the reference default;
@@ -181,7 +187,7 @@ const char *mstm_hip_stats_names(void);
  * start, runs the mixed solver from the better one (every answer is accepted
  * on its FP64 residual), and hands over to the plain FP64 solver from b if
  * the mixed solve does not reach eps; other incidence directions start from
- * zero.  MSTM_HIP_SOLVE_MIXED starts from zero.
+ * zero.  MSTM_HIP_SOLVE_MIXED starts from zero and hands over the same way.
  * ------------------------------------------------------------------------- */
 int mstm_hip_lig_nparams(int hidden, int nlayers, int Lnet);
 /* load (params != NULL) or unload the network of the learned solver */

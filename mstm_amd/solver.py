@@ -262,6 +262,12 @@ class HipContext:
         """Inner solver of the mixed refinement: 0 BiCG, 1 GMRES (default)."""
         self._chk(self.L.mstm_hip_set_inner_solver(self.h, int(kind)), "set_inner_solver")
 
+    def set_refinement_cycles(self, cycles: int):
+        """Largest number of FP64 refinement cycles of the mixed solvers
+        (default 6); a polarization still above eps then falls back to the
+        plain FP64 solver from b (0: at once)."""
+        self._chk(self.L.mstm_hip_set_refinement_cycles(self.h, int(cycles)), "set_refinement_cycles")
+
     def clear(self, clear_tables=True):
         self._chk(self.L.mstm_hip_clear(self.h, int(bool(clear_tables))), "clear")
 

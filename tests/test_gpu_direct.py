@@ -183,3 +183,30 @@ def test_mixed_inner_solvers_reach_reference_residual(gpu_ctx, inner):
         assert rel(rt["amnp"], rp["amnp"]) < 1e-5
     finally:
         gpu_ctx.set_inner_solver(1)
+
+
+@pytest.mark.parametrize("solver", [1, 2])
+def test_stalled_refinement_falls_back_to_plain_fp64(gpu_ctx, solver):
+    """With no refinement cycle allowed the mixed (1) and learned (2) solvers
+    must hand both polarizations to the plain FP64 solver from b, which then
+    follows the reference iterates exactly."""
+    from mstm_amd.solver import SOLVE_LEARNED
+
+    g = golden("cfg1_direct")
+    load_cluster(gpu_ctx, g)
+    if solver == SOLVE_LEARNED:
+        gpu_ctx.load_learned()
+    gpu_ctx.set_refinement_cycles(0)
+    try:
+        gpu_ctx.clear(clear_tables=False)
+        r = gpu_ctx.fixedorsoln(eps=1e-6, niter=10000, solver=solver)
+        st = gpu_ctx.stats()
+    finally:
+        gpu_ctx.set_refinement_cycles(6)
+        if solver == SOLVE_LEARNED:
+            gpu_ctx.set_learned(None)
+    assert r["istat"] == 0 and r["maxerr"] < 1e-6
+    assert st["lig_fallbacks"] >= 2 and st["refinements"] == 0
+    # the handover starts from b, as the reference does: same iterates
+    assert r["iterations"] == g["iterations"]
+    assert rel(r["amnp"], g["amnp"]) < FP64_TOL
