@@ -18,6 +18,8 @@
 #ifndef MSTM_HIP_H
 #define MSTM_HIP_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -162,6 +164,31 @@ int mstm_hip_scattering_matrix(mstm_hip_ctx *ctx, int T, const double *amnp0, in
  * scattering cross section (k^-2 units).  amnp0/T as for the scattering
  * matrix.  s00 (2 doubles) or g may be NULL. */
 int mstm_hip_asymmetry(mstm_hip_ctx *ctx, int T, const double *amnp0, double *s00, double *g);
+
+/* tmatrix_solution (mstm-solver-8.f90:22-332), iterative, one GPU: the
+ * cluster T-matrix about the origin of the last translation-order call
+ * (mstm_hip_translation_orders / _set_translation_orders; T = its
+ * t_matrix_order).  For l = 1..T and each column (k, q), k = -l..l, q = 1, 2,
+ * the unit TE/TM excitation of degree l at the origin (:169-175) is
+ * distributed to the spheres (distribute_from_common_origin,
+ * mstm-scatprops-26.f90:561-669, on the GPU), multiplied by the Mie matrices,
+ * solved (solver MSTM_HIP_SOLVE_PLAIN: cbicg's FP64 iterates from x = b, the
+ * two columns of one k together; MSTM_HIP_SOLVE_MIXED: the mixed solver with
+ * its plain-FP64 hand-over) and merged back at order l.  Stops after order l
+ * when the change of the orientation-averaged Q_ext is below conveps
+ * (conveps > 0, :322).
+ *   tmat        complex, the T-matrix file body in the reference's write order
+ *               (:243-256): per column, n = 1..l, m = -n..n, p = 1, 2;
+ *               capacity tmat_cap complex values >= sum_l 4(2l+1) l(l+2)
+ *   sphere_qeff nsphere x 3: per-sphere efficiency sums (ext, abs, sca) over
+ *               all columns (sphere_qeff of the reference); may be NULL
+ *   qteff       3: the T-matrix sums of :233-237; may be NULL
+ *   order_log   T x 4 per order: cumulative max iterations, Q_ext, Q_abs and
+ *               the change of Q_ext (the reference's progress lines); may be NULL
+ *   t_order     the order reached; maxiter / maxerr / istat as the reference. */
+int mstm_hip_tmatrix_solution(mstm_hip_ctx *ctx, double eps, int niter, double conveps, int solver, double *tmat,
+                              int64_t tmat_cap, double *sphere_qeff, double *qteff, double *order_log, int *t_order,
+                              int *maxiter, double *maxerr, int *istat);
 
 /* clear_fft_matrix (fft_translation-5.f90:42-70) / table release. */
 int mstm_hip_clear(mstm_hip_ctx *ctx, int clear_tables);

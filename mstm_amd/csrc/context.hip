@@ -134,6 +134,13 @@ struct mstm_hip_ctx {
         double *d_cg = nullptr, *d_mom = nullptr;
         int cg_T = 0;
         double orders_ms = 0, merge_ms = 0, scatmat_ms = 0, moments_ms = 0;
+        // T-matrix solution: distribute table and the unit-excitation columns
+        double *d_vccd = nullptr;
+        int vccd_T = 0, vccd_LS = 0;
+        double *d_dist = nullptr, *d_dq = nullptr;
+        size_t dist_bytes = 0, dq_bytes = 0;
+        double tmatrix_ms = 0;
+        int tm_solves = 0;
     } obs;
 };
 
@@ -285,8 +292,12 @@ static void release_observables(mstm_hip_ctx *c)
     dfree(c->obs.d_sm);
     dfree(c->obs.d_cg);
     dfree(c->obs.d_mom);
-    c->obs.vccm_T = c->obs.vccm_LS = c->obs.cg_T = 0;
+    dfree(c->obs.d_vccd);
+    dfree(c->obs.d_dist);
+    dfree(c->obs.d_dq);
+    c->obs.vccm_T = c->obs.vccm_LS = c->obs.cg_T = c->obs.vccd_T = c->obs.vccd_LS = 0;
     c->obs.part_bytes = c->obs.a0_bytes = c->obs.in_bytes = c->obs.ang_cap = 0;
+    c->obs.dist_bytes = c->obs.dq_bytes = 0;
 }
 
 // ------------------------------------------------------------------ C ABI
@@ -1542,7 +1553,7 @@ const char *mstm_hip_stats_names(void)
            "fft_setup_ms,fft_conv_ms,fft_launches,npad,ntiles,nblocks,lig_solves,lig_fallbacks,lig_ms,lig_err0,"
            "lig_errb,fgmres_steps,tile4_ms_fp64,tile4_ms_fp32,tile4_launches_fp64,tile4_launches_fp32,inner_solver,"
            "obs_orders_ms,obs_merge_ms,obs_scatmat_ms,obs_moments_ms,obs_T,fft_contract_ms_fp32,"
-           "fft_contract_launches_fp32,fft_contract_ms_fp64,fft_contract_launches_fp64";
+           "fft_contract_launches_fp32,fft_contract_ms_fp64,fft_contract_launches_fp64,tmatrix_ms,tmatrix_solves";
 }
 
 int mstm_hip_stats(const mstm_hip_ctx *c, double *out, int nout)
@@ -1587,7 +1598,9 @@ int mstm_hip_stats(const mstm_hip_ctx *c, double *out, int nout)
                   c->fft ? fft_stat(c->fft, 3) : 0.0,
                   c->fft ? fft_stat(c->fft, 4) : 0.0,
                   c->fft ? fft_stat(c->fft, 5) : 0.0,
-                  c->fft ? fft_stat(c->fft, 6) : 0.0};
+                  c->fft ? fft_stat(c->fft, 6) : 0.0,
+                  c->obs.tmatrix_ms,
+                  (double)c->obs.tm_solves};
     int k = std::min(nout, (int)(sizeof v / sizeof v[0]));
     for (int q = 0; q < k; q++) out[q] = v[q];
     return k;
@@ -1657,6 +1670,34 @@ static int ensure_vccm(mstm_hip_ctx *c, int T, int LS)
     HIPCHK(hipMemcpy(c->obs.d_vccm, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
     c->obs.vccm_T = T;
     c->obs.vccm_LS = LS;
+    return 0;
+}
+
+// vector-coupling sums of the distribute translation (target degree t <= LS,
+// source degree l' <= T): the same coefficients as ensure_vccm with the
+// roles of the two degrees exchanged, laid out [l'][t][m + LS][w - |t - l'|]
+static int ensure_vccd(mstm_hip_ctx *c, int T, int LS)
+{
+    if (c->obs.d_vccd && c->obs.vccd_T == T && c->obs.vccd_LS == LS) return 0;
+    dfree(c->obs.d_vccd);
+    const int W = 2 * LS + 1;
+    std::vector<double> h((size_t)(T + 1) * (LS + 1) * W * W, 0.0);
+    for (int l = 1; l <= T; l++)
+        for (int t = 1; t <= LS; t++) {
+            const long double c1 = sqrtl((long double)(2 * t + 1)) * sqrtl((long double)(2 * l + 1));
+            for (int m = -std::min(t, l); m <= std::min(t, l); m++)
+                for (int w = std::abs(t - l); w <= t + l; w++) {
+                    const long double c2 = -c1 * ((std::abs(m) & 1) ? -1.0L : 1.0L);
+                    const int k4 = (t - l + w) & 3;
+                    const long double sg = (k4 == 0 || k4 == 1) ? 1.0L : -1.0L;
+                    h[(((size_t)l * (LS + 1) + t) * W + (m + LS)) * W + (w - std::abs(t - l))] =
+                        (double)(c2 * clebsch(t, -m, l, m, w) * clebsch(t, -1, l, 1, w) * sg);
+                }
+        }
+    HIPCHK(dalloc(&c->obs.d_vccd, sizeof(double) * h.size()));
+    HIPCHK(hipMemcpy(c->obs.d_vccd, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    c->obs.vccd_T = T;
+    c->obs.vccd_LS = LS;
     return 0;
 }
 
@@ -1865,6 +1906,145 @@ int mstm_hip_asymmetry(mstm_hip_ctx *c, int T, const double *amnp0, double *s00,
     }
     if (g) *g = s[1] / (3.0 * s[0]);
     c->obs.moments_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+int mstm_hip_tmatrix_solution(mstm_hip_ctx *c, double eps, int niter, double conveps, int solver, double *tmat,
+                              int64_t tmat_cap, double *sphere_qeff, double *qteff, double *order_log, int *t_order,
+                              int *maxiter, double *maxerr, int *istat)
+{
+    if (!c || !c->nsph) return fail(c, MSTM_HIP_ESTATE, "tmatrix_solution: no spheres");
+    if (c->obs.T < 1 || c->obs.torder.size() != (size_t)c->nsph)
+        return fail(c, MSTM_HIP_ESTATE, "tmatrix_solution: no translation orders (mstm_hip_translation_orders)");
+    if (niter <= 0) return fail(c, MSTM_HIP_EUNSUP, "tmatrix_solution: niter <= 0 not supported");
+    if (solver != MSTM_HIP_SOLVE_PLAIN && solver != MSTM_HIP_SOLVE_MIXED)
+        return fail(c, MSTM_HIP_EUNSUP, "tmatrix_solution: solver %d not available", solver);
+    if (!tmat || !t_order || !maxiter || !maxerr || !istat) return fail(c, MSTM_HIP_EARG, "tmatrix_solution: null argument");
+    if (c->Lmax > MSTM_MAX_L)
+        return fail(c, MSTM_HIP_EUNSUP, "tmatrix_solution: sphere order %d > %d", c->Lmax, MSTM_MAX_L);
+    const int T = c->obs.T;
+    int64_t need = 0;
+    for (int l = 1; l <= T; l++) need += (int64_t)4 * (2 * l + 1) * l * (l + 2);
+    if (tmat_cap < need)
+        return fail(c, MSTM_HIP_EARG, "tmatrix_solution: tmat holds %lld complex values, order %d needs %lld",
+                    (long long)tmat_cap, T, (long long)need);
+    HIPCHK(hipSetDevice(c->device));
+    auto t0 = std::chrono::steady_clock::now();
+    c->obs.x_valid = false;
+    c->obs.a0_T = 0;
+    const int n = c->neqns, nsph = c->nsph, bT = T * (T + 2);
+    const size_t V = 2 * (size_t)n;
+    double *b = c->d_kv + V * KV_B, *x = c->d_kv + V * KV_X;
+    if (int rc = ensure_vccm(c, T, c->Lmax)) return rc;
+    if (int rc = ensure_vccd(c, T, c->Lmax)) return rc;
+    if (int rc = grow(c, &c->obs.d_dist, c->obs.dist_bytes, sizeof(double) * V * 2 * (2 * T + 1))) return rc;
+    if (int rc = grow(c, &c->obs.d_dq, c->obs.dq_bytes, sizeof(double) * 9 * nsph)) return rc;
+    const size_t per_group = sizeof(double) * 8 * (size_t)bT;
+    int G = std::min(nsph, 1024);
+    G = (int)std::max<size_t>(1, std::min<size_t>((size_t)G, ((size_t)1 << 30) / per_group));
+    if (int rc = grow(c, &c->obs.d_part, c->obs.part_bytes, per_group * G)) return rc;
+    if (int rc = grow(c, &c->obs.d_a0, c->obs.a0_bytes, per_group)) return rc;
+    // vol_radius (mstm-input-37.f90:1205-1211): real(4) exponent .333333
+    double vr = 0.0;
+    for (int i = 0; i < nsph; i++) vr += c->radius[i] * c->radius[i] * c->radius[i];
+    vr = std::pow(vr, (double).333333f);
+    const double vr2 = vr * vr;
+    std::vector<double> hq((size_t)9 * nsph), ha0((size_t)8 * bT), sq((size_t)3 * nsph, 0.0);
+    double qt[3] = {0.0, 0.0, 0.0}, qeffold = 0.0;
+    int64_t nt = 0;
+    int lreach = T;
+    *maxiter = 0;
+    *maxerr = 0.0;
+    *istat = 0;
+    c->obs.tm_solves = 0;
+    for (int l = 1; l <= T; l++) {
+        const int b0 = l * (l + 2);
+        HIPCHK(launch_distribute(l, c->d_order, c->d_offset, c->d_pos, nsph, c->Lmax, c->obs.r0, c->rimed,
+                                 c->obs.d_vccd, c->obs.d_dist, n, std::min(nsph, 1024), c->st));
+        for (int k0 = -l; k0 <= l; k0++) {
+            // columns (k0, q = 1) and (k0, q = 2): two independent systems solved together
+            double *dp = c->obs.d_dist + V * 2 * (k0 + l);
+            for (int q = 0; q < 2; q++)
+                HIPCHK(launch_mie_mult(dp + V * q, b + V * q, nsph, c->d_order, c->d_offset, c->d_mieoff, c->d_an, 1,
+                                       0, c->Lmax, c->st));
+            int it[2] = {0, 0}, rc = 0;
+            double err[2] = {0.0, 0.0};
+            const double eps2[2] = {eps, eps};
+            if (solver == MSTM_HIP_SOLVE_PLAIN) {
+                HIPCHK(hipMemcpyAsync(x, b, sizeof(double) * 2 * V, hipMemcpyDeviceToDevice, c->st));
+                rc = bicg2_dev(c, 2, niter, eps2, b, x, MSTM_HIP_FP64, 0, it, err);
+            } else {
+                HIPCHK(hipMemsetAsync(x, 0, sizeof(double) * 2 * V, c->st));
+                rc = refine2_dev(c, 2, niter, eps, b, x, 0, it, err, b);
+                for (int p = 0; p < 2 && !rc; p++)
+                    if (err[p] >= eps && it[p] < niter) {
+                        c->stats.lig_fallbacks++;
+                        HIPCHK(hipMemcpyAsync(x + V * p, b + V * p, sizeof(double) * V, hipMemcpyDeviceToDevice,
+                                              c->st));
+                        int it2 = 0;
+                        rc = bicg2_dev(c, 1, niter - it[p], &eps, b + V * p, x + V * p, MSTM_HIP_FP64, 0, &it2,
+                                       &err[p]);
+                        it[p] += it2;
+                    }
+            }
+            if (rc) return rc;
+            c->obs.tm_solves += 2;
+            for (int q = 0; q < 2; q++) {
+                *maxiter = std::max(*maxiter, it[q]);
+                *maxerr = std::max(*maxerr, err[q]);
+                if (it[q] > niter || err[q] > eps) *istat = 1;
+            }
+            // per-sphere efficiencies of each column (qefficiencyfactors, npol = 1)
+            HIPCHK(launch_qeff(x, dp, 2 * n, nsph, c->d_radius, c->d_order, c->d_offset, c->d_mieoff, c->d_an,
+                               c->rimed, c->obs.d_dq, c->st));
+            HIPCHK(hipMemcpyAsync(hq.data(), c->obs.d_dq, sizeof(double) * 9 * nsph, hipMemcpyDeviceToHost, c->st));
+            // both columns merged to the origin at order l (target orders min(l, torder_i))
+            HIPCHK(launch_merge(x, n, c->d_order, c->d_offset, c->obs.d_torder, c->d_pos, nsph, l, c->Lmax,
+                                c->obs.r0, c->rimed, c->obs.d_vccm, c->obs.d_part, G, c->obs.d_a0, c->st));
+            HIPCHK(hipMemcpyAsync(ha0.data(), c->obs.d_a0, sizeof(double) * 8 * b0, hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+            for (int q = 0; q < 2; q++) {
+                for (int i = 0; i < nsph; i++)
+                    for (int e = 0; e < 3; e++) sq[3 * i + e] += hq[9 * i + 3 * (q + 1) + e];
+                const double *a0 = ha0.data() + 4 * (size_t)b0 * q;  // [mode][b0] complex
+                const double d1 = -2.0 / vr2 * a0[2 * ((size_t)q * b0 + amn(k0, l, l))];
+                double d3 = 0.0;
+                for (int e = 0; e < 2 * b0; e++) d3 += a0[2 * e] * a0[2 * e] + a0[2 * e + 1] * a0[2 * e + 1];
+                d3 *= 2.0 / vr2;
+                qt[0] += d1;
+                qt[1] += d1 - d3;
+                qt[2] += d3;
+                for (int nn = 1; nn <= l; nn++)
+                    for (int m = -nn; m <= nn; m++)
+                        for (int p = 0; p < 2; p++) {
+                            const size_t e = (size_t)p * b0 + amn(m, nn, l);
+                            tmat[2 * nt] = a0[2 * e];
+                            tmat[2 * nt + 1] = a0[2 * e + 1];
+                            nt++;
+                        }
+            }
+        }
+        double qe[2] = {0.0, 0.0};
+        for (int i = 0; i < nsph; i++)
+            for (int e = 0; e < 2; e++) qe[e] += sq[3 * i + e] * (c->radius[i] * c->radius[i]) / vr2;
+        const double converr = qe[0] - qeffold;
+        qeffold = qe[0];
+        if (order_log) {
+            order_log[4 * (l - 1)] = *maxiter;
+            order_log[4 * (l - 1) + 1] = qe[0];
+            order_log[4 * (l - 1) + 2] = qe[1];
+            order_log[4 * (l - 1) + 3] = converr;
+        }
+        if (conveps > 0.0 && converr < conveps) {
+            lreach = l;
+            break;
+        }
+    }
+    *t_order = lreach;
+    if (sphere_qeff) std::memcpy(sphere_qeff, sq.data(), sizeof(double) * 3 * nsph);
+    if (qteff) std::memcpy(qteff, qt, sizeof(qt));
+    c->obs.tmatrix_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    collect_events(c);
     return 0;
 }
 

@@ -64,6 +64,9 @@ hipError_t launch_merge(const double *amnp, int64_t ldc, const int *order, const
                         double *part, int G, double *a0, hipStream_t st);
 hipError_t launch_scatmat(const double *a0, int T, int nang, const double *ct, const double *phi, double rimed,
                           double *sm, hipStream_t st);
+hipError_t launch_distribute(int l, const int *order, const int *offset, const double *pos, int nsph, int LS,
+                             const double *r0, double rimed, const double *vccd, double *out, int64_t ldc, int G,
+                             hipStream_t st);
 int s00_moment_blocks(int T);
 hipError_t launch_s00_moments(const double *a0, int T, const double *cgm, const double *cg1, double *out,
                               hipStream_t st);

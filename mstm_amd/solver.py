@@ -218,6 +218,34 @@ class HipContext:
         s00, g = self.asymmetry()
         return dict(T=T, torder=tord, amnp0=a0, theta=th, scatmat=sm, s00=s00, g=g)
 
+    def tmatrix_solution(self, eps: float = 1e-6, niter: int = 10000, conveps: float = 1e-6, solver: int = 0,
+                         teps: float = 1e-5, maxt: int = 100):
+        """tmatrix_solution (mstm-solver-8.f90:22-332) on the GPU: the cluster
+        T-matrix about the mean sphere position (translation orders from
+        tranordertest at teps, as main_calling_program sets them).  Returns
+        dict(T = order reached, tmat = the T-matrix file body in its write
+        order, sqeff (nsphere, 3), qteff (3), qlog (T, 4): cumulative max
+        iterations, Q_ext, Q_abs and the change of Q_ext per order; maxiter,
+        maxerr, istat).  solver: SOLVE_PLAIN (the reference's FP64 iterates)
+        or SOLVE_MIXED."""
+        T, _ = self.translation_orders(None, teps, maxt)
+        ntm = sum(4 * (2 * l + 1) * l * (l + 2) for l in range(1, T + 1))
+        tm = np.zeros(ntm, np.complex128)
+        sq = np.zeros((self.nsphere, 3))
+        qt = np.zeros(3)
+        ql = np.zeros((T, 4))
+        tr = np.zeros(1, np.int32)
+        mi = np.zeros(1, np.int32)
+        me = np.zeros(1)
+        ist = np.zeros(1, np.int32)
+        self._chk(self.L.mstm_hip_tmatrix_solution(self.h, float(eps), int(niter), float(conveps), int(solver),
+                                                   _d(tm.view(np.float64)), ntm, _d(sq), _d(qt), _d(ql), _i(tr),
+                                                   _i(mi), _d(me), _i(ist)), "tmatrix_solution")
+        Tr = int(tr[0])
+        nused = sum(4 * (2 * l + 1) * l * (l + 2) for l in range(1, Tr + 1))
+        return dict(T=Tr, T_in=T, tmat=tm[:nused], sqeff=sq, qteff=qt, qlog=ql[:Tr], maxiter=int(mi[0]),
+                    maxerr=float(me[0]), istat=int(ist[0]))
+
     # -- learned initial guess (no reference counterpart) ---------------------
     def set_learned(self, params=None, hidden: int = 128, nlayers: int = 3, lnet: int = 4):
         """Load a network for SOLVE_LEARNED (None unloads it)."""

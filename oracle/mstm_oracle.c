@@ -1575,3 +1575,105 @@ void orc_fosm(int T, const zc *amnp, double rimed, double *s00, double *s02, dou
     free(cmm2);
     free(cmp2);
 }
+
+/* distribute_from_common_origin (mstm-scatprops-26.f90:561-669), vswf_type
+ * 1, one column: the common-origin expansion a0 (order l, layout
+ * (0:l+1, l, 2)) is carried to every sphere along r_i - r0, order l -> L_i,
+ * rotation path where max(L_i, l) >= translation_switch_order (:642).    */
+void orc_distribute(const orc_cluster *c, int l, const double *r0, const zc *a0, zc *amnp)
+{
+    zc ri[2] = {c->rimed, c->rimed};
+    memset(amnp, 0, sizeof(zc) * c->neqns);
+    for (int i = 0; i < c->n; i++) {
+        int L = c->order[i];
+        double rt[3] = {c->pos[3 * i] - r0[0], c->pos[3 * i + 1] - r0[1], c->pos[3 * i + 2] - r0[2]};
+        orc_pair_translate(1, rt, ri, l, L, 0, 0, imax(L, l) >= c->switch_order, a0, amnp + c->offset[i]);
+    }
+}
+
+/* tmatrix_solution (mstm-solver-8.f90:22-332), serial, iterative branch.
+ * For l = 1..T and (k, q) in the reference's order (k = -l..l, q = 1, 2):
+ * the unit TE/TM excitation of (k, l, q) at the origin (:169-175) is
+ * distributed to the spheres, multiplied by the Mie matrices, solved with
+ * cbicg from x = b, and merged back to the origin at order l (target order
+ * min(l, torder_i)).  Outputs:
+ *   tmat   the T-matrix file body in its write order (:243-256): per
+ *          column, rows n = 1..l, m = -n..n, p = 1, 2 (TE/TM form);
+ *   sqeff  [sphere][3] sums of the per-sphere efficiencies over all
+ *          columns (qefficiencyfactors with npol = 1: ext, abs, sca);
+ *   qteff  [3] the T-matrix efficiency sums of :233-237;
+ *   qlog   [l][4] per order: cumulative max iterations, qeff(1), qeff(2),
+ *          converr (:265-276).
+ * Stops after order l when converr < conveps (conveps > 0, :322).
+ * Returns the order reached; *istat = 1 if a solve missed eps or niter.   */
+int orc_tmatrix(const orc_cluster *c, int T, const int *torder, const double *r0, double vol_radius, double eps,
+                int niter, double conveps, zc *tmat, double *sqeff, double *qteff, double *qlog, int *maxiter,
+                double *maxerr, int *istat)
+{
+    int ne = c->neqns;
+    zc *dist = malloc(sizeof(zc) * 2 * ne), *b = malloc(sizeof(zc) * 2 * ne), *x = malloc(sizeof(zc) * 2 * ne);
+    zc *a0 = malloc(sizeof(zc) * 2 * T * (T + 2)), *u = malloc(sizeof(zc) * 2 * T * (T + 2));
+    zc *am0 = malloc(sizeof(zc) * 2 * T * (T + 2));
+    double *dq = malloc(sizeof(double) * 9 * c->n);
+    double qeffold = 0.0, vr2 = vol_radius * vol_radius;
+    size_t nt = 0;
+    int l;
+    *istat = 0;
+    *maxiter = 0;
+    *maxerr = 0.0;
+    memset(sqeff, 0, sizeof(double) * 3 * c->n);
+    qteff[0] = qteff[1] = qteff[2] = 0.0;
+    for (l = 1; l <= T; l++) {
+        int b0 = l * (l + 2);
+        for (int k = -l; k <= l; k++) {
+            /* both q of one k: their solves are independent, done one by one */
+            for (int q = 1; q <= 2; q++) {
+                memset(u, 0, sizeof(zc) * 2 * b0);
+                u[amnaddr(k, l, l) - 1] = 0.5;
+                u[amnaddr(k, l, l) - 1 + b0] = -0.5 * ((q & 1) ? -1.0 : 1.0);
+                orc_distribute(c, l, r0, u, dist + (size_t)(q - 1) * ne);
+                orc_mie_mult(c, 1, 0, dist + (size_t)(q - 1) * ne, b + (size_t)(q - 1) * ne);
+                memcpy(x + (size_t)(q - 1) * ne, b + (size_t)(q - 1) * ne, sizeof(zc) * ne);
+                double err = 0.0;
+                int it = orc_cbicg(c, 0, niter, eps, b + (size_t)(q - 1) * ne, x + (size_t)(q - 1) * ne, &err);
+                if (it > *maxiter) *maxiter = it;
+                if (err > *maxerr) *maxerr = err;
+                if (it > niter || err > eps) *istat = 1;
+            }
+            orc_qeff(c, x, dist, dq);
+            for (int q = 1; q <= 2; q++) {
+                for (int i = 0; i < c->n; i++)
+                    for (int e = 0; e < 3; e++) sqeff[3 * i + e] += dq[9 * i + 3 * q + e];
+                orc_merge(c, l, torder, r0, 1, x + (size_t)(q - 1) * ne, am0);
+                int ka = amnaddr(k, l, l) - 1 + (q - 1) * b0;
+                double d1 = -2.0 / vr2 * creal(am0[ka]), d3 = 0.0;
+                for (int e = 0; e < 2 * b0; e++) d3 += creal(am0[e] * conj(am0[e]));
+                d3 *= 2.0 / vr2;
+                qteff[0] += d1;
+                qteff[1] += d1 - d3;
+                qteff[2] += d3;
+                for (int n = 1; n <= l; n++)
+                    for (int m = -n; m <= n; m++)
+                        for (int p = 0; p < 2; p++) tmat[nt++] = am0[amnaddr(m, n, l) - 1 + p * b0];
+            }
+        }
+        double qe[2] = {0.0, 0.0};
+        for (int i = 0; i < c->n; i++)
+            for (int e = 0; e < 2; e++) qe[e] += sqeff[3 * i + e] * (c->rad[i] * c->rad[i]) / vr2;
+        double converr = qe[0] - qeffold;
+        qeffold = qe[0];
+        qlog[4 * (l - 1)] = *maxiter;
+        qlog[4 * (l - 1) + 1] = qe[0];
+        qlog[4 * (l - 1) + 2] = qe[1];
+        qlog[4 * (l - 1) + 3] = converr;
+        if (conveps > 0.0 && converr < conveps) break;
+    }
+    free(dist);
+    free(b);
+    free(x);
+    free(a0);
+    free(u);
+    free(am0);
+    free(dq);
+    return l <= T ? l : T;
+}

@@ -83,6 +83,12 @@ void orc_merge(const orc_cluster *c, int T, const int *torder, const double *r0,
 void orc_scatmat(int T, const zc *a0, double ct, double phi, double rimed, double *sm, zc *sa_out);
 void orc_fosm(int T, const zc *amnp, double rimed, double *s00, double *s02, double *sp22, double *sm22);
 
+/* T-matrix solution (random-orientation path) */
+void orc_distribute(const orc_cluster *c, int l, const double *r0, const zc *a0, zc *amnp);
+int orc_tmatrix(const orc_cluster *c, int T, const int *torder, const double *r0, double vol_radius, double eps,
+                int niter, double conveps, zc *tmat, double *sqeff, double *qteff, double *qlog, int *maxiter,
+                double *maxerr, int *istat);
+
 #ifdef __cplusplus
 }
 #endif

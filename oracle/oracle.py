@@ -67,6 +67,10 @@ def lib():
         L.orc_merge.argtypes = [vp, C.c_int, ip, dp, C.c_int, vp, vp]
         L.orc_scatmat.argtypes = [C.c_int, vp, C.c_double, C.c_double, C.c_double, dp, vp]
         L.orc_fosm.argtypes = [C.c_int, vp, C.c_double, dp, dp, dp, dp]
+        L.orc_distribute.argtypes = [vp, C.c_int, dp, vp, vp]
+        L.orc_tmatrix.restype = C.c_int
+        L.orc_tmatrix.argtypes = [vp, C.c_int, ip, dp, C.c_double, C.c_double, C.c_int, C.c_double, vp, dp, dp, dp,
+                                  ip, dp, ip]
         L.orc_cg.restype = C.c_double
         L.orc_cg.argtypes = [C.c_int] * 5
         L.orc_bcof.restype = C.c_double
@@ -195,6 +199,44 @@ class OracleCluster:
         r0 = np.ascontiguousarray(origin, np.float64)
         lib().orc_merge(self.h, T, _i(t), _d(r0), nrhs, a.ctypes.data, out.ctypes.data)
         return out
+
+    def distribute(self, l, origin, a0):
+        """distribute_from_common_origin of one (2, l(l+2)) helicity column -> (neqns,)."""
+        a = np.ascontiguousarray(a0, np.complex128)
+        out = np.zeros(self.neqns, np.complex128)
+        r0 = np.ascontiguousarray(origin, np.float64)
+        lib().orc_distribute(self.h, l, _d(r0), a.ctypes.data, out.ctypes.data)
+        return out
+
+    def tmatrix(self, eps=1e-6, niter=10000, conveps=1e-6, teps=1e-5, T=None, torder=None, origin=None):
+        """tmatrix_solution (mstm-solver-8.f90:22-332), serial iterative branch.
+
+        Orders as main_calling_program sets them (origin = mean position,
+        tranordertest at teps); returns dict with T (order reached), tmat (the
+        T-matrix file body), sqeff (nsphere, 3), qteff, qlog (T, 4: cumulative
+        max iterations, qext, qabs, converr), maxiter, maxerr, istat."""
+        r0 = np.ascontiguousarray(self.pos.mean(axis=0) if origin is None else origin, np.float64)
+        if torder is None:
+            T0, torder = self.translation_orders(r0, teps)
+            T = T0 if T is None else T
+        t = np.ascontiguousarray(torder, np.int32)
+        vr = 0.0  # vol_radius (mstm-input-37.f90:1205-1211), sequential sum, real(4) exponent
+        for r in self.radius:
+            vr += float(r) * float(r) * float(r)
+        vr = vr ** float(np.float32(0.333333))
+        ntm = sum(2 * (2 * l + 1) * 2 * l * (l + 2) for l in range(1, T + 1))
+        tm = np.zeros(ntm, np.complex128)
+        sq = np.zeros((self.n, 3))
+        qt = np.zeros(3)
+        ql = np.zeros((T, 4))
+        mi = np.zeros(1, np.int32)
+        me = np.zeros(1)
+        ist = np.zeros(1, np.int32)
+        Tr = lib().orc_tmatrix(self.h, T, _i(t), _d(r0), vr, eps, niter, conveps, tm.ctypes.data, _d(sq), _d(qt),
+                               _d(ql), _i(mi), _d(me), _i(ist))
+        nused = sum(2 * (2 * l + 1) * 2 * l * (l + 2) for l in range(1, Tr + 1))
+        return dict(T=int(Tr), tmat=tm[:nused], sqeff=sq, qteff=qt, qlog=ql[:Tr], maxiter=int(mi[0]),
+                    maxerr=float(me[0]), istat=int(ist[0]), vol_radius=vr, origin=r0, torder=t)
 
     def local_pairs(self):
         n = lib().orc_local_pairs(self.h, None)
@@ -366,10 +408,31 @@ def run_reference(pos, radius, ri, mode, ain=None, outdir=None, timeout=None, **
             raw = open(os.path.join(outdir, name), "rb").read()
             if key in ("order", "offset", "torder"):
                 res[key] = np.frombuffer(raw, np.int32).copy()
-            elif key in ("qeff", "qtot", "rotcoef", "scatmat", "s00", "s02", "sp22", "sm22"):
+            elif key in ("qeff", "qtot", "rotcoef", "scatmat", "s00", "s02", "sp22", "sm22", "sqeff"):
                 res[key] = np.frombuffer(raw, np.float64).copy()
             else:
                 res[key] = np.frombuffer(raw, np.complex128).copy()
+    tfile = prefix + "_tmat.dat"
+    if os.path.exists(tfile):  # tmatrix_solution's T-matrix file (mstm-solver-8.f90:125-130, 243-256)
+        with open(tfile) as f:
+            head = f.readline().split()
+            res["tmat_header"] = np.array([int(head[0]), int(head[1])], np.int32)
+            vals = []
+            for line in f:
+                line = line.strip()
+                if line.startswith("("):
+                    re_, im_ = line.strip("()").split(",")
+                    vals.append(complex(float(re_.replace("D", "e")), float(im_.replace("D", "e"))))
+        res["tmat"] = np.array(vals, np.complex128)
+    lfile = prefix + "_tmlog.txt"
+    if os.path.exists(lfile):  # per-order lines: l, maxiter, qeff(1:2), converr, dqeffi(1,1), ...
+        rows = []
+        with open(lfile) as f:
+            for line in f:
+                v = line.split()
+                if len(v) >= 6 and v[0].isdigit():
+                    rows.append([float(x) for x in v[:6]])
+        res["tmlog"] = np.array(rows)
     if own:
         import shutil
         shutil.rmtree(outdir, ignore_errors=True)

@@ -17,6 +17,10 @@
 ! translation orders about the mean sphere position (tranordertest),
 ! merge_to_common_origin of both polarizations, scatteringmatrix on the
 ! default 1-degree grid theta = -180..180 and fosmexpansion.
+!   mode='tmatrix'  the same translation orders, then tmatrix_solution
+!                   (mstm-solver-8.f90:22-332, iterative, serial): the
+!                   T-matrix file it writes, its per-order log and the
+!                   per-sphere efficiency sums
 ! All outputs are raw little-endian stream files <outprefix>_<name>.bin plus
 ! a small text summary <outprefix>_info.txt.
 !
@@ -39,12 +43,14 @@
       real(8) :: mie_eps,eps,cell_vf,alpha,sinc,vr,maxerr,t0,t1,rmed
       real(8) :: line(6),tmin(3),tmax(3),xg
       logical :: use_fft,store_tables,observe
-      real(8) :: teps
+      real(8) :: teps,conveps
+      real(8) :: r0t(3)
+      real(8), allocatable :: sqeff(:,:)
       complex(8), allocatable :: ain(:,:),aout(:,:),gout(:,:),amnp(:,:),pmnp0(:,:),pmnpan(:)
       real(8), allocatable :: qeff(:,:,:),qvabs(:,:)
       real(8) :: qtot(3,3)
       namelist /job/ mode,posfile,nsphere,mie_eps,use_fft,cell_vf,node_order_in,nbr_model, &
-         eps,niter,alpha,sinc,infile,outprefix,reps,store_tables,rmed,observe,teps
+         eps,niter,alpha,sinc,infile,outprefix,reps,store_tables,rmed,observe,teps,conveps
 
       mode='setup'
       posfile='cluster.pos'
@@ -65,6 +71,7 @@
       rmed=1.d0
       observe=.false.
       teps=1.d-5
+      conveps=1.d-6
       call get_command_argument(1,jobfile)
       open(10,file=trim(jobfile),status='old')
       read(10,nml=job)
@@ -187,6 +194,35 @@
          call dump_d('qeff',qeff,9*nsphere)
          call dump_d('qtot',qtot,9)
          if(observe) call observables()
+      elseif(trim(mode).eq.'tmatrix') then
+         allocate(sqeff(3,nsphere))
+! vol_radius as main_calling_program sets it (mstm-input-37.f90:1205-1211)
+         vr=0.d0
+         do i=1,nsphere
+            vr=vr+sphere_radius(i)**3
+         enddo
+         vol_radius=vr**.333333
+         write(20,'(a,es25.17)') 'vol_radius ',vol_radius
+         call set_orders(r0t)
+         write(20,'(a,i10)') 't_matrix_order_in ',t_matrix_order
+         write(20,'(a,3es25.17)') 'cluster_origin ',r0t
+         call dump_i('torder',translation_order,nsphere)
+         run_print_unit=21
+         open(21,file=trim(outprefix)//'_tmlog.txt')
+! tmatrix_solution writes its T-matrix file on unit 20: the summary is
+! closed around the call and reopened for appending
+         close(20)
+         t0=mstm_mpi_wtime()
+         call tmatrix_solution(solution_method='i',solution_eps=eps,convergence_eps=conveps, &
+            max_iterations=niter,t_matrix_file=trim(outprefix)//'_tmat.dat', &
+            sphere_qeff=sqeff,solution_status=istat)
+         t1=mstm_mpi_wtime()
+         close(21)
+         open(20,file=trim(outprefix)//'_info.txt',access='append')
+         write(20,'(a,i10)') 't_matrix_order ',t_matrix_order
+         write(20,'(a,i10)') 'istat ',istat
+         write(20,'(a,es25.17)') 'solve_seconds ',t1-t0
+         call dump_d('sqeff',sqeff,3*nsphere)
       elseif(trim(mode).eq.'special') then
          call special_dumps()
       endif
@@ -225,12 +261,14 @@
 !
 ! post-solve observables of a single-origin run (see the header)
 !
-         subroutine observables()
-         integer :: j,nt,nsm
-         real(8) :: r0(3),rt,ct,ph
-         real(8), allocatable :: sm(:,:),s00(:,:,:),s02(:,:,:),sp22(:,:,:),sm22(:,:,:)
-         complex(8) :: rimed(2),sa(4)
-         complex(8), allocatable :: a0(:,:)
+!
+! cluster origin, translation orders and t_matrix_order as
+! main_calling_program sets them (mstm-input-37.f90:1159-1197)
+!
+         subroutine set_orders(r0)
+         integer :: j
+         real(8) :: r0(3),rt
+         complex(8) :: rimed(2)
          r0=0.d0
          do j=1,nsphere
             r0=r0+sphere_position(:,j)
@@ -248,6 +286,15 @@
             translation_order(j)=min(translation_order(j),max_t_matrix_order)
             t_matrix_order=max(t_matrix_order,translation_order(j))
          enddo
+         end subroutine set_orders
+
+         subroutine observables()
+         integer :: j,nt,nsm
+         real(8) :: r0(3),ct,ph
+         real(8), allocatable :: sm(:,:),s00(:,:,:),s02(:,:,:),sp22(:,:,:),sm22(:,:,:)
+         complex(8) :: sa(4)
+         complex(8), allocatable :: a0(:,:)
+         call set_orders(r0)
          nt=t_matrix_order
          allocate(a0(2*nt*(nt+2),2))
          a0=(0.d0,0.d0)

@@ -25,6 +25,12 @@ Cases
                 so they stay disjoint), m = 1.5+0.02i: direct matvec, bare H,
                 solve, observables; and the same cluster through the FFT
                 operator (poly30_fft)
+  tmat5         5 spheres x=1, m=1.5+0.01i, seeded RSA in a sphere (fv 0.10):
+                tmatrix_solution with the reference defaults (solution and
+                convergence eps 1e-6)
+  tmat_poly6    6 spheres of mixed size (x = 0.3 .. 1.6 evenly, orders 3..5) on seeded
+                RSA centres (placed at x = 1.6, fv 0.10), m = 1.4+0.05i:
+                tmatrix_solution at solution eps 1e-8, convergence eps 1e-7
 Every solve also runs the single-origin post-processing (translation orders,
 merge_to_common_origin, scatteringmatrix on the default grid,
 fosmexpansion); those outputs are stored under obs_* keys.
@@ -95,6 +101,17 @@ def observe_case(name, pos, radius, ri):
     print(name, "neqns", neqns, "T", res["obs_T"])
 
 
+def tmatrix_case(name, pos, radius, ri, eps, conveps, niter=10000):
+    ri = np.asarray(ri, np.complex128)
+    r = O.run_reference(pos, radius, ri, "tmatrix", eps=eps, niter=niter, conveps=conveps)
+    res = dict(pos=pos, radius=radius, ri=ri, order=r["order"], eps=eps, conveps=conveps, niter=niter,
+               torder=r["torder"], origin=np.array(r["cluster_origin"]), vol_radius=r["vol_radius"],
+               T_in=int(r["t_matrix_order_in"]), T=int(r["t_matrix_order"]), istat=int(r["istat"]),
+               tmat_header=r["tmat_header"], tmat=r["tmat"], sqeff=r["sqeff"].reshape(-1, 3), tmlog=r["tmlog"])
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **res)
+    print(name, "T", res["T_in"], "->", res["T"], "orders", sorted(set(res["order"].tolist())), "tmat", res["tmat"].size)
+
+
 def main():
     c = G.rsa_in_sphere(20, 2.0, 0.10, seed=1, ri=1.5 + 0.01j)
     case("cfg1_direct", c.pos, c.radius, c.ri, seed=11)
@@ -111,7 +128,20 @@ def main():
     radius = np.round(np.random.default_rng(8).uniform(0.4, 2.4, c.n), 6)
     case("poly30", c.pos, radius, c.ri, seed=15)
     case("poly30_fft", c.pos, radius, c.ri, fft=True, seed=16)
+    c = G.rsa_in_sphere(5, 1.0, 0.10, seed=21, ri=1.5 + 0.01j)
+    tmatrix_case("tmat5", c.pos, c.radius, c.ri, eps=1e-6, conveps=1e-6)
+    c = G.rsa_in_sphere(6, 1.6, 0.10, seed=24, ri=1.4 + 0.05j)
+    radius = np.linspace(0.3, 1.6, c.n)
+    tmatrix_case("tmat_poly6", c.pos, radius, c.ri, eps=1e-8, conveps=1e-7)
+
+
+def tmatrix_only():
+    c = G.rsa_in_sphere(5, 1.0, 0.10, seed=21, ri=1.5 + 0.01j)
+    tmatrix_case("tmat5", c.pos, c.radius, c.ri, eps=1e-6, conveps=1e-6)
+    c = G.rsa_in_sphere(6, 1.6, 0.10, seed=24, ri=1.4 + 0.05j)
+    radius = np.linspace(0.3, 1.6, c.n)
+    tmatrix_case("tmat_poly6", c.pos, radius, c.ri, eps=1e-8, conveps=1e-7)
 
 
 if __name__ == "__main__":
-    main()
+    tmatrix_only() if sys.argv[1:] == ["tmatrix"] else main()
