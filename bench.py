@@ -230,6 +230,65 @@ def bench_fft(args):
     ctx.close()
 
 
+def bench_tmatrix(args):
+    """T-matrix solution (tmatrix_solution, mstm-solver-8.f90:22-332) of a
+    200-sphere cluster (x = 1, m = 1.5+0.01i, seeded RSA in a sphere at fv 0.20)
+    with the reference defaults (solution and convergence eps 1e-6): every
+    order to convergence, 2(2l+1) unit-excitation solves per order.  The FP64
+    solver with cbicg's iterates and the mixed FP32/FP64 solver are timed; the
+    reference (oracle/_ref, serial) is timed on the order-1 columns of the
+    same cluster and scaled by the number of solves."""
+    from mstm_amd import geometry as G
+    from mstm_amd.solver import SOLVE_MIXED, SOLVE_PLAIN, HipContext
+
+    cl = G.rsa_in_sphere(200, 1.0, 0.20, seed=41, ri=1.5 + 0.01j)
+    ctx = HipContext(0)
+    ctx.set_spheres(cl.pos, cl.radius, cl.ri, 1.0, 1e-6)
+    res = {}
+    for name, solver in (("plain_fp64", SOLVE_PLAIN), ("mixed", SOLVE_MIXED)):
+        for _ in range(args.warmup):
+            ctx.tmatrix_solution(eps=1e-6, conveps=1e-6, solver=solver)
+        times = []
+        for k in range(args.steps):
+            t0 = time.perf_counter()
+            r = ctx.tmatrix_solution(eps=1e-6, conveps=1e-6, solver=solver)
+            times.append(time.perf_counter() - t0)
+            print(f"[bench] tmatrix {name} step {k}: {times[-1]:.3f} s (order {r['T']})", file=sys.stderr, flush=True)
+        st = ctx.stats()
+        res[name] = dict(s=float(np.mean(times)), T=r["T"], T_in=r["T_in"], solves=int(st["tmatrix_solves"]),
+                         maxiter=r["maxiter"], istat=r["istat"], qext=float(r["qlog"][-1, 1]),
+                         qabs=float(r["qlog"][-1, 2]), tmat=r["tmat"])
+    d = float(np.abs(res["mixed"]["tmat"] - res["plain_fp64"]["tmat"]).max() / np.abs(res["plain_fp64"]["tmat"]).max())
+    nsolve = res["plain_fp64"]["solves"]
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            from oracle import oracle as O
+
+            if O.have_reference_harness():
+                t0 = time.time()
+                rr = O.run_reference(cl.pos, cl.radius, cl.ri, "tmatrix", eps=1e-6, niter=10000, conveps=1e-6,
+                                     tcap=1, timeout=600)
+                wall = time.time() - t0
+                per = float(rr["solve_seconds"]) / 6.0
+                cpu = dict(value=per * nsolve, unit="s", cores=1, kind="reference",
+                           sample=(f"reference tmatrix_solution (serial) on the order-1 columns of the same cluster: "
+                                   f"6 solves in {rr['solve_seconds']:.2f} s ({wall:.1f} s wall incl. setup), "
+                                   f"{per:.2f} s/solve x {nsolve} solves"))
+        except Exception as e:  # pragma: no cover
+            cpu = dict(value=None, unit="s", cores=1, kind="reference", sample=f"unavailable: {e}")
+    best = min(("plain_fp64", "mixed"), key=lambda k: res[k]["s"])
+    st = ctx.stats()
+    _line("s per T-matrix solution (all orders to convergence, residual 1e-6)", round(res[best]["s"], 4), "s",
+          args.steps, args.warmup, round(res[best]["s"] * 1e3, 2), "f64 (plain) / f32+f64 (mixed)",
+          "200 spheres x=1.0 m=1.5+0.01i RSA sphere fv=0.20 seed=41, T-matrix to convergence eps 1e-6",
+          {"solver": best, "order": res[best]["T"], "order_bound": res[best]["T_in"], "solves": nsolve,
+           **{k: {kk: v for kk, v in res[k].items() if kk != "tmat"} for k in res},
+           "mixed_vs_plain_tmat_rel": d, "tile4_ms_fp64": st["tile4_ms_fp64"],
+           "tile4_launches_fp64": int(st["tile4_launches_fp64"]), "cpu_baseline": cpu})
+    ctx.close()
+
+
 def observe(ctx):
     """Single-origin observables of the solution the last fixedorsoln left on the device."""
     ob = ctx.observables(None)
@@ -251,11 +310,12 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many MiB (rocprof FETCH/WRITE calibration)")
     ap.add_argument("--workload", default="direct5000",
-                    choices=["direct5000", "fft-cylinder", "fractal", "batch"],
+                    choices=["direct5000", "fft-cylinder", "fractal", "batch", "tmatrix"],
                     help="direct5000 (default, the headline line); the others print their own line: "
                          "fft-cylinder = 1000 spheres x=3 m=1.6+0.02i in a cylinder at fv 0.25 (FFT path), "
                          "fractal = 2000-monomer aggregate x=0.5 m=1.75+0.4i (FFT path, 8 orientations), "
-                         "batch = 256 clusters x 32 directions x 2 polarizations (batched MFMA solver)")
+                         "batch = 256 clusters x 32 directions x 2 polarizations (batched MFMA solver), "
+                         "tmatrix = T-matrix of a 200-sphere cluster x=1 (all orders to convergence)")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -274,6 +334,8 @@ def main():
         return bench_batch(args)
     if args.workload in ("fft-cylinder", "fractal"):
         return bench_fft(args)
+    if args.workload == "tmatrix":
+        return bench_tmatrix(args)
 
     from mstm_amd import geometry as G
     from mstm_amd.solver import FP32, FP64, SOLVE_LEARNED, SOLVE_MIXED, SOLVE_PLAIN, HipContext, qtotcalc

@@ -125,6 +125,17 @@
          complex(c_double_complex) :: amnp0(*)
          real(c_double) :: s00(2),g
          end function mstm_hip_asymmetry
+         integer(c_int) function mstm_hip_tmatrix_solution(ctx,eps,niter,conveps,solver,tmat,tmat_cap, &
+            sphere_qeff,qteff,order_log,t_order,maxiter,maxerr,istat) bind(c,name='mstm_hip_tmatrix_solution')
+         import :: c_ptr, c_int, c_int64_t, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         real(c_double), value :: eps,conveps
+         integer(c_int), value :: niter,solver
+         integer(c_int64_t), value :: tmat_cap
+         complex(c_double_complex) :: tmat(*)
+         real(c_double) :: sphere_qeff(*),qteff(*),order_log(*),maxerr
+         integer(c_int) :: t_order,maxiter,istat
+         end function mstm_hip_tmatrix_solution
          integer(c_int) function mstm_hip_clear(ctx,clear_tables) bind(c,name='mstm_hip_clear')
          import :: c_ptr, c_int
          type(c_ptr), value :: ctx

@@ -13,6 +13,7 @@
 !   hip_scatteringmatrix    <- the scatteringmatrix loop of
 !                              scattering_matrix_calculation (mstm-input-37.f90:3036-3062)
 !   hip_asymmetry           <- s00(1,1,0:1) of fosmexpansion (mstm-scatprops-26.f90:1820)
+!   hip_tmatrix_solution    <- tmatrix_solution    (mstm-solver-8.f90:22-25)
 ! The GPU context is created on first use and refilled when
 ! initialize_solver / initial_run is true (geometry, orders and an_mie are
 ! taken from the MSTM modules, so the Mie data are the host's own).
@@ -236,5 +237,89 @@
          if(.not.hip_loaded) call hip_load_state()
          call hip_check(mstm_hip_asymmetry(hip_ctx,int(nodrt,c_int),amn0,s00,g),'asymmetry')
          end subroutine hip_asymmetry
+
+!
+! tmatrix_solution (mstm-solver-8.f90:22-332), iterative branch, on the GPU
+! about the host's cluster_origin with its translation_order and
+! t_matrix_order.  Writes the T-matrix file in the reference's format
+! (:125-130, 243-256, 310-317), sets t_matrix_order to the order reached and
+! returns sphere_qeff and solution_status as the reference does.
+! solution_method 'm' selects the mixed FP32/FP64 solver; 'i' or absent the
+! FP64 one with cbicg's iterates.  procs_per_soln and mpi_comm are accepted
+! and not used (one GPU).
+!
+         subroutine hip_tmatrix_solution(solution_method,solution_eps,convergence_eps, &
+            max_iterations,t_matrix_file,procs_per_soln,mpi_comm, &
+            sphere_qeff,solution_status,sphere_excitation_list)
+         character*1, optional :: solution_method
+         character*128, optional :: t_matrix_file
+         integer, optional :: max_iterations,procs_per_soln,mpi_comm,solution_status
+         real(8), optional :: solution_eps,convergence_eps,sphere_qeff(3,number_spheres)
+         logical, optional :: sphere_excitation_list(number_spheres)
+         integer :: l,i,k,n,nt0
+         integer(c_int) :: solver,niter,tord,mi,ist
+         integer(c_int), allocatable :: tr(:)
+         integer(c_int64_t) :: ntm
+         real(c_double) :: eps,ceps,me,qt(3)
+         real(c_double), allocatable :: sq(:,:),qlog(:,:)
+         complex(c_double_complex), allocatable :: tm(:)
+         character*128 :: tfile
+         if(.not.hip_loaded) call hip_load_state()
+         solver=mstm_hip_solve_plain
+         if(present(solution_method)) then
+            if(solution_method.eq.'m') then
+               solver=mstm_hip_solve_mixed
+            elseif(solution_method.ne.'i') then
+               write(run_print_unit,'('' mstm_hip: only the iterative T-matrix solution is on the GPU path'')')
+               stop
+            endif
+         endif
+         if(present(sphere_excitation_list)) then
+            if(.not.all(sphere_excitation_list)) then
+               write(run_print_unit,'('' mstm_hip: sphere excitation lists are not on the GPU path'')')
+               stop
+            endif
+         endif
+         eps=1.d-6
+         if(present(solution_eps)) eps=solution_eps
+         ceps=1.d-6
+         if(present(convergence_eps)) ceps=convergence_eps
+         niter=100
+         if(present(max_iterations)) niter=max_iterations
+         tfile='tmatrix_temp.dat'
+         if(present(t_matrix_file)) tfile=t_matrix_file
+         nt0=t_matrix_order
+         allocate(tr(number_spheres))
+         tr=translation_order(1:number_spheres)
+         call hip_check(mstm_hip_set_translation_orders(hip_ctx,cluster_origin,tr,int(nt0,c_int)), &
+            'set_translation_orders')
+         ntm=0
+         do l=1,nt0
+            ntm=ntm+4*(2*l+1)*l*(l+2)
+         enddo
+         allocate(tm(ntm),sq(3,number_spheres),qlog(4,nt0))
+         call hip_check(mstm_hip_tmatrix_solution(hip_ctx,eps,niter,ceps,solver,tm,ntm,sq,qt,qlog,tord,mi,me,ist), &
+            'tmatrix_solution')
+         write(run_print_unit,'(''  n   # its  qext         qabs           error'')')
+         do l=1,tord
+            write(run_print_unit,'(i4,i5,3e13.5)') l,nint(qlog(1,l)),qlog(2:4,l)
+         enddo
+         open(20,file=tfile)
+         write(20,'(2i4)') tord,tord
+         i=0
+         do l=1,tord
+            do k=1,2*(2*l+1)
+               do n=1,2*l*(l+2)
+                  i=i+1
+                  write(20,'(''('',e18.10,'','',e18.10,'')'')') tm(i)
+               enddo
+            enddo
+         enddo
+         close(20)
+         t_matrix_order=tord
+         if(present(sphere_qeff)) sphere_qeff=sq
+         if(present(solution_status)) solution_status=ist
+         deallocate(tr,tm,sq,qlog)
+         end subroutine hip_tmatrix_solution
 
       end module mstm_hip_solver

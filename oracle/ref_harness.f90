@@ -20,7 +20,7 @@
 !   mode='tmatrix'  the same translation orders, then tmatrix_solution
 !                   (mstm-solver-8.f90:22-332, iterative, serial): the
 !                   T-matrix file it writes, its per-order log and the
-!                   per-sphere efficiency sums
+!                   per-sphere efficiency sums (tcap > 0 caps the order)
 ! All outputs are raw little-endian stream files <outprefix>_<name>.bin plus
 ! a small text summary <outprefix>_info.txt.
 !
@@ -44,13 +44,14 @@
       real(8) :: line(6),tmin(3),tmax(3),xg
       logical :: use_fft,store_tables,observe
       real(8) :: teps,conveps
+      integer :: tcap
       real(8) :: r0t(3)
       real(8), allocatable :: sqeff(:,:)
       complex(8), allocatable :: ain(:,:),aout(:,:),gout(:,:),amnp(:,:),pmnp0(:,:),pmnpan(:)
       real(8), allocatable :: qeff(:,:,:),qvabs(:,:)
       real(8) :: qtot(3,3)
       namelist /job/ mode,posfile,nsphere,mie_eps,use_fft,cell_vf,node_order_in,nbr_model, &
-         eps,niter,alpha,sinc,infile,outprefix,reps,store_tables,rmed,observe,teps,conveps
+         eps,niter,alpha,sinc,infile,outprefix,reps,store_tables,rmed,observe,teps,conveps,tcap
 
       mode='setup'
       posfile='cluster.pos'
@@ -72,6 +73,7 @@
       observe=.false.
       teps=1.d-5
       conveps=1.d-6
+      tcap=0
       call get_command_argument(1,jobfile)
       open(10,file=trim(jobfile),status='old')
       read(10,nml=job)
@@ -204,6 +206,8 @@
          vol_radius=vr**.333333
          write(20,'(a,es25.17)') 'vol_radius ',vol_radius
          call set_orders(r0t)
+! tcap > 0 truncates the T-matrix order (a bounded sample for timing)
+         if(tcap.gt.0) t_matrix_order=min(t_matrix_order,tcap)
          write(20,'(a,i10)') 't_matrix_order_in ',t_matrix_order
          write(20,'(a,3es25.17)') 'cluster_origin ',r0t
          call dump_i('torder',translation_order,nsphere)

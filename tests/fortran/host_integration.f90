@@ -83,6 +83,7 @@
       ! mean position as main_calling_program sets them, then the reference
       ! merge / scatteringmatrix / fosmexpansion against the hip_* forms
       call observables_check()
+      call tmatrix_check()
       if(bad.ne.0) then
          write(*,'(a)') 'HOST INTEGRATION FAILED'
          call exit(1)
@@ -133,4 +134,61 @@
          write(*,'(a,i4,3es12.4,f10.6)') 'observables T rel_err(merge,scatmat,s00) g: ',nt,e1,e2,e3,gg
          if(e1.gt.1.d-12.or.e2.gt.1.d-12.or.e3.gt.1.d-12) bad=1
          end subroutine observables_check
+!
+! tmatrix_solution against hip_tmatrix_solution on the same state, to
+! common-origin order 3 (the orders and origin of observables_check): the
+! two T-matrix files, the order reached and sphere_qeff
+!
+         subroutine tmatrix_check()
+         integer :: j,ios,st_r,st_g,h_r(2),h_g(2),nv
+         real(8) :: vr,sq_r(3,n),sq_g(3,n),e1,e2,v_r(2),v_g(2),vmax
+         character(len=128) :: f_r,f_g
+         character(len=64) :: l_r,l_g
+         vr=0.d0
+         do j=1,n
+            vr=vr+sphere_radius(j)**3
+         enddo
+         vol_radius=vr**.333333
+         cluster_origin=0.d0
+         do j=1,n
+            cluster_origin=cluster_origin+sphere_position(:,j)
+         enddo
+         cluster_origin=cluster_origin/dble(n)
+         f_r='tmat_ref.dat'
+         f_g='tmat_gpu.dat'
+         t_matrix_order=3
+         call tmatrix_solution(solution_method='i',solution_eps=1.d-6,convergence_eps=1.d-6, &
+            max_iterations=10000,t_matrix_file=f_r,sphere_qeff=sq_r,solution_status=st_r)
+         t_matrix_order=3
+         call hip_tmatrix_solution(solution_method='i',solution_eps=1.d-6,convergence_eps=1.d-6, &
+            max_iterations=10000,t_matrix_file=f_g,sphere_qeff=sq_g,solution_status=st_g)
+         open(31,file=f_r,status='old')
+         open(32,file=f_g,status='old')
+         read(31,'(2i4)') h_r
+         read(32,'(2i4)') h_g
+         e1=0.d0
+         vmax=0.d0
+         nv=0
+         do
+            read(31,'(a)',iostat=ios) l_r
+            if(ios.ne.0) exit
+            read(32,'(a)',iostat=ios) l_g
+            if(ios.ne.0) then
+               nv=-1
+               exit
+            endif
+            read(l_r(2:index(l_r,')')-1),*) v_r
+            read(l_g(2:index(l_g,')')-1),*) v_g
+            e1=max(e1,maxval(abs(v_r-v_g)))
+            vmax=max(vmax,maxval(abs(v_r)))
+            nv=nv+1
+         enddo
+         close(31)
+         close(32)
+         e1=e1/vmax
+         e2=maxval(abs(sq_r-sq_g))/maxval(abs(sq_r))
+         write(*,'(a,2i4,i8,2es12.4)') 'tmatrix_solution order(ref,gpu) values rel_err(tmat,sphere_qeff): ', &
+            h_r(1),h_g(1),nv,e1,e2
+         if(h_r(1).ne.h_g(1).or.nv.le.0.or.st_r.ne.st_g.or.e1.gt.1.d-9.or.e2.gt.1.d-10) bad=1
+         end subroutine tmatrix_check
       end program host_integration

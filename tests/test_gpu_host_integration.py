@@ -2,7 +2,9 @@
 GPU through the ISO_C_BINDING shim (mstm_amd/fortran/mstm_hip_solver.f90);
 oracle/_ref/host_integration compares each hip_* procedure with the
 reference procedure it replaces on the same state (fixedorsoln,
-sphereinteraction with A and A^H columns, cbicg) at 1e-10 relative."""
+sphereinteraction with A and A^H columns, cbicg) at 1e-10 relative, the
+single-origin observables at 1e-12 and tmatrix_solution (to order 3: the
+two T-matrix files at their 11 printed digits, sphere_qeff at 1e-10)."""
 import os
 import subprocess
 
