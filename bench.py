@@ -35,20 +35,55 @@ METRIC = "s/solve to residual 1e-6 + matvec HBM GB/s; mixed/learned vs plain FP6
 HBM_PEAK_GBPS = 8000.0
 
 
-def table_bytes_per_pair(L: int, elem: int) -> int:
-    """Minimal rotation + axial table bytes of one unordered pair (equal
-    orders L): sum_{n<=L}(2n+1)^2 reals + 2 sum_m (L-m1+1)^2 complex."""
+def direct_sym(L: int, elem: int) -> bool:
+    """The direct tiles store the symmetric table layout up to order 5 (FP32)
+    / 4 (FP64) (mstm_amd/csrc/common.hpp, direct_sym)."""
+    return L <= (5 if elem == 4 else 4)
+
+
+def survey_table_bytes_per_pair(L: int, elem: int) -> int:
+    """Full rotation + axial table of one unordered pair (equal orders L):
+    sum_{n<=L}(2n+1)^2 reals + 2 sum_{|m|<=L} (L-m1+1)^2 complex."""
     rot = sum((2 * n + 1) ** 2 for n in range(1, L + 1))
     ax = sum((L - max(1, abs(m)) + 1) ** 2 for m in range(-L, L + 1))
     return rot * elem + 2 * ax * 2 * elem
 
 
-def tile_flops_per_pair(L: int, nrhs: int = 2) -> int:
-    """Algorithmic flops of both directions of one pair for nrhs columns:
-    16 (S_rot + S_ax) per direction and column (equal orders)."""
+def sym_counts(L: int):
+    """(rotation reals, axial complex) of the symmetric layout: blocks m >= 0
+    only; half-sum/half-difference Wigner rows (n+1 values per degree for
+    m = 0, 2n+1 for m > 0) and one axial block per mode (one for m = 0)."""
+    rot = sum((n + 1) if m == 0 else (2 * n + 1) for m in range(0, L + 1) for n in range(max(1, m), L + 1))
+    ax = sum((1 if m == 0 else 2) * (L - max(1, m) + 1) ** 2 for m in range(0, L + 1))
+    return rot, ax
+
+
+def table_bytes_per_pair(L: int, elem: int) -> int:
+    """Table bytes of one unordered pair as the direct tiles store them."""
+    if direct_sym(L, elem):
+        rot, ax = sym_counts(L)
+        return rot * elem + ax * 2 * elem
+    return survey_table_bytes_per_pair(L, elem)
+
+
+def survey_tile_flops_per_pair(L: int, nrhs: int = 2) -> int:
+    """Flops of both directions of one pair for nrhs columns with full
+    rotations: 16 (S_rot + S_ax) per direction and column (equal orders)."""
     s_rot = sum((2 * n + 1) ** 2 for n in range(1, L + 1))
     s_ax = sum((L - max(1, abs(m)) + 1) ** 2 for m in range(-L, L + 1))
     return 2 * nrhs * 16 * (s_rot + s_ax)
+
+
+def tile_flops_per_pair(L: int, nrhs: int = 2, elem: int = 4) -> int:
+    """Flops the direct tiles execute for both directions of one pair and
+    nrhs columns: the symmetric rotations need R = sum of the stored row
+    lengths multiply-adds per rotation instead of S_rot; axial work is
+    unchanged: 16 (R + S_ax) per direction and column."""
+    if not direct_sym(L, elem):
+        return survey_tile_flops_per_pair(L, nrhs)
+    rot, _ = sym_counts(L)
+    s_ax = sum((L - max(1, abs(m)) + 1) ** 2 for m in range(-L, L + 1))
+    return 2 * nrhs * 16 * (rot + s_ax)
 
 
 def cpu_baseline(cluster, iters_plain: int, budget_s: float = 20.0):
@@ -452,16 +487,19 @@ def main():
     if os.path.exists(pmc):
         rec = json.load(open(pmc)).get("by_columns", {}).get(str(ncol))
         traffic = rec["hbm_bytes_per_launch"] if rec else None
+    fl32 = npairs * tile_flops_per_pair(L, ncol, 4)
+    surv32 = npairs * survey_table_bytes_per_pair(L, 4) + vec_bytes
     roof = dict(kernel=f"direct_tile_kernel<float,{L},{ncol}>", bound="hbm", achieved=round(ach32, 1),
                 peak=HBM_PEAK_GBPS, unit="GB/s", frac=round(ach32 / HBM_PEAK_GBPS, 4), traffic=traffic,
                 algorithmic_bytes=alg32, columns=ncol, avg_launch_ms=round(t32, 4),
-                flops_per_launch=npairs * tile_flops_per_pair(L, ncol),
-                tflops=round(npairs * tile_flops_per_pair(L, ncol) / (t32 * 1e-3) / 1e12, 2),
-                flop_per_byte=round(npairs * tile_flops_per_pair(L, ncol) / alg32, 2),
-                note=("algorithmic intensity below the FP32 ridge (157.3 TF / 8 TB/s = 19.7 flop/B), so quoted "
-                      "against HBM; tables are staged in LDS once per tile step (HBM traffic 1.15x algorithmic), "
-                      "VALU issues about 48% of cycles at one wave per SIMD (packed FP32 is disabled on this pool; "
-                      "SQ counters in profiles/direct_tile_cfg5_sq_counters.txt)"))
+                flops_per_launch=fl32, tflops=round(fl32 / (t32 * 1e-3) / 1e12, 2),
+                flop_per_byte=round(fl32 / alg32, 2),
+                full_table_bytes=surv32, full_table_GBps=round(surv32 / (t32 * 1e-3) / 1e9, 1),
+                note=("algorithmic bytes are the symmetric tables the kernel reads (blocks m >= 0, half-sum/"
+                      "half-difference Wigner rows, one axial block per mode): half of the full rotate-axial "
+                      "table (full_table_bytes, the count of the unsymmetrized algorithm); tables staged in LDS "
+                      "once per tile step; packed FP32 is disabled on this pool, so the FP32 vector peak is "
+                      "78.6 TF and the ridge 9.8 flop/B; SQ counters in profiles/direct_tile_cfg5_sq_counters.txt"))
     matvec = dict(fp32_tile_ms=round(t32, 4), fp32_columns=ncol, fp32_hbm_GBps=round(ach32, 1))
     if st_lig.get("tile_launches_fp32", 0) > 0:
         t2 = st_lig["tile_ms_fp32"] / st_lig["tile_launches_fp32"]

@@ -478,8 +478,7 @@ int mstm_hip_setup_direct(mstm_hip_ctx *c, int prec)
     DirectOp &d = c->dop[prec];
     const int L = c->Lmax;
     const size_t esz = prec == MSTM_HIP_FP64 ? sizeof(double) : sizeof(float);
-    const int VW = 16 / (int)esz;
-    size_t slot = (size_t)slot_len(L, L, VW);
+    size_t slot = (size_t)direct_slot_len(L, (int)esz);  // symmetric layout up to order 5 / 4
     d.tab_bytes = (size_t)c->ntiles * MSTM_SLOTS * MSTM_SLOTS * slot * esz;
     size_t nbp = (size_t)L * (L + 2);
     HIPCHK(dalloc(&d.tab, d.tab_bytes));

@@ -15,17 +15,20 @@
 //     (target, partner block): no atomics, deterministic order;
 //   * pair tables are laid out in HBM in exactly this consumption order and
 //     interleaved over the 16 slots, so each table load is 16 x 16 B;
-//   * when a step's table group (16 slots) fits in 64 KB of LDS, all waves
-//     of the tile copy it there once per step and read it from LDS: every
-//     table byte then leaves the L2 once per tile instead of once per lane
-//     that needs it (4 lanes x 2 or 4 waves);
+//   * up to order 5 (FP32) / 4 (FP64) the tables use the symmetric layout
+//     of common.hpp: only blocks m >= 0, half-sum/half-difference Wigner rows
+//     and one axial block per mode serve the blocks m and -m, which halves
+//     the table bytes and the rotation work;
+//   * in that range all waves of the tile copy a step's table group (16
+//     slots) into LDS once per step and read it there: every table byte
+//     then leaves the L2 once per tile instead of once per lane that needs
+//     it (4 lanes x 2 or 4 waves);
 //   * the pipelined form (two-column FP32, order <= 4) also copies the
 //     tile's two source blocks into LDS once per tile, instead of every lane
 //     reloading its source sphere from L2 in every step;
-//   * the back rotation sums d^t(m,k) z_m(t) over the m blocks of a step
-//     and applies the output phase exp(-i sigma k phi) once per
-//     coefficient (with LDS-staged tables, whose freed registers hold the
-//     extra accumulators); larger orders fold the phase into every term.
+//   * the back rotation sums over the m blocks of a step and applies the
+//     output phase exp(-i sigma k phi) once per coefficient (staged forms);
+//     larger orders fold the phase into every term.
 // The reduce kernel sums the partials and applies the Mie/con_tran post-op
 // of sphereinteraction (mstm-scatprops-26.f90:158-168).
 #include <utility>
@@ -74,14 +77,14 @@ __device__ __forceinline__ void load_sec(const T *__restrict__ p0, T *r)
     }
 }
 
-// LDS staging of a step's table group (see the file header)
-template <typename T, int L> constexpr bool tile_staged()
-{
-    return (int64_t)slot_len(L, L, 16 / (int)sizeof(T)) * MSTM_SLOTS * (int64_t)sizeof(T) <= 65536;
-}
-// two-column FP32 tiles up to order 4 also software-pipeline the staging:
-// step s+1's group is loaded into registers while step s computes
-template <typename T, int L, int NR> constexpr bool tile_pipelined() { return sizeof(T) == 4 && L <= 4 && NR == 2; }
+// LDS staging of a step's table group (see the file header): exactly the
+// orders with the symmetric table layout
+template <typename T, int L> constexpr bool tile_staged() { return direct_sym(L, (int)sizeof(T)); }
+// two-column FP32 tiles up to order 4 keep the tile's two source blocks in
+// LDS, and software-pipeline the staging: step s+1's group is loaded into
+// registers while step s computes
+template <typename T, int L, int NR> constexpr bool tile_src_staged() { return sizeof(T) == 4 && L <= 4 && NR == 2; }
+template <typename T, int L, int NR> constexpr bool tile_pipelined() { return tile_src_staged<T, L, NR>(); }
 // occupancy target: FP32 tiles up to order 4 fit 2 waves/SIMD without
 // spilling (the pipelined form holds its prefetch in the full register file
 // at 1 wave/SIMD instead); larger states use the full register file
@@ -92,15 +95,15 @@ template <typename T, int L, int NR> constexpr int tile_waves()
 // register prefetch of a table group for the pipelined form
 template <typename T, int L, int NR> struct GroupPrefetch {
     static constexpr int VW = 16 / sizeof(T);
-    static constexpr int NV = slot_len(L, L, VW) * MSTM_SLOTS / VW;
+    static constexpr int NV = direct_slot_len(L, sizeof(T)) * MSTM_SLOTS / VW;
     static constexpr int NT = 64 * NR;
     static constexpr int PF = tile_pipelined<T, L, NR>() ? (NV + NT - 1) / NT : 1;
     typedef typename vec_of<T, VW>::type V;
     V r[PF];
     __device__ __forceinline__ void load(const T *__restrict__ tab, int tile, int s)
     {
-        const V *g = reinterpret_cast<const V *>(tab + ((int64_t)tile * MSTM_SLOTS + s) * (int64_t)slot_len(L, L, VW) *
-                                                           MSTM_SLOTS);
+        const V *g = reinterpret_cast<const V *>(tab + ((int64_t)tile * MSTM_SLOTS + s) *
+                                                           (int64_t)direct_slot_len(L, sizeof(T)) * MSTM_SLOTS);
 #pragma unroll
         for (int q = 0; q < PF; q++) {
             const int i = threadIdx.x + q * NT;
@@ -119,28 +122,236 @@ template <typename T, int L, int NR> struct GroupPrefetch {
 };
 template <typename T, int L> constexpr int tile_lds_elems()
 {
-    return tile_staged<T, L>() ? slot_len(L, L, 16 / (int)sizeof(T)) * MSTM_SLOTS : 1;
+    return tile_staged<T, L>() ? direct_slot_len(L, sizeof(T)) * MSTM_SLOTS : 1;
 }
 // pipelined form: the tile's two source blocks (16 spheres x 2 columns x 2
 // modes x NB complex each) also sit in LDS for the whole tile
 template <typename T, int L, int NR> constexpr int tile_src_elems()
 {
-    return tile_pipelined<T, L, NR>() ? 2 * MSTM_SLOTS * 4 * L * (L + 2) * 2 : 1;
+    return tile_src_staged<T, L, NR>() ? 2 * MSTM_SLOTS * 4 * L * (L + 2) * 2 : 1;
 }
 // copy table group s of a tile into LDS, all threads of the workgroup
 template <typename T, int L, int NR>
 __device__ __forceinline__ void stage_group(const T *__restrict__ tab, int tile, int s, T *__restrict__ grp)
 {
     constexpr int VW = 16 / sizeof(T);
-    constexpr int NV = slot_len(L, L, VW) * MSTM_SLOTS / VW;
+    constexpr int NV = direct_slot_len(L, sizeof(T)) * MSTM_SLOTS / VW;
     typedef typename vec_of<T, VW>::type V;
-    const V *g = reinterpret_cast<const V *>(tab + ((int64_t)tile * MSTM_SLOTS + s) * (int64_t)slot_len(L, L, VW) *
-                                                       MSTM_SLOTS);
+    const V *g = reinterpret_cast<const V *>(tab + ((int64_t)tile * MSTM_SLOTS + s) *
+                                                       (int64_t)direct_slot_len(L, sizeof(T)) * MSTM_SLOTS);
     V *d = reinterpret_cast<V *>(grp);
     for (int q = threadIdx.x; q < NV; q += 64 * NR) d[q] = g[q];
 }
-// One wave's work on a tile; DIR = 0 forward (I <- J), 1 reverse (J <- I);
-// the wave serves right-hand sides 2*rpair and 2*rpair+1 of NR.
+// One wave's work on a tile with the symmetric table layout (staged in LDS);
+// DIR = 0 forward (I <- J), 1 reverse (J <- I); the wave serves right-hand
+// sides 2*rpair and 2*rpair+1 of NR.  Per step and m = 0..L: the rotation of
+// blocks m and -m from the half-sum/half-difference rows (see common.hpp),
+// the axial products of both (A_{-m} = A_m of the other mode), and the back
+// rotation into P/Q; the output phase is applied once per coefficient.
+template <typename T, int L, int DIR, int NR>
+__device__ __forceinline__ void tile_body_sym(int2 bt, int tile, const T *__restrict__ tab, cplx<T> *__restrict__ part,
+                                              int npad, int rpair, T *__restrict__ grp,
+                                              const cplx<T> *__restrict__ srcl, const cplx<T> *__restrict__ xin)
+{
+    constexpr bool PIPE = tile_pipelined<T, L, NR>();
+    constexpr int VW = 16 / sizeof(T);
+    constexpr int NB = L * (L + 2);
+    constexpr int S = MSTM_SLOTS;
+    const int lane = threadIdx.x & 63;
+    const int kslot = lane >> 2, rhs = 2 * rpair + ((lane >> 1) & 1), mode = lane & 1;
+    const bool diag = bt.x == bt.y;
+    const int tgt = (DIR ? bt.y : bt.x) * S + kslot;
+
+    cplx<T> W[NB];
+#pragma unroll
+    for (int q = 0; q < NB; q++) W[q] = cmk((T)0, (T)0);
+
+    GroupPrefetch<T, L, NR> pre;
+    if constexpr (PIPE) pre.load(tab, tile, 0);
+
+#pragma unroll 1
+    for (int s = 0; s < S; s++) {
+        int slot, src;
+        if (!DIR) {
+            slot = kslot;
+            src = bt.y * S + ((kslot + s) & (S - 1));
+        } else {
+            slot = (kslot - s + S) & (S - 1);
+            src = bt.x * S + slot;
+        }
+        const cplx<T> *xs = xin + ((int64_t)(rhs * npad + src) * 2 + mode) * NB;
+        // staged copy: block 0 = bt.y (forward sources), 1 = bt.x (reverse sources)
+        const cplx<T> *xl = srcl + (((DIR * S + (src % S)) * 2 + (rhs & 1)) * 2 + mode) * NB;
+        __syncthreads();
+        cplx<T> xv[PIPE ? NB : 1];
+        if constexpr (PIPE) {
+            pre.store(grp);
+            if (!(diag && s == 0)) {
+#pragma unroll
+                for (int n = 1; n <= L; n++)
+#pragma unroll
+                    for (int k = -n; k <= n; k++) xv[nk(n, k)] = xl[amn(DIR ? -k : k, n, L)];
+            }
+            if (s + 1 < S) pre.load(tab, tile, s + 1);
+        } else {
+            stage_group<T, L, NR>(tab, tile, s, grp);
+        }
+        __syncthreads();
+        if (diag && s == 0) continue;
+        const T *p0 = grp + slot * VW;
+        // exp(i sigma k phi), sigma = +1 forward, -1 reverse (im of :6690)
+        T pv[2];
+        load_sec<T, sphi_off(L, VW), 2>(p0, pv);
+        cplx<T> e1 = cmk(pv[0], DIR ? -pv[1] : pv[1]);
+        cplx<T> ph[2 * L + 1];
+        ph[L] = cmk((T)1, (T)0);
+#pragma unroll
+        for (int k = 1; k <= L; k++) {
+            ph[L + k] = cmul(ph[L + k - 1], e1);
+            ph[L - k] = cconj(ph[L + k]);
+        }
+        // source coefficients (m-flip for the reverse direction, phase), then
+        // in place u_k at k >= 0 and v_k at -k
+        cplx<T> xp[NB];
+#pragma unroll
+        for (int n = 1; n <= L; n++)
+#pragma unroll
+            for (int k = -n; k <= n; k++) {
+                cplx<T> v;
+                if constexpr (PIPE) v = xv[nk(n, k)];
+                else if constexpr (tile_src_staged<T, L, NR>()) v = xl[amn(DIR ? -k : k, n, L)];
+                else v = xs[amn(DIR ? -k : k, n, L)];
+                if (DIR && (k & 1)) v = cmk(-v.re, -v.im);
+                xp[nk(n, k)] = cmul(ph[L + k], v);
+            }
+#pragma unroll
+        for (int n = 1; n <= L; n++)
+#pragma unroll
+            for (int k = 1; k <= n; k++) {
+                const cplx<T> a = xp[nk(n, k)], b = xp[nk(n, -k)];
+                if (k & 1) {
+                    xp[nk(n, k)] = csub(a, b);
+                    xp[nk(n, -k)] = cadd(a, b);
+                } else {
+                    xp[nk(n, k)] = cadd(a, b);
+                    xp[nk(n, -k)] = csub(a, b);
+                }
+            }
+        // back-rotation accumulators: P_k at k >= 0, Q_k at -k
+        cplx<T> PQ[NB];
+#pragma unroll
+        for (int q = 0; q < NB; q++) PQ[q] = cmk((T)0, (T)0);
+        static_for<L + 1>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            constexpr int m1 = imx(1, m);
+            constexpr int E0 = ssec_off(m, L, VW);
+            constexpr int RL = srot_len(m, L);
+            constexpr int AL = sax_len(m, L);
+            constexpr int AP = pad_to(AL, VW);
+            constexpr int ND = L - m1 + 1;
+            constexpr T sgm = (m & 1) ? (T)-1 : (T)1;
+            T dr[RL];
+            load_sec<T, E0, RL>(p0, dr);
+            // forward rotation of blocks m (y) and -m (yn)
+            cplx<T> y[ND], yn[ND];
+#pragma unroll
+            for (int n = m1; n <= L; n++) {
+                const int ro = srot_deg(m, n);
+                cplx<T> yp = cmk((T)0, (T)0);
+#pragma unroll
+                for (int k = 0; k <= n; k++) crfma(yp, dr[ro + k], xp[nk(n, k)]);
+                if constexpr (m == 0) {
+                    y[n - m1] = yp;
+                } else {
+                    cplx<T> ym = cmk((T)0, (T)0);
+#pragma unroll
+                    for (int k = 1; k <= n; k++) crfma(ym, dr[ro + n + k], xp[nk(n, -k)]);
+                    y[n - m1] = cadd(yp, ym);
+                    yn[n - m1] = cscale(csub(yp, ym), sgm);
+                }
+            }
+            // axial products: A_m of this lane's mode for m, of the other mode for -m
+            // (transposed for the reverse direction)
+            cplx<T> z[ND], zn[ND];
+            {
+                T da[AL];
+                load_sec<T, E0 + pad_to(RL, VW), AL>(p0 + (int64_t)(m == 0 ? 0 : mode) * (AP / VW) * S * VW, da);
+#pragma unroll
+                for (int t = 0; t < ND; t++) {
+                    cplx<T> acc = cmk((T)0, (T)0);
+#pragma unroll
+                    for (int q = 0; q < ND; q++) {
+                        const int e = DIR ? 2 * (q + ND * t) : 2 * (t + ND * q);
+                        cfma(acc, cmk(da[e], da[e + 1]), y[q]);
+                    }
+                    z[t] = acc;
+                }
+            }
+            if constexpr (m > 0) {
+                T db[AL];
+                load_sec<T, E0 + pad_to(RL, VW), AL>(p0 + (int64_t)(1 - mode) * (AP / VW) * S * VW, db);
+#pragma unroll
+                for (int t = 0; t < ND; t++) {
+                    cplx<T> acc = cmk((T)0, (T)0);
+#pragma unroll
+                    for (int q = 0; q < ND; q++) {
+                        const int e = DIR ? 2 * (q + ND * t) : 2 * (t + ND * q);
+                        cfma(acc, cmk(db[e], db[e + 1]), yn[q]);
+                    }
+                    zn[t] = acc;
+                }
+            }
+            // back rotation: P_k += E_k (z_m + (-1)^m z_-m), Q_k += O_k (z_m - (-1)^m z_-m)
+#pragma unroll
+            for (int t = m1; t <= L; t++) {
+                const int ro = srot_deg(m, t);
+                cplx<T> za = z[t - m1], zb = z[t - m1];
+                if constexpr (m > 0) {
+                    za = cadd(z[t - m1], cscale(zn[t - m1], sgm));
+                    zb = csub(z[t - m1], cscale(zn[t - m1], sgm));
+                }
+#pragma unroll
+                for (int k = 0; k <= t; k++) crfma(PQ[nk(t, k)], dr[ro + k], za);
+                if constexpr (m > 0) {
+#pragma unroll
+                    for (int k = 1; k <= t; k++) crfma(PQ[nk(t, -k)], dr[ro + t + k], zb);
+                }
+            }
+        });
+        // U_0 = P_0, U_k = P_k + Q_k, U_-k = (-1)^k (P_k - Q_k); output phase once per coefficient
+#pragma unroll
+        for (int t = 1; t <= L; t++) {
+            cfma(W[nk(t, 0)], ph[L], PQ[nk(t, 0)]);
+#pragma unroll
+            for (int k = 1; k <= t; k++) {
+                const cplx<T> pk = PQ[nk(t, k)], qk = PQ[nk(t, -k)];
+                const cplx<T> up = cadd(pk, qk);
+                const cplx<T> um = (k & 1) ? csub(qk, pk) : csub(pk, qk);
+                cfma(W[nk(t, k)], ph[L - k], up);
+                cfma(W[nk(t, -k)], ph[L + k], um);
+            }
+        }
+    }
+    // partial for (target, partner block); m-flip back for the reverse wave
+    const int partner = DIR ? bt.x : bt.y;
+    cplx<T> *out = part + (((int64_t)partner * NR + rhs) * npad + tgt) * 2 * NB + mode * NB;
+#pragma unroll
+    for (int n = 1; n <= L; n++)
+#pragma unroll
+        for (int k = -n; k <= n; k++) {
+            cplx<T> v;
+            if (!DIR) v = W[nk(n, k)];
+            else {
+                v = W[nk(n, -k)];
+                if (k & 1) v = cmk(-v.re, -v.im);
+            }
+            out[amn(k, n, L)] = v;
+        }
+}
+
+// One wave's work on a tile with the full table layout (orders above the
+// symmetric range): tables read from L2/HBM directly, output phase folded
+// into every term; DIR and rpair as for tile_body_sym.
 template <typename T, int L, int DIR, int NR>
 __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, int tile, const cplx<T> *__restrict__ xin,
                                           cplx<T> *__restrict__ part, int npad, int rpair, T *__restrict__ grp,
@@ -323,7 +534,7 @@ __global__ __launch_bounds__(64 * NR) __attribute__((amdgpu_waves_per_eu(tile_wa
     __shared__ __attribute__((aligned(16))) T grp[tile_lds_elems<T, L>()];
     __shared__ __attribute__((aligned(16))) T srcs[tile_src_elems<T, L, NR>()];
     cplx<T> *srcl = reinterpret_cast<cplx<T> *>(srcs);
-    if constexpr (tile_pipelined<T, L, NR>()) {
+    if constexpr (tile_src_staged<T, L, NR>()) {
         // the two source blocks of the tile, all threads; visible after the
         // first step's barrier
         constexpr int NB = L * (L + 2), BLK = MSTM_SLOTS * 2 * NB;  // one column r: 16 spheres x 2 modes
@@ -335,9 +546,14 @@ __global__ __launch_bounds__(64 * NR) __attribute__((amdgpu_waves_per_eu(tile_wa
                 xin[((int64_t)(r * npad + blk * MSTM_SLOTS + i) * 2) * NB + mo];
         }
     }
-    if (!(w & 1)) tile_body<T, L, 0, NR>(tab, bt, tile, xin, part, npad, w >> 1, grp, srcl);
-    else if (bt.x != bt.y) tile_body<T, L, 1, NR>(tab, bt, tile, xin, part, npad, w >> 1, grp, srcl);
-    else
+    if constexpr (tile_staged<T, L>()) {
+        if (!(w & 1)) tile_body_sym<T, L, 0, NR>(bt, tile, tab, part, npad, w >> 1, grp, srcl, xin);
+        else if (bt.x != bt.y) tile_body_sym<T, L, 1, NR>(bt, tile, tab, part, npad, w >> 1, grp, srcl, xin);
+    } else {
+        if (!(w & 1)) tile_body<T, L, 0, NR>(tab, bt, tile, xin, part, npad, w >> 1, grp, srcl);
+        else if (bt.x != bt.y) tile_body<T, L, 1, NR>(tab, bt, tile, xin, part, npad, w >> 1, grp, srcl);
+    }
+    if (bt.x == bt.y && (w & 1))
     {
         // idle reverse wave of a diagonal tile: its share of the staging and the same barriers
         GroupPrefetch<T, L, NR> pre;

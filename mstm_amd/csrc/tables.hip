@@ -13,7 +13,7 @@
 // along rtran = r_target - r_source, written interleaved for slot k.
 template <typename T>
 __device__ void build_slot(T *__restrict__ grp, int k, int LT, int LS, int itype, const double *rt, double rimed,
-                           const double *__restrict__ vcc, int LM)
+                           const double *__restrict__ vcc, int LM, bool sym = false)
 {
     const int VW = 16 / (int)sizeof(T);
     const int nmax = imx(LT, LS), nmin = imn(LT, LS);
@@ -64,7 +64,18 @@ __device__ void build_slot(T *__restrict__ grp, int k, int LT, int LS, int itype
                 grp[ilv(e0 + kk + n, k, VW)] = (T)v;
             }
         };
-        put_row(0, rowp, 0.0);
+        // symmetric layout (LT = LS): half-sum / half-difference rows of block m >= 0
+        auto put_sym = [&](int m, const double *row) {
+            int e0 = ssec_off(m, LT, VW) + srot_deg(m, n);
+            for (int kk = 0; kk <= n; kk++) {
+                double sg = (kk & 1) ? -1.0 : 1.0;
+                double e = (kk == 0) ? row[n] : 0.5 * (row[kk + n] + sg * row[-kk + n]);
+                grp[ilv(e0 + kk, k, VW)] = (T)e;
+                if (m > 0 && kk > 0) grp[ilv(e0 + n + kk, k, VW)] = (T)(0.5 * (row[kk + n] - sg * row[-kk + n]));
+            }
+        };
+        if (sym) put_sym(0, rowp);
+        else put_row(0, rowp, 0.0);
         int im = 1;
         for (int m = 1; m <= kmx; m++) {
             im = -im;
@@ -79,9 +90,13 @@ __device__ void build_slot(T *__restrict__ grp, int k, int LT, int LS, int itype
                                 (double)kk * sbe * rowp[kk + n]) *
                                fmn;
             }
-            put_row(m, rowc, 0.0);
-            // d(-m, -k) = d(m, k) (-1)^(k+m)
-            put_row(-m, rowc, (double)im);
+            if (sym) {
+                put_sym(m, rowc);
+            } else {
+                put_row(m, rowc, 0.0);
+                // d(-m, -k) = d(m, k) (-1)^(k+m)
+                put_row(-m, rowc, (double)im);
+            }
             for (int kk = -n; kk <= n; kk++) rowp[kk + n] = rowc[kk + n];
         }
     }
@@ -100,6 +115,7 @@ __device__ void build_slot(T *__restrict__ grp, int k, int LT, int LS, int itype
 #define ACT(n, l) act[(n) * (MSTM_MAX_L + 1) + (l)]
     for (int b = 0; b < 2 * nmin + 1; b++) {
         int m = mblock_m(b);
+        if (sym && m < 0) continue;  // A_{-m} of one mode is A_m of the other
         int m1 = imx(1, iab(m));
         // seed: row n = m1 over all l, column l = LS over n > m1
         for (int pass = 0; pass < 2; pass++) {
@@ -133,9 +149,10 @@ __device__ void build_slot(T *__restrict__ grp, int k, int LT, int LS, int itype
                 }
             }
         }
-        int e0 = sec_off(b, LT, LS, VW) + pad_to(rot_len(m, LT, LS), VW);
-        int ap = pad_to(ax_len(m, LT, LS), VW);
-        for (int p = 0; p < 2; p++)
+        int e0 = sym ? ssec_off(m, LT, VW) + pad_to(srot_len(m, LT), VW)
+                     : sec_off(b, LT, LS, VW) + pad_to(rot_len(m, LT, LS), VW);
+        int ap = sym ? pad_to(sax_len(m, LT), VW) : pad_to(ax_len(m, LT, LS), VW);
+        for (int p = 0; p < (sym ? sax_modes(m) : 2); p++)
             for (int l = m1; l <= LS; l++)
                 for (int n = m1; n <= LT; n++) {
                     int e = e0 + p * ap + 2 * ((n - m1) + (LT - m1 + 1) * (l - m1));
@@ -144,7 +161,7 @@ __device__ void build_slot(T *__restrict__ grp, int k, int LT, int LS, int itype
                 }
     }
 #undef ACT
-    int ep = phi_off(LT, LS, VW);
+    int ep = sym ? sphi_off(LT, VW) : phi_off(LT, LS, VW);
     grp[ilv(ep, k, VW)] = (T)cph;
     grp[ilv(ep + 1, k, VW)] = (T)sph;
 }
@@ -173,15 +190,14 @@ __global__ void direct_tables_kernel(T *__restrict__ tab, const int2 *__restrict
         return;
     }
     double rt[3] = {pos[3 * it] - pos[3 * is], pos[3 * it + 1] - pos[3 * is + 1], pos[3 * it + 2] - pos[3 * is + 2]};
-    build_slot<T>(grp, k, L, L, 3, rt, rimed, vcc, LM);
+    build_slot<T>(grp, k, L, L, 3, rt, rimed, vcc, LM, direct_sym(L, (int)sizeof(T)));
 }
 
 template <typename T>
 hipError_t launch_direct_tables(T *tab, const int2 *tiles, int ntiles, int L, int nsph, const double *pos, double rimed,
                                 const double *vcc, int LM, hipStream_t st)
 {
-    const int VW = 16 / (int)sizeof(T);
-    int64_t se = slot_len(L, L, VW);
+    int64_t se = direct_slot_len(L, (int)sizeof(T));
     int64_t nq = (int64_t)ntiles * MSTM_SLOTS * MSTM_SLOTS;
     int bs = 128;
     int64_t nb = (nq + bs - 1) / bs;
