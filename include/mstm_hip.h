@@ -39,6 +39,10 @@ extern "C" {
 /* FP64 flexible GMRES(8) with FP32 BiCG inner solves; a loaded learned
  * network is the inner preconditioner of the first Arnoldi vector */
 #define MSTM_HIP_SOLVE_FGMRES 3
+/* direct solution (solution_method other than 'i'): dense A = I - T H and its
+ * LU factors (up to 16384 equations), both polarizations back-substituted;
+ * iterations 0 */
+#define MSTM_HIP_SOLVE_LU 4
 
 typedef struct mstm_hip_ctx mstm_hip_ctx;
 
@@ -164,6 +168,17 @@ int mstm_hip_scattering_matrix(mstm_hip_ctx *ctx, int T, const double *amnp0, in
  * scattering cross section (k^-2 units).  amnp0/T as for the scattering
  * matrix.  s00 (2 doubles) or g may be NULL. */
 int mstm_hip_asymmetry(mstm_hip_ctx *ctx, int T, const double *amnp0, double *s00, double *g);
+
+/* direct_solver (mstm-solver-8.f90:571-642): anp = A^{-1} pnp with the dense
+ * interaction matrix A = I - T H (general_interaction_matrix with mie_mult,
+ * mstm-modules-33.f90:5854-5985, built from the FP64 operator) and its LU
+ * factors (lu_decomposition's scaled partial pivoting, :951-1020), made when
+ * initialize != 0 or none are held (they stay until the spheres change or
+ * mstm_hip_clear(ctx, 1)); then niter refinement steps r = A x - p,
+ * x -= LU^{-1} r.  solution_error (may be NULL): sqrt(sum |r|^2) / neqns of
+ * the last step, 0 when niter = 0.  Up to 16384 equations. */
+int mstm_hip_direct_solver(mstm_hip_ctx *ctx, const double *pnp, double *anp, int initialize, int niter,
+                           double *solution_error);
 
 /* tmatrix_solution (mstm-solver-8.f90:22-332), iterative, one GPU: the
  * cluster T-matrix about the origin of the last translation-order call

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmstm_hip.so")
-SOURCES = ["context", "tables", "direct", "mie", "krylov", "fft", "learned", "batch", "observe"]
+SOURCES = ["context", "tables", "direct", "mie", "krylov", "fft", "learned", "batch", "observe", "lu"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MSTM_OFFLOAD_ARCH", "gfx950")
 # packed FP32 ops stay disabled for all device code (known-bad forms on this pool)

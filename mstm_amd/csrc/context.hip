@@ -142,6 +142,14 @@ struct mstm_hip_ctx {
         double tmatrix_ms = 0;
         int tm_solves = 0;
     } obs;
+    // dense interaction matrix A = I - T H and its LU factors (direct solution)
+    struct {
+        double *A = nullptr, *LU = nullptr, *vv = nullptr, *work = nullptr;
+        int *piv = nullptr, *err = nullptr;
+        int n = 0;
+        bool ready = false;
+        double build_ms = 0, factor_ms = 0, solve_ms = 0;
+    } lu;
 };
 
 static int fail(mstm_hip_ctx *c, int code, const char *fmt, ...)
@@ -249,8 +257,21 @@ static void release_lig_buffers(mstm_hip_ctx *c)
     c->lig.cap = 0;
 }
 
+static void release_lu(mstm_hip_ctx *c)
+{
+    dfree(c->lu.A);
+    dfree(c->lu.LU);
+    dfree(c->lu.vv);
+    dfree(c->lu.work);
+    dfree(c->lu.piv);
+    dfree(c->lu.err);
+    c->lu.n = 0;
+    c->lu.ready = false;
+}
+
 static void release_spheres(mstm_hip_ctx *c)
 {
+    release_lu(c);
     release_lig_buffers(c);
     release_direct(c, 0);
     release_direct(c, 1);
@@ -1267,6 +1288,101 @@ static int fgmres_dev(mstm_hip_ctx *c, int niter, double eps, const double *b, d
 
 extern "C" {
 
+// ---- dense LU (direct_solver, mstm-solver-8.f90:571-642) ----------------
+static const int LU_MAX_N = 16384;  // two n x n complex FP64 matrices: 8.6 GB at the limit
+
+// A = I - T H column by column (the FP64 operator on unit vectors, four
+// columns per product), then its LU factors with the reference's scaled
+// pivot rule
+static int lu_setup(mstm_hip_ctx *c)
+{
+    const int n = c->neqns;
+    if (n > LU_MAX_N) return fail(c, MSTM_HIP_EUNSUP, "direct solver: %d equations > %d", n, LU_MAX_N);
+    if (c->lu.n != n) {
+        release_lu(c);
+        const size_t mat = sizeof(double) * 2 * (size_t)n * n;
+        HIPCHK(dalloc(&c->lu.A, mat));
+        HIPCHK(dalloc(&c->lu.LU, mat));
+        HIPCHK(dalloc(&c->lu.vv, sizeof(double) * n));
+        HIPCHK(dalloc(&c->lu.work, sizeof(double) * 2 * (size_t)n * 8));
+        HIPCHK(dalloc(&c->lu.piv, sizeof(int) * n));
+        HIPCHK(dalloc(&c->lu.err, sizeof(int)));
+        c->lu.n = n;
+    }
+    c->lu.ready = false;
+    auto t0 = std::chrono::steady_clock::now();
+    const int flags[4] = {2, 2, 2, 2};  // mie_mult, no con_tran
+    double *ue = c->lu.work, *th = c->lu.work + 2 * (size_t)n * 4;
+    const bool timing = c->timing;  // the build's products stay out of the tile timings
+    c->timing = false;
+    int rc = 0;
+    for (int j0 = 0; j0 < n && !rc; j0 += 4) {
+        const int nc = std::min(4, n - j0);
+        hipError_t e = launch_lu_unit(ue, n, n, j0, nc, c->st);
+        if (e == hipSuccess) rc = direct_matvec(c, nc, ue, th, flags, MSTM_HIP_FP64);
+        if (e == hipSuccess && !rc) e = launch_lu_store(c->lu.A, n, j0, nc, th, n, c->st);
+        if (e != hipSuccess) rc = fail(c, MSTM_HIP_EHIP, "direct solver: %s", hipGetErrorString(e));
+    }
+    c->timing = timing;
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    auto t1 = std::chrono::steady_clock::now();
+    c->lu.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    HIPCHK(hipMemcpyAsync(c->lu.LU, c->lu.A, sizeof(double) * 2 * (size_t)n * n, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemsetAsync(c->lu.err, 0, sizeof(int), c->st));
+    HIPCHK(launch_lu_factor(c->lu.LU, n, c->lu.vv, c->lu.piv, c->lu.err, c->st));
+    int err = 0;
+    HIPCHK(hipMemcpyAsync(&err, c->lu.err, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->lu.factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+    if (err) return fail(c, MSTM_HIP_EARG, "direct solver: lu decomposition failed (a zero row)");
+    c->lu.ready = true;
+    return 0;
+}
+
+// x <- LU^{-1} P x for nrhs device vectors 2*neqns doubles apart
+static int lu_solve_dev(mstm_hip_ctx *c, double *x, int nrhs)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(launch_lu_solve(c->lu.LU, c->lu.n, c->lu.piv, x, c->lu.n, nrhs, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->lu.solve_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+int mstm_hip_direct_solver(mstm_hip_ctx *c, const double *pnp, double *anp, int initialize, int niter,
+                           double *solution_error)
+{
+    if (!c || !c->nsph) return fail(c, MSTM_HIP_ESTATE, "direct_solver: no spheres");
+    if (!pnp || !anp || niter < 0) return fail(c, MSTM_HIP_EARG, "direct_solver: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    c->obs.x_valid = false;
+    if (initialize || !c->lu.ready)
+        if (int rc = lu_setup(c)) return rc;
+    const int n = c->neqns;
+    double *p = c->lu.work, *x = c->lu.work + 2 * (size_t)n, *r = c->lu.work + 4 * (size_t)n;
+    HIPCHK(hipMemcpyAsync(p, pnp, sizeof(double) * 2 * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(x, p, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, c->st));
+    if (int rc = lu_solve_dev(c, x, 1)) return rc;
+    // niter refinement steps r = A x - p, x <- x - LU^{-1} r (:627-633)
+    double serr = 0.0;
+    std::vector<double> hr(2 * (size_t)n);
+    for (int it = 0; it < niter; it++) {
+        HIPCHK(launch_lu_resid(c->lu.A, n, x, p, r, c->st));
+        HIPCHK(hipMemcpyAsync(hr.data(), r, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        double ss = 0.0;
+        for (int i = 0; i < n; i++) ss += hr[2 * i] * hr[2 * i] + hr[2 * i + 1] * hr[2 * i + 1];
+        serr = std::sqrt(ss) / (double)n;
+        if (int rc = lu_solve_dev(c, r, 1)) return rc;
+        HIPCHK(launch_zaxpby(-1.0, 0.0, r, 1.0, 0.0, x, n, c->st));
+    }
+    HIPCHK(hipMemcpyAsync(anp, x, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (solution_error) *solution_error = serr;
+    return 0;
+}
+
 int mstm_hip_sphereinteraction(mstm_hip_ctx *c, int nrhs, const double *ain, double *aout, const int *con_tran,
                                const int *mie_mult, int use_fft, int prec)
 {
@@ -1315,8 +1431,10 @@ int mstm_hip_fixedorsoln(mstm_hip_ctx *c, double alpha, double sinc, int dir, do
     if (dir != 1 && dir != 2) return fail(c, MSTM_HIP_EARG, "fixedorsoln: dir must be 1 or 2");
     if (niter <= 0) return fail(c, MSTM_HIP_EUNSUP, "fixedorsoln: niter <= 0 not supported");
     if (solver != MSTM_HIP_SOLVE_PLAIN && solver != MSTM_HIP_SOLVE_MIXED && solver != MSTM_HIP_SOLVE_LEARNED &&
-        solver != MSTM_HIP_SOLVE_FGMRES)
+        solver != MSTM_HIP_SOLVE_FGMRES && solver != MSTM_HIP_SOLVE_LU)
         return fail(c, MSTM_HIP_EUNSUP, "fixedorsoln: solver %d not available", solver);
+    if (solver == MSTM_HIP_SOLVE_LU && use_fft)
+        return fail(c, MSTM_HIP_EUNSUP, "fixedorsoln: the direct solution uses the direct operator");
     if (!amnp) return fail(c, MSTM_HIP_EARG, "fixedorsoln: amnp is required");
     if (solver == MSTM_HIP_SOLVE_LEARNED && !c->lig.ready)
         return fail(c, MSTM_HIP_ESTATE, "fixedorsoln: the learned solver needs a network (mstm_hip_set_learned)");
@@ -1348,6 +1466,11 @@ int mstm_hip_fixedorsoln(mstm_hip_ctx *c, double alpha, double sinc, int dir, do
     const double eps2[2] = {eps, eps};
     if (solver == MSTM_HIP_SOLVE_PLAIN) {
         rc = bicg2_dev(c, 2, niter, eps2, b, x, MSTM_HIP_FP64, use_fft, it, err);
+    } else if (solver == MSTM_HIP_SOLVE_LU) {
+        // direct_solver with number_iterations = 0 (:491-493): LU of A once,
+        // both polarizations back-substituted together; iter = 0
+        if (!c->lu.ready) rc = lu_setup(c);
+        if (!rc) rc = lu_solve_dev(c, x, 2);
     } else if (solver == MSTM_HIP_SOLVE_MIXED) {
         // the refinement starts from x = 0 (residual b, no product): the
         // reference's start x = b has |r|^2/|b|^2 >> 1 on dense clusters
@@ -1538,6 +1661,7 @@ int mstm_hip_clear(mstm_hip_ctx *c, int clear_tables)
     if (clear_tables) {
         release_direct(c, 0);
         release_direct(c, 1);
+        release_lu(c);
         if (c->fft) fft_clear(c->fft);
     }
     c->stats = Stats();
@@ -1552,7 +1676,8 @@ const char *mstm_hip_stats_names(void)
            "fft_setup_ms,fft_conv_ms,fft_launches,npad,ntiles,nblocks,lig_solves,lig_fallbacks,lig_ms,lig_err0,"
            "lig_errb,fgmres_steps,tile4_ms_fp64,tile4_ms_fp32,tile4_launches_fp64,tile4_launches_fp32,inner_solver,"
            "obs_orders_ms,obs_merge_ms,obs_scatmat_ms,obs_moments_ms,obs_T,fft_contract_ms_fp32,"
-           "fft_contract_launches_fp32,fft_contract_ms_fp64,fft_contract_launches_fp64,tmatrix_ms,tmatrix_solves";
+           "fft_contract_launches_fp32,fft_contract_ms_fp64,fft_contract_launches_fp64,tmatrix_ms,tmatrix_solves,"
+           "lu_build_ms,lu_factor_ms,lu_solve_ms";
 }
 
 int mstm_hip_stats(const mstm_hip_ctx *c, double *out, int nout)
@@ -1599,7 +1724,10 @@ int mstm_hip_stats(const mstm_hip_ctx *c, double *out, int nout)
                   c->fft ? fft_stat(c->fft, 5) : 0.0,
                   c->fft ? fft_stat(c->fft, 6) : 0.0,
                   c->obs.tmatrix_ms,
-                  (double)c->obs.tm_solves};
+                  (double)c->obs.tm_solves,
+                  c->lu.build_ms,
+                  c->lu.factor_ms,
+                  c->lu.solve_ms};
     int k = std::min(nout, (int)(sizeof v / sizeof v[0]));
     for (int q = 0; q < k; q++) out[q] = v[q];
     return k;

@@ -68,5 +68,12 @@ hipError_t launch_distribute(int l, const int *order, const int *offset, const d
                              const double *r0, double rimed, const double *vccd, double *out, int64_t ldc, int G,
                              hipStream_t st);
 int s00_moment_blocks(int T);
+
+// lu.hip (complex FP64, column-major, leading dimension n)
+hipError_t launch_lu_unit(double *x, int64_t ld, int n, int j0, int ncol, hipStream_t st);
+hipError_t launch_lu_store(double *A, int n, int j0, int ncol, const double *th, int64_t ld, hipStream_t st);
+hipError_t launch_lu_factor(double *LUm, int n, double *vv, int *piv, int *err, hipStream_t st);
+hipError_t launch_lu_solve(const double *LUm, int n, const int *piv, double *b, int64_t ld, int nrhs, hipStream_t st);
+hipError_t launch_lu_resid(const double *A, int n, const double *x, const double *p, double *r, hipStream_t st);
 hipError_t launch_s00_moments(const double *a0, int T, const double *cgm, const double *cg1, double *out,
                               hipStream_t st);

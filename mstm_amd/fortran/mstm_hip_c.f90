@@ -9,7 +9,7 @@
       implicit none
       integer(c_int), parameter :: mstm_hip_fp64=0, mstm_hip_fp32=1
       integer(c_int), parameter :: mstm_hip_solve_plain=0, mstm_hip_solve_mixed=1, &
-         mstm_hip_solve_learned=2, mstm_hip_solve_fgmres=3
+         mstm_hip_solve_learned=2, mstm_hip_solve_fgmres=3, mstm_hip_solve_lu=4
       interface
          integer(c_int) function mstm_hip_create(ctx,device) bind(c,name='mstm_hip_create')
          import :: c_ptr, c_int
@@ -125,6 +125,14 @@
          complex(c_double_complex) :: amnp0(*)
          real(c_double) :: s00(2),g
          end function mstm_hip_asymmetry
+         integer(c_int) function mstm_hip_direct_solver(ctx,pnp,anp,initialize,niter,solution_error) &
+            bind(c,name='mstm_hip_direct_solver')
+         import :: c_ptr, c_int, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         complex(c_double_complex) :: pnp(*),anp(*)
+         integer(c_int), value :: initialize,niter
+         real(c_double) :: solution_error
+         end function mstm_hip_direct_solver
          integer(c_int) function mstm_hip_tmatrix_solution(ctx,eps,niter,conveps,solver,tmat,tmat_cap, &
             sphere_qeff,qteff,order_log,t_order,maxiter,maxerr,istat) bind(c,name='mstm_hip_tmatrix_solution')
          import :: c_ptr, c_int, c_int64_t, c_double, c_double_complex

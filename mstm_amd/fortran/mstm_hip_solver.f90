@@ -14,6 +14,7 @@
 !                              scattering_matrix_calculation (mstm-input-37.f90:3036-3062)
 !   hip_asymmetry           <- s00(1,1,0:1) of fosmexpansion (mstm-scatprops-26.f90:1820)
 !   hip_tmatrix_solution    <- tmatrix_solution    (mstm-solver-8.f90:22-25)
+!   hip_direct_solver       <- direct_solver       (mstm-solver-8.f90:571-572)
 ! The GPU context is created on first use and refilled when
 ! initialize_solver / initial_run is true (geometry, orders and an_mie are
 ! taken from the MSTM modules, so the Mie data are the host's own).
@@ -117,10 +118,11 @@
          deallocate(p)
          end subroutine hip_load_network
 !
-! fixedorsoln (mstm-solver-8.f90:350-569), iterative branch; solution_method
-! 'm' selects the mixed FP32/FP64 solver, 'l' the mixed solver from the
-! learned start (hip_load_network first), 'g' flexible GMRES, anything else
-! the FP64 one
+! fixedorsoln (mstm-solver-8.f90:350-569); solution_method 'i' (or absent)
+! is the iterative FP64 solver with cbicg's iterates, 'm' the mixed FP32/FP64
+! solver, 'l' the mixed solver from the learned start (hip_load_network
+! first), 'g' flexible GMRES; any other letter is the direct (LU) solution,
+! as in the reference (dirsoln = solution_method .ne. 'i', :378-380)
 !
          subroutine hip_fixedorsoln(alpha,sinc,dir,eps,niter,amnp,qeff,qeffdim,maxerr,maxiter, &
             iterwrite,istat,mpi_comm,excited_spheres,solution_method,initialize_solver)
@@ -138,9 +140,15 @@
          if(init.or.(.not.hip_loaded)) call hip_load_state()
          solver=mstm_hip_solve_plain
          if(present(solution_method)) then
-            if(solution_method.eq.'m') solver=mstm_hip_solve_mixed
-            if(solution_method.eq.'l') solver=mstm_hip_solve_learned
-            if(solution_method.eq.'g') solver=mstm_hip_solve_fgmres
+            if(solution_method.eq.'m') then
+               solver=mstm_hip_solve_mixed
+            elseif(solution_method.eq.'l') then
+               solver=mstm_hip_solve_learned
+            elseif(solution_method.eq.'g') then
+               solver=mstm_hip_solve_fgmres
+            elseif(solution_method.ne.'i') then
+               solver=mstm_hip_solve_lu
+            endif
          endif
          ufft=0
          if(fft_translation_option) ufft=1
@@ -321,5 +329,29 @@
          if(present(solution_status)) solution_status=ist
          deallocate(tr,tm,sq,qlog)
          end subroutine hip_tmatrix_solution
+
+!
+! direct_solver (mstm-solver-8.f90:571-642): LU factors of the dense
+! A = I - T H on the GPU (made when initialize_solver is true or none are
+! held), back substitution, number_iterations refinement steps (default 3)
+!
+         subroutine hip_direct_solver(pnp,anp,initialize_solver,solution_error, &
+            number_iterations,solution_eps,mpi_comm)
+         logical, optional :: initialize_solver
+         integer, optional :: number_iterations,mpi_comm
+         real(8), optional :: solution_error,solution_eps
+         complex(8) :: pnp(number_eqns),anp(number_eqns)
+         integer(c_int) :: init,niter
+         real(c_double) :: serr
+         if(.not.hip_loaded) call hip_load_state()
+         init=1
+         if(present(initialize_solver)) then
+            if(.not.initialize_solver) init=0
+         endif
+         niter=3
+         if(present(number_iterations)) niter=number_iterations
+         call hip_check(mstm_hip_direct_solver(hip_ctx,pnp,anp,init,niter,serr),'direct_solver')
+         if(present(solution_error)) solution_error=serr
+         end subroutine hip_direct_solver
 
       end module mstm_hip_solver

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libmstm_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "mstm_hip.h")
 
 FP64, FP32 = 0, 1
-SOLVE_PLAIN, SOLVE_MIXED, SOLVE_LEARNED, SOLVE_FGMRES = 0, 1, 2, 3
+SOLVE_PLAIN, SOLVE_MIXED, SOLVE_LEARNED, SOLVE_FGMRES, SOLVE_LU = 0, 1, 2, 3, 4
 
 
 class MstmHipError(RuntimeError):
@@ -52,6 +52,7 @@ _SIGNATURES = {
     "mstm_hip_asymmetry": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
     "mstm_hip_set_inner_solver": (C.c_int, [_vp, C.c_int]),
     "mstm_hip_set_refinement_cycles": (C.c_int, [_vp, C.c_int]),
+    "mstm_hip_direct_solver": (C.c_int, [_vp, _dp, _dp, C.c_int, C.c_int, _dp]),
     "mstm_hip_tmatrix_solution": (C.c_int, [_vp, C.c_double, C.c_int, C.c_double, C.c_int, _dp, C.c_int64, _dp, _dp,
                                             _dp, _ip, _ip, _dp, _ip]),
     "mstm_hip_stats": (C.c_int, [_vp, _dp, C.c_int]),

@@ -11,7 +11,7 @@ import ctypes as C
 
 import numpy as np
 
-from .lib import FP32, FP64, SOLVE_FGMRES, SOLVE_LEARNED, SOLVE_MIXED, SOLVE_PLAIN, MstmHipError, load_library  # noqa: F401
+from .lib import FP32, FP64, SOLVE_FGMRES, SOLVE_LEARNED, SOLVE_LU, SOLVE_MIXED, SOLVE_PLAIN, MstmHipError, load_library  # noqa: F401
 
 
 def _d(a):
@@ -217,6 +217,17 @@ class HipContext:
         sm = self.scattering_matrix(ct, phi)
         s00, g = self.asymmetry()
         return dict(T=T, torder=tord, amnp0=a0, theta=th, scatmat=sm, s00=s00, g=g)
+
+    def direct_solver(self, pnp, initialize: bool = True, niter: int = 3):
+        """direct_solver (mstm-solver-8.f90:571-642): (anp, solution_error)
+        with anp = A^{-1} pnp by the LU factors of the dense interaction
+        matrix A = I - T H, then niter refinement steps."""
+        p = np.ascontiguousarray(pnp, np.complex128)
+        out = np.zeros_like(p)
+        e = np.zeros(1)
+        self._chk(self.L.mstm_hip_direct_solver(self.h, _d(p.view(np.float64)), _d(out.view(np.float64)),
+                                                int(bool(initialize)), int(niter), _d(e)), "direct_solver")
+        return out, float(e[0])
 
     def tmatrix_solution(self, eps: float = 1e-6, niter: int = 10000, conveps: float = 1e-6, solver: int = 0,
                          teps: float = 1e-5, maxt: int = 100):

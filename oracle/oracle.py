@@ -167,7 +167,9 @@ class OracleCluster:
 
     def qeff(self, amnp, pmnp):
         q = np.zeros((self.n, 3, 3))
-        lib().orc_qeff(self.h, np.ascontiguousarray(amnp).ctypes.data, np.ascontiguousarray(pmnp).ctypes.data, _d(q))
+        a = np.ascontiguousarray(amnp, np.complex128)  # held until the call returns
+        p = np.ascontiguousarray(pmnp, np.complex128)
+        lib().orc_qeff(self.h, a.ctypes.data, p.ctypes.data, _d(q))
         return q
 
     def qtot(self, qeff):
@@ -199,6 +201,24 @@ class OracleCluster:
         r0 = np.ascontiguousarray(origin, np.float64)
         lib().orc_merge(self.h, T, _i(t), _d(r0), nrhs, a.ctypes.data, out.ctypes.data)
         return out
+
+    def interaction_matrix(self):
+        """general_interaction_matrix with mie_mult (mstm-modules-33.f90:5854-5985):
+        A = I - T H, column q = e_q - sphereinteraction(e_q) (dense, neqns^2)."""
+        ne = self.neqns
+        A = np.eye(ne, dtype=np.complex128)
+        for q0 in range(0, ne, 256):
+            q1 = min(ne, q0 + 256)
+            e = np.zeros((q1 - q0, ne), np.complex128)
+            e[np.arange(q1 - q0), np.arange(q0, q1)] = 1.0
+            A[:, q0:q1] -= self.sphereinteraction(e, [0] * (q1 - q0), [1] * (q1 - q0)).T
+        return A
+
+    def direct_solve(self, b, A=None):
+        """direct_solver (mstm-solver-8.f90:571-642) restated: the exact solution
+        of A x = b for the columns of b (LAPACK LU of the dense A)."""
+        A = self.interaction_matrix() if A is None else A
+        return np.ascontiguousarray(np.linalg.solve(A, np.asarray(b, np.complex128).T).T)
 
     def distribute(self, l, origin, a0):
         """distribute_from_common_origin of one (2, l(l+2)) helicity column -> (neqns,)."""

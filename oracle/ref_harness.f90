@@ -10,7 +10,7 @@
 !   mode='matvec'   sphereinteraction on two RHS read from infile
 !                   (con_tran = F,T; mie_mult = T,T), plus the bare
 !                   external_to_external_expansion of the same RHS
-!   mode='solve'    fixedorsoln (BiCG) + qtotcalc
+!   mode='solve'    fixedorsoln (BiCG; method='d': the LU branch) + qtotcalc
 !   mode='special'  rotcoef / axialtrancoefrecurrence / gentranmatrix samples
 ! With observe=.true., 'solve' continues as a single-origin run of
 ! main_calling_program does (mstm-input-37.f90:1159-1197, 1503-1535):
@@ -45,13 +45,14 @@
       logical :: use_fft,store_tables,observe
       real(8) :: teps,conveps
       integer :: tcap
+      character(len=1) :: method
       real(8) :: r0t(3)
       real(8), allocatable :: sqeff(:,:)
       complex(8), allocatable :: ain(:,:),aout(:,:),gout(:,:),amnp(:,:),pmnp0(:,:),pmnpan(:)
       real(8), allocatable :: qeff(:,:,:),qvabs(:,:)
       real(8) :: qtot(3,3)
       namelist /job/ mode,posfile,nsphere,mie_eps,use_fft,cell_vf,node_order_in,nbr_model, &
-         eps,niter,alpha,sinc,infile,outprefix,reps,store_tables,rmed,observe,teps,conveps,tcap
+         eps,niter,alpha,sinc,infile,outprefix,reps,store_tables,rmed,observe,teps,conveps,tcap,method
 
       mode='setup'
       posfile='cluster.pos'
@@ -74,6 +75,7 @@
       teps=1.d-5
       conveps=1.d-6
       tcap=0
+      method='i'
       call get_command_argument(1,jobfile)
       open(10,file=trim(jobfile),status='old')
       read(10,nml=job)
@@ -182,7 +184,7 @@
             call qefficiencyfactors(nsphere,2,amnp,pmnp0,qeff)
          else
             call fixedorsoln(alpha,sinc,1,eps,niter,amnp,qeff,3,maxerr,maxiter,0,istat, &
-               solution_method='i',initialize_solver=.true.)
+               solution_method=method,initialize_solver=.true.)
          endif
          t1=mstm_mpi_wtime()
          call qtotcalc(nsphere,3,xg,qeff,qvabs,qtot)

@@ -31,6 +31,9 @@ Cases
   tmat_poly6    6 spheres of mixed size (x = 0.3 .. 1.6 evenly, orders 3..5) on seeded
                 RSA centres (placed at x = 1.6, fv 0.10), m = 1.4+0.05i:
                 tmatrix_solution at solution eps 1e-8, convergence eps 1e-7
+  lu_cfg1, lu_poly30  the clusters of cfg1_direct and poly30 through the
+                direct (LU) branch of fixedorsoln (solution_method 'd'):
+                solution and efficiencies
 Every solve also runs the single-origin post-processing (translation orders,
 merge_to_common_origin, scatteringmatrix on the default grid,
 fosmexpansion); those outputs are stored under obs_* keys.
@@ -101,6 +104,16 @@ def observe_case(name, pos, radius, ri):
     print(name, "neqns", neqns, "T", res["obs_T"])
 
 
+def lu_case(name, pos, radius, ri):
+    ri = np.asarray(ri, np.complex128)
+    r = O.run_reference(pos, radius, ri, "solve", method="'d'")
+    ne = int(r["neqns"])
+    res = dict(pos=pos, radius=radius, ri=ri, order=r["order"], amnp=r["amnp"].reshape(2, ne),
+               qeff=r["qeff"].reshape(-1, 3, 3), qtot=r["qtot"].reshape(3, 3))
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **res)
+    print(name, "neqns", ne)
+
+
 def tmatrix_case(name, pos, radius, ri, eps, conveps, niter=10000):
     ri = np.asarray(ri, np.complex128)
     r = O.run_reference(pos, radius, ri, "tmatrix", eps=eps, niter=niter, conveps=conveps)
@@ -133,6 +146,15 @@ def main():
     c = G.rsa_in_sphere(6, 1.6, 0.10, seed=24, ri=1.4 + 0.05j)
     radius = np.linspace(0.3, 1.6, c.n)
     tmatrix_case("tmat_poly6", c.pos, radius, c.ri, eps=1e-8, conveps=1e-7)
+    lu_only()
+
+
+def lu_only():
+    c = G.rsa_in_sphere(20, 2.0, 0.10, seed=1, ri=1.5 + 0.01j)
+    lu_case("lu_cfg1", c.pos, c.radius, c.ri)
+    c = G.rsa_in_sphere(30, 2.4, 0.10, seed=8, ri=1.5 + 0.02j)
+    radius = np.round(np.random.default_rng(8).uniform(0.4, 2.4, c.n), 6)
+    lu_case("lu_poly30", c.pos, radius, c.ri)
 
 
 def tmatrix_only():
@@ -144,4 +166,4 @@ def tmatrix_only():
 
 
 if __name__ == "__main__":
-    tmatrix_only() if sys.argv[1:] == ["tmatrix"] else main()
+    {"tmatrix": tmatrix_only, "lu": lu_only}.get(sys.argv[1] if len(sys.argv) > 1 else "", main)()
