@@ -1677,3 +1677,287 @@ int orc_tmatrix(const orc_cluster *c, int T, const int *torder, const double *r0
     free(dq);
     return l <= T ? l : T;
 }
+
+/* the expansion-coefficient stage of ranorientscatmatrix (:2336-2472):
+ * dm/dmcf [u+T+1][ku][n][p][n1][p1] (u = -T-1..T+1) -> sm, smcf.          */
+void orc_ranorient_coef(int T, const zc *dm, const zc *dmcf, double rimed, double *sm, double *smcf)
+{
+    const int W = 2 * T;
+#define DMI(A, u, ku, n, p, n1, p1) \
+    A[(((((size_t)((u) + T + 1) * 3 + (ku) - 1) * T + (n) - 1) * 2 + (p) - 1) * T + (n1) - 1) * 2 + (p1) - 1]
+    size_t nfm = (size_t)3 * T * 2 * T * 2;
+    zc *fm = malloc(sizeof(zc) * nfm), *fmcf = malloc(sizeof(zc) * nfm);
+#define FM(A, k, n, p, n1, p1) A[((((size_t)(k) - 1) * T + (n) - 1) * 2 + (p) - 1) * T * 2 + ((n1) - 1) * 2 + (p1) - 1]
+    /* aw(0:2,-1:1,w), bw likewise, cw(w), dw(w) */
+    zc *aw = calloc((size_t)9 * (W + 1), sizeof(zc)), *bw = calloc((size_t)9 * (W + 1), sizeof(zc));
+    zc *awcf = calloc((size_t)9 * (W + 1), sizeof(zc)), *bwcf = calloc((size_t)9 * (W + 1), sizeof(zc));
+    zc *cw = calloc(W + 1, sizeof(zc)), *dw = calloc(W + 1, sizeof(zc));
+    zc *cwcf = calloc(W + 1, sizeof(zc)), *dwcf = calloc(W + 1, sizeof(zc));
+#define AW(A, i, k, w) A[((w) * 3 + (k) + 1) * 3 + (i)]
+    for (int w = 0; w <= W; w++) {
+        memset(fm, 0, sizeof(zc) * nfm);
+        memset(fmcf, 0, sizeof(zc) * nfm);
+        const double s2w = sqrt((double)(w + w + 1));
+        for (int u = -T - 1; u <= T + 1; u++) {
+            for (int k = -1; k <= 1; k += 2) {
+                int m = u + k, ik = (k + 1) / 2 + 1, ns = imax(1, iabs_(m));
+                for (int n = ns; n <= T; n++)
+                    for (int n1 = ns; n1 <= T; n1++) {
+                        if (n + n1 < w || iabs_(n - n1) > w) continue;
+                        double fl = -((n & 1) ? -1.0 : 1.0) * orc_cg(n, m, w, 0, n1) * s2w / sqrt((double)(n1 + n1 + 1));
+                        for (int p = 1; p <= 2; p++)
+                            for (int p1 = 1; p1 <= 2; p1++) {
+                                FM(fm, ik, n, p, n1, p1) += DMI(dm, u, ik, n, p, n1, p1) * fl;
+                                FM(fmcf, ik, n, p, n1, p1) += DMI(dmcf, u, ik, n, p, n1, p1) * fl;
+                            }
+                    }
+            }
+            if (w < 2) continue;
+            int m = u + 1, m1 = u - 1, ns = imax(1, iabs_(m)), n1s = imax(1, iabs_(m1));
+            for (int n = ns; n <= T; n++)
+                for (int n1 = n1s; n1 <= T; n1++) {
+                    if (n + n1 < w || iabs_(n - n1) > w) continue;
+                    double fl = -((n & 1) ? -1.0 : 1.0) * orc_cg(n, m, w, -2, n1) * s2w / sqrt((double)(n1 + n1 + 1));
+                    for (int p = 1; p <= 2; p++)
+                        for (int p1 = 1; p1 <= 2; p1++) {
+                            FM(fm, 3, n, p, n1, p1) += DMI(dm, u, 3, n, p, n1, p1) * fl;
+                            FM(fmcf, 3, n, p, n1, p1) += DMI(dmcf, u, 3, n, p, n1, p1) * fl;
+                        }
+                }
+        }
+        for (int n = 1; n <= T; n++) {
+            int n1s = imax(1, iabs_(n - w)), n1e = imin(T, n + w);
+            double in = (n & 1) ? -1.0 : 1.0;
+            for (int n1 = n1s; n1 <= n1e; n1++) {
+                double fl = 2.0 * in * orc_cg(n, 1, w, 0, n1) * s2w / sqrt((double)(n1 + n1 + 1));
+                int i = (n + n1 - w) % 2 + 1;
+                for (int p = 1; p <= 2; p++) {
+                    int p1 = (2 - i) * p + (i - 1) * (3 - p);
+                    for (int k = -1; k <= 1; k += 2) {
+                        int ik = (k + 1) / 2 + 1;
+                        AW(aw, 0, k, w) += FM(fm, ik, n, p, n1, p1) * fl;
+                        AW(bw, 0, k, w) += FM(fm, ik, n, p, n1, 3 - p1) * fl;
+                        AW(awcf, 0, k, w) += FM(fmcf, ik, n, p, n1, p1) * fl;
+                        AW(bwcf, 0, k, w) += FM(fmcf, ik, n, p, n1, 3 - p1) * fl;
+                    }
+                    AW(bw, 2, 0, w) += FM(fm, 3, n, p, n1, 3 - p1) * fl;
+                    AW(aw, 2, 0, w) += FM(fm, 3, n, p, n1, p1) * fl;
+                    AW(bwcf, 2, 0, w) += FM(fmcf, 3, n, p, n1, 3 - p1) * fl;
+                    AW(awcf, 2, 0, w) += FM(fmcf, 3, n, p, n1, p1) * fl;
+                }
+            }
+            if (w < 2) continue;
+            for (int n1 = n1s; n1 <= n1e; n1++) {
+                double vc = orc_cg(n, 1, w, -2, n1);
+                double fl = 2.0 * in * vc * s2w / sqrt((double)(n1 + n1 + 1));
+                int i = (n + n1 - w) % 2 + 1;
+                for (int p = 1; p <= 2; p++) {
+                    int p1 = (2 - i) * p + (i - 1) * (3 - p);
+                    double sp1 = (p1 & 1) ? -1.0 : 1.0, sq1 = ((3 - p1) & 1) ? -1.0 : 1.0;
+                    for (int k = -1; k <= 1; k += 2) {
+                        int ik = (k + 1) / 2 + 1;
+                        AW(aw, 2, k, w) += FM(fm, ik, n, p, n1, p1) * fl * sp1;
+                        AW(bw, 2, k, w) += FM(fm, ik, n, p, n1, 3 - p1) * fl * sq1;
+                        AW(awcf, 2, k, w) += FM(fmcf, ik, n, p, n1, p1) * fl * sp1;
+                        AW(bwcf, 2, k, w) += FM(fmcf, ik, n, p, n1, 3 - p1) * fl * sq1;
+                    }
+                }
+                double fl2 = 2.0 * (((n1 + w) & 1) ? -1.0 : 1.0) * vc * s2w / sqrt((double)(n1 + n1 + 1));
+                for (int p = 1; p <= 2; p++)
+                    for (int p1 = 1; p1 <= 2; p1++) {
+                        double sp1 = (p1 & 1) ? -1.0 : 1.0, sp = (p & 1) ? -1.0 : 1.0;
+                        cw[w] += FM(fm, 3, n, p, n1, p1) * fl * sp1;
+                        dw[w] += FM(fm, 3, n, p, n1, p1) * fl2 * sp;
+                        cwcf[w] += FM(fmcf, 3, n, p, n1, p1) * fl * sp1;
+                        dwcf[w] += FM(fmcf, 3, n, p, n1, p1) * fl2 * sp;
+                    }
+            }
+        }
+    }
+    for (int pass = 0; pass < 2; pass++) {
+        const zc *a = pass ? awcf : aw, *b = pass ? bwcf : bw, *c = pass ? cwcf : cw, *d = pass ? dwcf : dw;
+        double *S = pass ? smcf : sm;
+        for (int n = 0; n <= W; n++) {
+#define S_(i, j) S[((i) - 1) + 4 * ((j) - 1) + 16 * n]
+            S_(1, 1) = creal(AW(a, 0, -1, n) + AW(a, 0, 1, n));
+            S_(1, 2) = creal(AW(a, 2, -1, n) + AW(a, 2, 1, n));
+            S_(1, 3) = 2.0 * cimag(AW(a, 2, 0, n));
+            S_(1, 4) = creal(AW(a, 0, 1, n) - AW(a, 0, -1, n));
+            S_(2, 2) = creal(d[n]);
+            S_(2, 3) = cimag(d[n]);
+            S_(2, 4) = creal(AW(a, 2, 1, n) - AW(a, 2, -1, n));
+            S_(3, 1) = -cimag(AW(b, 2, -1, n) + AW(b, 2, 1, n));
+            S_(3, 2) = cimag(c[n]);
+            S_(3, 3) = creal(c[n]);
+            S_(3, 4) = cimag(AW(b, 2, -1, n) - AW(b, 2, 1, n));
+            S_(4, 1) = creal(AW(b, 0, -1, n) + AW(b, 0, 1, n));
+            S_(4, 2) = 2.0 * creal(AW(b, 2, 0, n));
+            S_(4, 4) = creal(AW(b, 0, 1, n) - AW(b, 0, -1, n));
+            for (int e = 0; e < 16; e++) S[16 * n + e] = S[16 * n + e] / rimed / 2.0;
+#undef S_
+        }
+    }
+    free(fm);
+    free(fmcf);
+    free(aw);
+    free(bw);
+    free(awcf);
+    free(bwcf);
+    free(cw);
+    free(dw);
+    free(cwcf);
+    free(dwcf);
+#undef DMI
+#undef FM
+#undef AW
+}
+
+/* ranorientscatmatrix (mstm-scatprops-26.f90:1963-2530): the expansion
+ * coefficients sm(4,4,0:2T) / smcf(4,4,0:2T) (Fortran order, column-major
+ * per w) of the orientation-averaged scattering matrix and of its coherent
+ * part, from the T-matrix file body of tmatrix_solution (order T, TE/TM).
+ * As the reference: the file is read into complex(4) and its rows n > l
+ * filled by reciprocity (:2066-2076); the Gaussian-beam weights (beam width
+ * cbeam, 0 = plane wave) go into the same complex(4) array (:2143-2169);
+ * the rest is FP64.  Vector-coupling values C(j1 m1; j2 m2 | J m1+m2) come
+ * from orc_cg (vcfunc, mstm-modules-33.f90:950-991, gives the same values by
+ * recurrence).                                                            */
+typedef float complex fzc;
+static inline zc ipow(int e)
+{
+    int r = ((e % 4) + 4) % 4;
+    return r == 0 ? 1.0 : (r == 1 ? I : (r == 2 ? -1.0 : -I));
+}
+void orc_ranorient(int T, const zc *tmat, double rimed, double cbeam, double *sm, double *smcf)
+{
+    const int nblk = T * (T + 2), W = 2 * T;
+    /* tc(p, mn, q, kl), 1-based mn/kl -> [((kl-1)*2 + q-1)*nblk + mn-1][p-1] */
+    fzc *tc = calloc((size_t)4 * nblk * nblk, sizeof(fzc));
+#define TC(p, mn, q, kl) tc[((((size_t)(kl) - 1) * 2 + (q) - 1) * nblk + (mn) - 1) * 2 + (p) - 1]
+    size_t it = 0;
+    for (int l = 1; l <= T; l++)
+        for (int k = -l; k <= l; k++) {
+            int kl = l * (l + 1) + k, klm = l * (l + 1) - k;
+            for (int q = 1; q <= 2; q++)
+                for (int n = 1; n <= l; n++)
+                    for (int m = -n; m <= n; m++) {
+                        int mn = n * (n + 1) + m, mnm = n * (n + 1) - m;
+                        for (int p = 1; p <= 2; p++) {
+                            zc v = tmat[it++];
+                            TC(p, mn, q, kl) = (float)creal(v) + (float)cimag(v) * I;
+                        }
+                        if (n < l) {
+                            float ikm = ((m + k) & 1) ? -1.0f : 1.0f;
+                            for (int p = 1; p <= 2; p++) TC(q, klm, p, mnm) = TC(p, mn, q, kl) * ikm;
+                        }
+                    }
+        }
+    /* Gaussian-beam modified T matrix */
+    zc *pp = calloc((size_t)(T + 1) * 4, sizeof(zc));
+#define PP(n, a, b) pp[(n) * 4 + ((a) - 1) * 2 + (b) - 1]
+    for (int n = 1; n <= T; n++) {
+        double gbn = exp(-pow(((double)n + .5) * cbeam, 2.0));
+        zc cin = ipow(n + 1);
+        PP(n, 1, 1) = -.5 * cin * sqrt((double)(n + n + 1)) * gbn;
+        PP(n, 2, 1) = -PP(n, 1, 1);
+        PP(n, 1, 2) = -PP(n, 1, 1);
+        PP(n, 2, 2) = PP(n, 2, 1);
+    }
+    for (int n = 1; n <= T; n++)
+        for (int m = -n; m <= n; m++) {
+            int mn = n * (n + 1) + m;
+            for (int p = 1; p <= 2; p++)
+                for (int l = 1; l <= T; l++)
+                    for (int k = -l; k <= l; k++) {
+                        int kl = l * (l + 1) + k;
+                        zc a = TC(p, mn, 1, kl);
+                        TC(p, mn, 1, kl) = (fzc)(TC(p, mn, 1, kl) * PP(l, 1, 1) + TC(p, mn, 2, kl) * PP(l, 1, 2));
+                        TC(p, mn, 2, kl) = (fzc)(a * PP(l, 2, 1) + TC(p, mn, 2, kl) * PP(l, 2, 2));
+                    }
+        }
+    /* dm(u, ku, n, p, n1, p1), u = -T-1..T+1 */
+    const int NU = 2 * T + 3;
+    size_t ndm = (size_t)NU * 3 * T * 2 * T * 2;
+    zc *dm = calloc(ndm, sizeof(zc)), *dmcf = calloc(ndm, sizeof(zc));
+#define DM(A, u, ku, n, p, n1, p1) \
+    A[(((((size_t)((u) + T + 1) * 3 + (ku) - 1) * T + (n) - 1) * 2 + (p) - 1) * T + (n1) - 1) * 2 + (p1) - 1]
+    zc *bm = malloc(sizeof(zc) * 2 * nblk * 2), *bmcf = malloc(sizeof(zc) * 2 * nblk * 2);
+#define BM(A, k, mn, p) A[(((k) - 1) * nblk + (mn) - 1) * 2 + (p) - 1]
+    zc *am = malloc(sizeof(zc) * 2 * (T + 2) * 2), *amcf = malloc(sizeof(zc) * 2 * (T + 2) * 2);
+#define AM(A, k, l, p) A[(((k) - 1) * (T + 2) + (l)) * 2 + (p) - 1]
+    for (int w = 0; w <= W; w++)
+        for (int v = -w; v <= w; v++) {
+            memset(bm, 0, sizeof(zc) * 4 * nblk);
+            memset(bmcf, 0, sizeof(zc) * 4 * nblk);
+            for (int n = 1; n <= T; n++) {
+                int nn1 = n * (n + 1);
+                memset(am, 0, sizeof(zc) * 4 * (T + 2));
+                memset(amcf, 0, sizeof(zc) * 4 * (T + 2));
+                for (int t = -n; t <= n; t++) {
+                    int tn = nn1 + t, lmax = imin(T, w + n);
+                    for (int l = imax(1, imax(iabs_(v - t), iabs_(n - w))); l <= lmax; l++) {
+                        double vc = orc_cg(w, v, n, -t, l);
+                        int tvl = l * (l + 1) + t - v;
+                        for (int k = 1; k <= 2; k++)
+                            for (int p = 1; p <= 2; p++) {
+                                AM(am, k, l, p) += vc * (zc)TC(p, tn, k, tvl);
+                                if (l == n && v == 0) {
+                                    zc tct = 0.0;
+                                    for (int kt = -n; kt <= n; kt++) tct += (zc)TC(p, nn1 + kt, k, nn1 + kt);
+                                    AM(amcf, k, l, p) += vc * tct / (double)(n + n + 1);
+                                }
+                            }
+                    }
+                }
+                for (int m = -n; m <= n; m++) {
+                    int mn = nn1 + m;
+                    for (int k = 1; k <= 2; k++) {
+                        int u = m - (-3 + 2 * k);
+                        if (iabs_(u) > w) continue;
+                        int lmax = imin(T, w + n);
+                        for (int l = imax(1, iabs_(w - n)); l <= lmax; l++) {
+                            double fl = -((l & 1) ? -1.0 : 1.0) * orc_cg(w, -u, n, m, l) / (double)(l + l + 1);
+                            for (int p = 1; p <= 2; p++) {
+                                BM(bm, k, mn, p) += AM(am, k, l, p) * fl;
+                                if (v == 0) BM(bmcf, k, mn, p) += AM(amcf, k, l, p) * fl;
+                            }
+                        }
+                    }
+                }
+            }
+            for (int u = -imin(w, T + 1); u <= imin(w, T + 1); u++)
+                for (int ku = 1; ku <= 3; ku++) {
+                    int k = ku == 1 ? -1 : 1, k1 = ku == 2 ? 1 : -1;
+                    int m = u + k, ns = imax(1, iabs_(m)), ik = (k + 1) / 2 + 1, ik1 = (k1 + 1) / 2 + 1, m1 = u + k1;
+                    for (int n = ns; n <= T; n++) {
+                        int nu = n * (n + 1) + m;
+                        int n1s = imax(1, imax(iabs_(m1), n - W)), n1e = imin(T, n + W);
+                        for (int n1 = n1s; n1 <= n1e; n1++) {
+                            zc cin = ipow(n - n1);
+                            int nu1 = n1 * (n1 + 1) + m1;
+                            double fl = -sqrt((double)(n + n + 1)) * sqrt((double)(n1 + n1 + 1)) * (double)(w + w + 1);
+                            for (int p = 1; p <= 2; p++)
+                                for (int p1 = 1; p1 <= 2; p1++) {
+                                    DM(dm, u, ku, n, p, n1, p1) += BM(bm, ik, nu, p) * cin * fl * conj(BM(bm, ik1, nu1, p1));
+                                    if (v == 0)
+                                        DM(dmcf, u, ku, n, p, n1, p1) +=
+                                            BM(bmcf, ik, nu, p) * cin * fl * conj(BM(bmcf, ik1, nu1, p1));
+                                }
+                        }
+                    }
+                }
+        }
+    orc_ranorient_coef(T, dm, dmcf, rimed, sm, smcf);
+    free(tc);
+    free(pp);
+    free(dm);
+    free(dmcf);
+    free(bm);
+    free(bmcf);
+    free(am);
+    free(amcf);
+#undef TC
+#undef PP
+#undef BM
+#undef AM
+}

@@ -85,6 +85,8 @@ void orc_fosm(int T, const zc *amnp, double rimed, double *s00, double *s02, dou
 
 /* T-matrix solution (random-orientation path) */
 void orc_distribute(const orc_cluster *c, int l, const double *r0, const zc *a0, zc *amnp);
+void orc_ranorient(int T, const zc *tmat, double rimed, double cbeam, double *sm, double *smcf);
+void orc_ranorient_coef(int T, const zc *dm, const zc *dmcf, double rimed, double *sm, double *smcf);
 int orc_tmatrix(const orc_cluster *c, int T, const int *torder, const double *r0, double vol_radius, double eps,
                 int niter, double conveps, zc *tmat, double *sqeff, double *qteff, double *qlog, int *maxiter,
                 double *maxerr, int *istat);

@@ -68,6 +68,7 @@ def lib():
         L.orc_scatmat.argtypes = [C.c_int, vp, C.c_double, C.c_double, C.c_double, dp, vp]
         L.orc_fosm.argtypes = [C.c_int, vp, C.c_double, dp, dp, dp, dp]
         L.orc_distribute.argtypes = [vp, C.c_int, dp, vp, vp]
+        L.orc_ranorient.argtypes = [C.c_int, vp, C.c_double, C.c_double, dp, dp]
         L.orc_tmatrix.restype = C.c_int
         L.orc_tmatrix.argtypes = [vp, C.c_int, ip, dp, C.c_double, C.c_double, C.c_int, C.c_double, vp, dp, dp, dp,
                                   ip, dp, ip]
@@ -265,6 +266,16 @@ class OracleCluster:
         return p
 
 
+def ranorient(T, tmat, rimed=1.0, cbeam=0.0):
+    """ranorientscatmatrix (mstm-scatprops-26.f90:1963-2530) on a T-matrix file
+    body: (sm, smcf), each (2T+1, 4, 4) with [w][j-1][i-1] = Fortran sm(i,j,w)."""
+    t = np.ascontiguousarray(tmat, np.complex128)
+    sm = np.zeros((2 * T + 1, 4, 4))
+    smcf = np.zeros((2 * T + 1, 4, 4))
+    lib().orc_ranorient(int(T), t.ctypes.data, float(rimed), float(cbeam), _d(sm), _d(smcf))
+    return sm, smcf
+
+
 # the reference's pi: a real(4) literal widened to real(8) (mstm-modules-33.f90:18)
 PI_R4 = float(np.float32(np.pi))
 
@@ -428,7 +439,7 @@ def run_reference(pos, radius, ri, mode, ain=None, outdir=None, timeout=None, **
             raw = open(os.path.join(outdir, name), "rb").read()
             if key in ("order", "offset", "torder"):
                 res[key] = np.frombuffer(raw, np.int32).copy()
-            elif key in ("qeff", "qtot", "rotcoef", "scatmat", "s00", "s02", "sp22", "sm22", "sqeff"):
+            elif key in ("qeff", "qtot", "rotcoef", "scatmat", "s00", "s02", "sp22", "sm22", "sqeff", "rsm", "rsmcf"):
                 res[key] = np.frombuffer(raw, np.float64).copy()
             else:
                 res[key] = np.frombuffer(raw, np.complex128).copy()

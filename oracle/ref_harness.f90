@@ -20,7 +20,9 @@
 !   mode='tmatrix'  the same translation orders, then tmatrix_solution
 !                   (mstm-solver-8.f90:22-332, iterative, serial): the
 !                   T-matrix file it writes, its per-order log and the
-!                   per-sphere efficiency sums (tcap > 0 caps the order)
+!                   per-sphere efficiency sums (tcap > 0 caps the order);
+!                   ranorient=.true. then runs ranorientscatmatrix on that
+!                   file (scattering-matrix expansion sm, smcf)
 ! All outputs are raw little-endian stream files <outprefix>_<name>.bin plus
 ! a small text summary <outprefix>_info.txt.
 !
@@ -46,13 +48,17 @@
       real(8) :: teps,conveps
       integer :: tcap
       character(len=1) :: method
+      logical :: ranorient
+      character(len=30) :: tfile30
+      character(len=128) :: tfile128
+      real(8), allocatable :: rsm(:,:,:),rsmcf(:,:,:)
       real(8) :: r0t(3)
       real(8), allocatable :: sqeff(:,:)
       complex(8), allocatable :: ain(:,:),aout(:,:),gout(:,:),amnp(:,:),pmnp0(:,:),pmnpan(:)
       real(8), allocatable :: qeff(:,:,:),qvabs(:,:)
       real(8) :: qtot(3,3)
       namelist /job/ mode,posfile,nsphere,mie_eps,use_fft,cell_vf,node_order_in,nbr_model, &
-         eps,niter,alpha,sinc,infile,outprefix,reps,store_tables,rmed,observe,teps,conveps,tcap,method
+         eps,niter,alpha,sinc,infile,outprefix,reps,store_tables,rmed,observe,teps,conveps,tcap,method,ranorient
 
       mode='setup'
       posfile='cluster.pos'
@@ -76,6 +82,7 @@
       conveps=1.d-6
       tcap=0
       method='i'
+      ranorient=.false.
       call get_command_argument(1,jobfile)
       open(10,file=trim(jobfile),status='old')
       read(10,nml=job)
@@ -219,8 +226,11 @@
 ! closed around the call and reopened for appending
          close(20)
          t0=mstm_mpi_wtime()
+! t_matrix_file is character*128 in tmatrix_solution: pass a blank-padded
+! variable of that length
+         tfile128=trim(outprefix)//'_tmat.dat'
          call tmatrix_solution(solution_method='i',solution_eps=eps,convergence_eps=conveps, &
-            max_iterations=niter,t_matrix_file=trim(outprefix)//'_tmat.dat', &
+            max_iterations=niter,t_matrix_file=tfile128, &
             sphere_qeff=sqeff,solution_status=istat)
          t1=mstm_mpi_wtime()
          close(21)
@@ -229,6 +239,21 @@
          write(20,'(a,i10)') 'istat ',istat
          write(20,'(a,es25.17)') 'solve_seconds ',t1-t0
          call dump_d('sqeff',sqeff,3*nsphere)
+         if(ranorient) then
+! ranorientscatmatrix (mstm-scatprops-26.f90:1963-2530) on the file just
+! written (run in the output directory: the file name is character*30)
+            allocate(rsm(4,4,0:2*t_matrix_order),rsmcf(4,4,0:2*t_matrix_order))
+            rsm=0.d0
+            rsmcf=0.d0
+            cross_section_radius=vol_radius
+            tfile30='out_tmat.dat'
+            t0=mstm_mpi_wtime()
+            call ranorientscatmatrix(tfile30,rsm,rsmcf,keep_quiet=.true.)
+            t1=mstm_mpi_wtime()
+            write(20,'(a,es25.17)') 'ranorient_seconds ',t1-t0
+            call dump_d('rsm',rsm,size(rsm))
+            call dump_d('rsmcf',rsmcf,size(rsmcf))
+         endif
       elseif(trim(mode).eq.'special') then
          call special_dumps()
       endif

@@ -27,7 +27,8 @@ Cases
                 operator (poly30_fft)
   tmat5         5 spheres x=1, m=1.5+0.01i, seeded RSA in a sphere (fv 0.10):
                 tmatrix_solution with the reference defaults (solution and
-                convergence eps 1e-6)
+                convergence eps 1e-6), then ranorientscatmatrix on its file
+                (orientation-averaged scattering-matrix expansion sm, smcf)
   tmat_poly6    6 spheres of mixed size (x = 0.3 .. 1.6 evenly, orders 3..5) on seeded
                 RSA centres (placed at x = 1.6, fv 0.10), m = 1.4+0.05i:
                 tmatrix_solution at solution eps 1e-8, convergence eps 1e-7
@@ -116,8 +117,10 @@ def lu_case(name, pos, radius, ri):
 
 def tmatrix_case(name, pos, radius, ri, eps, conveps, niter=10000):
     ri = np.asarray(ri, np.complex128)
-    r = O.run_reference(pos, radius, ri, "tmatrix", eps=eps, niter=niter, conveps=conveps)
+    r = O.run_reference(pos, radius, ri, "tmatrix", eps=eps, niter=niter, conveps=conveps, ranorient=True)
+    T = int(r["t_matrix_order"])
     res = dict(pos=pos, radius=radius, ri=ri, order=r["order"], eps=eps, conveps=conveps, niter=niter,
+               rsm=r["rsm"].reshape(2 * T + 1, 4, 4), rsmcf=r["rsmcf"].reshape(2 * T + 1, 4, 4),
                torder=r["torder"], origin=np.array(r["cluster_origin"]), vol_radius=r["vol_radius"],
                T_in=int(r["t_matrix_order_in"]), T=int(r["t_matrix_order"]), istat=int(r["istat"]),
                tmat_header=r["tmat_header"], tmat=r["tmat"], sqeff=r["sqeff"].reshape(-1, 3), tmlog=r["tmlog"])

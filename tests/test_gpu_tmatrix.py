@@ -3,7 +3,9 @@
 * Reference parity (tests/golden/tmat5, tmat_poly6): the solver's FP64
   iterates follow cbicg's, so the order reached, every progress line's
   cumulative iteration count, the T-matrix (the reference file holds 11
-  significant digits: 1e-9) and the per-sphere efficiency sums agree.
+  significant digits: 1e-9) and the per-sphere efficiency sums agree; the
+  orientation-averaged scattering-matrix expansion of the GPU T-matrix
+  (ranorientscatmatrix, restated in the oracle) matches the reference's.
 * Oracle parity at a tight solution eps (1e-10 relative), a sphere exactly at
   the common origin (the zero-translation copy of the distribution), and the
   mixed solver against the FP64 one.
@@ -36,6 +38,12 @@ def test_tmatrix_matches_reference(gpu_ctx, name):
     assert r["tmat"].size == g["tmat"].size
     assert rel(r["tmat"], g["tmat"]) < 1e-9, rel(r["tmat"], g["tmat"])
     assert rel(r["sqeff"], g["sqeff"]) < 1e-10
+    # random orientation: the orientation average of the GPU T-matrix against the
+    # reference's average of its own (oracle restatement of ranorientscatmatrix)
+    from oracle import oracle as O
+
+    sm, smcf = O.ranorient(r["T"], r["tmat"])
+    assert rel(sm, g["rsm"]) < 1e-8 and rel(smcf, g["rsmcf"]) < 1e-8
 
 
 def _oracle_tmatrix(pos, radius, ri, eps, conveps):

@@ -6,6 +6,12 @@ the translation orders, the cumulative iteration counts and Q_ext/Q_abs of
 every order's progress line, the T-matrix file (written by the reference
 with 11 significant digits) and the per-sphere efficiency sums.
 
+The orientation average of ranorientscatmatrix (mstm-scatprops-26.f90:1963-2530)
+restated in the oracle is checked on the reference's own T-matrix files
+against the reference's sm / smcf, and, without the reference, for a single
+sphere: its orientation average is its fixed-orientation result, so the
+(1,1), (1,4), (4,1), (4,4) coefficients equal fosmexpansion's s00.
+
 Without the reference:
   * reciprocity, T^{pq}_{mn,kl} = (-1)^{m+k} T^{qp}_{-k l,-m n}, which ties
     the distribution of the excitation to the merge of the solution (a sign,
@@ -55,3 +61,22 @@ def test_tmatrix_reciprocity_and_extinction():
     bad = o["tmat"].copy()
     bad[1::2] *= -1.0
     assert reciprocity_error(bad, o["T"]) > 1e-2
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_ranorient_matches_reference(name):
+    g = golden(name)
+    sm, smcf = O.ranorient(int(g["T"]), g["tmat"])
+    assert rel(sm, g["rsm"]) < 1e-13 and rel(smcf, g["rsmcf"]) < 1e-13
+
+
+def test_ranorient_single_sphere_equals_fixed_orientation():
+    cl = O.OracleCluster(np.zeros((1, 3)), [1.7], [1.45 + 0.03j])
+    o = cl.tmatrix(eps=1e-14, conveps=0.0)
+    sm, _ = O.ranorient(o["T"], o["tmat"])
+    ob = O.observables(cl, cl.fixedorsoln(eps=1e-14)["amnp"])
+    s00 = np.asarray(ob["s00"])  # [w][j][i] = s00(i, j, w)
+    assert ob["T"] == o["T"]
+    # the average reads the T-matrix into complex(4), as the reference does: 1e-8 relative here
+    for i, j in ((0, 0), (0, 3), (3, 0), (3, 3)):
+        assert np.abs(sm[:, j, i] - s00[:, j, i]).max() < 1e-6 * np.abs(sm[:, 0, 0]).max()
