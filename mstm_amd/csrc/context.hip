@@ -1665,6 +1665,7 @@ int mstm_hip_clear(mstm_hip_ctx *c, int clear_tables)
         if (c->fft) fft_clear(c->fft);
     }
     c->stats = Stats();
+    c->lu.solve_ms = 0;
     if (c->fft) fft_reset_stats(c->fft);
     return 0;
 }
