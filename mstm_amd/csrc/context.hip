@@ -146,6 +146,7 @@ struct mstm_hip_ctx {
     struct {
         double *A = nullptr, *LU = nullptr, *vv = nullptr, *work = nullptr;
         int *piv = nullptr, *err = nullptr;
+        void *scr = nullptr;
         int n = 0;
         bool ready = false;
         double build_ms = 0, factor_ms = 0, solve_ms = 0;
@@ -265,6 +266,7 @@ static void release_lu(mstm_hip_ctx *c)
     dfree(c->lu.work);
     dfree(c->lu.piv);
     dfree(c->lu.err);
+    dfree(c->lu.scr);
     c->lu.n = 0;
     c->lu.ready = false;
 }
@@ -1307,30 +1309,71 @@ static int lu_setup(mstm_hip_ctx *c)
         HIPCHK(dalloc(&c->lu.work, sizeof(double) * 2 * (size_t)n * 8));
         HIPCHK(dalloc(&c->lu.piv, sizeof(int) * n));
         HIPCHK(dalloc(&c->lu.err, sizeof(int)));
+        HIPCHK(dalloc(&c->lu.scr, lu_factor_scratch_bytes()));
         c->lu.n = n;
     }
     c->lu.ready = false;
     auto t0 = std::chrono::steady_clock::now();
-    const int flags[4] = {2, 2, 2, 2};  // mie_mult, no con_tran
-    double *ue = c->lu.work, *th = c->lu.work + 2 * (size_t)n * 4;
-    const bool timing = c->timing;  // the build's products stay out of the tile timings
-    c->timing = false;
+    bool uniform = true;
+    for (int i = 0; i < c->nsph; i++) uniform = uniform && c->order[i] == c->Lmax;
     int rc = 0;
-    for (int j0 = 0; j0 < n && !rc; j0 += 4) {
-        const int nc = std::min(4, n - j0);
-        hipError_t e = launch_lu_unit(ue, n, n, j0, nc, c->st);
-        if (e == hipSuccess) rc = direct_matvec(c, nc, ue, th, flags, MSTM_HIP_FP64);
-        if (e == hipSuccess && !rc) e = launch_lu_store(c->lu.A, n, j0, nc, th, n, c->st);
-        if (e != hipSuccess) rc = fail(c, MSTM_HIP_EHIP, "direct solver: %s", hipGetErrorString(e));
+    if (uniform) {
+        // spheres of one order: every off-diagonal block of A straight from its
+        // pair table (FP64, full layout), in chunks of pairs
+        if (int r2 = ensure_vcc(c, c->Lmax)) return r2;
+        const int L = c->Lmax, CH = 4096, se = slot_len(L, L, 2);
+        double *tab = nullptr, *d_rt = nullptr;
+        int2 *d_pr = nullptr;
+        HIPCHK(hipMemsetAsync(c->lu.A, 0, sizeof(double) * 2 * (size_t)n * n, c->st));
+        HIPCHK(launch_lu_identity(c->lu.A, n, c->st));
+        hipError_t e = dalloc(&tab, sizeof(double) * (size_t)CH * se);
+        if (e == hipSuccess) e = dalloc(&d_rt, sizeof(double) * 3 * CH);
+        if (e == hipSuccess) e = dalloc(&d_pr, sizeof(int2) * CH);
+        std::vector<double> rt(3 * CH);
+        std::vector<int2> pr(CH);
+        int np = 0;
+        auto flush = [&]() {
+            if (e != hipSuccess || np == 0) return;
+            e = hipMemcpyAsync(d_rt, rt.data(), sizeof(double) * 3 * np, hipMemcpyHostToDevice, c->st);
+            if (e == hipSuccess) e = hipMemcpyAsync(d_pr, pr.data(), sizeof(int2) * np, hipMemcpyHostToDevice, c->st);
+            if (e == hipSuccess) e = launch_list_tables<double>(tab, d_rt, np, L, L, 3, c->rimed, c->d_vcc, c->vcc_LM, c->st);
+            if (e == hipSuccess) e = launch_lu_expand(tab, se, d_pr, np, L, n, c->d_an, c->lu.A, c->st);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->st);  // rt/pr are refilled next
+            np = 0;
+        };
+        for (int i = 0; i < c->nsph && e == hipSuccess; i++)
+            for (int j = i + 1; j < c->nsph && e == hipSuccess; j++) {
+                pr[np] = make_int2(i, j);
+                for (int a = 0; a < 3; a++) rt[3 * np + a] = c->pos[3 * i + a] - c->pos[3 * j + a];
+                if (++np == CH) flush();
+            }
+        flush();
+        dfree(tab);
+        dfree(d_rt);
+        dfree(d_pr);
+        if (e != hipSuccess) return fail(c, MSTM_HIP_EHIP, "direct solver build: %s", hipGetErrorString(e));
+    } else {
+        // mixed orders: the columns e_j - T H e_j from the FP64 operator, four at a time
+        const int flags[4] = {2, 2, 2, 2};  // mie_mult, no con_tran
+        double *ue = c->lu.work, *th = c->lu.work + 2 * (size_t)n * 4;
+        const bool timing = c->timing;  // the build's products stay out of the tile timings
+        c->timing = false;
+        for (int j0 = 0; j0 < n && !rc; j0 += 4) {
+            const int nc = std::min(4, n - j0);
+            hipError_t e = launch_lu_unit(ue, n, n, j0, nc, c->st);
+            if (e == hipSuccess) rc = direct_matvec(c, nc, ue, th, flags, MSTM_HIP_FP64);
+            if (e == hipSuccess && !rc) e = launch_lu_store(c->lu.A, n, j0, nc, th, n, c->st);
+            if (e != hipSuccess) rc = fail(c, MSTM_HIP_EHIP, "direct solver: %s", hipGetErrorString(e));
+        }
+        c->timing = timing;
     }
-    c->timing = timing;
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->st));
     auto t1 = std::chrono::steady_clock::now();
     c->lu.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     HIPCHK(hipMemcpyAsync(c->lu.LU, c->lu.A, sizeof(double) * 2 * (size_t)n * n, hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipMemsetAsync(c->lu.err, 0, sizeof(int), c->st));
-    HIPCHK(launch_lu_factor(c->lu.LU, n, c->lu.vv, c->lu.piv, c->lu.err, c->st));
+    HIPCHK(launch_lu_factor(c->lu.LU, n, c->lu.vv, c->lu.piv, c->lu.err, c->lu.scr, c->st));
     int err = 0;
     HIPCHK(hipMemcpyAsync(&err, c->lu.err, sizeof(int), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));

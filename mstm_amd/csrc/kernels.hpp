@@ -71,8 +71,12 @@ int s00_moment_blocks(int T);
 
 // lu.hip (complex FP64, column-major, leading dimension n)
 hipError_t launch_lu_unit(double *x, int64_t ld, int n, int j0, int ncol, hipStream_t st);
+hipError_t launch_lu_expand(const double *tab, int slot_elems, const int2 *pairs, int npairs, int L, int n,
+                            const double *an, double *A, hipStream_t st);
+hipError_t launch_lu_identity(double *A, int n, hipStream_t st);
 hipError_t launch_lu_store(double *A, int n, int j0, int ncol, const double *th, int64_t ld, hipStream_t st);
-hipError_t launch_lu_factor(double *LUm, int n, double *vv, int *piv, int *err, hipStream_t st);
+size_t lu_factor_scratch_bytes();
+hipError_t launch_lu_factor(double *LUm, int n, double *vv, int *piv, int *err, void *scratch, hipStream_t st);
 hipError_t launch_lu_solve(const double *LUm, int n, const int *piv, double *b, int64_t ld, int nrhs, hipStream_t st);
 hipError_t launch_lu_resid(const double *A, int n, const double *x, const double *p, double *r, hipStream_t st);
 hipError_t launch_s00_moments(const double *a0, int T, const double *cgm, const double *cg1, double *out,

@@ -140,6 +140,99 @@ __global__ __launch_bounds__(LU_PANEL_THREADS) void lu_panel_kernel(zc *A, int n
     }
 }
 
+// The same panel factorization spread over the chip, three launches per
+// column j: (1) block partial maxima of vv(i)|a(i,j)| over rows >= j, (2) one
+// workgroup picks the pivot (ties to the larger row), swaps rows j and imax
+// across the panel and keeps 1/a(j,j), (3) scaling of column j and the
+// rank-1 update of the panel's remaining columns, one thread per row.
+#define LU_PIV_BLOCKS 256
+__global__ __launch_bounds__(256) void lu_pivot_kernel(const zc *A, int n, int j, const double *vv, double *pval,
+                                                       int *pidx)
+{
+    __shared__ double sv[4];
+    __shared__ int si[4];
+    const zc *colj = A + (int64_t)j * n;
+    double bv = -1.0;
+    int bi = -1;
+    for (int i = j + blockIdx.x * 256 + threadIdx.x; i < n; i += 256 * gridDim.x) {
+        const double d = vv[i] * zabs(colj[i]);
+        if (d >= bv) {
+            bv = d;
+            bi = i;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_down(bv, off, 64);
+        const int oi = __shfl_down(bi, off, 64);
+        if (ov > bv || (ov == bv && oi > bi)) {
+            bv = ov;
+            bi = oi;
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sv[threadIdx.x >> 6] = bv;
+        si[threadIdx.x >> 6] = bi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; w++)
+            if (sv[w] > bv || (sv[w] == bv && si[w] > bi)) {
+                bv = sv[w];
+                bi = si[w];
+            }
+        pval[blockIdx.x] = bv;
+        pidx[blockIdx.x] = bi;
+    }
+}
+
+__global__ __launch_bounds__(64) void lu_pick_kernel(zc *A, int n, int c0, int nb, int j, double *vv, int *piv,
+                                                     const double *pval, const int *pidx, int nblk, zc *rcp)
+{
+    __shared__ int im_s;
+    if (threadIdx.x == 0) {
+        double v = pval[0];
+        int ix = pidx[0];
+        for (int b = 1; b < nblk; b++)
+            if (pval[b] > v || (pval[b] == v && pidx[b] > ix)) {
+                v = pval[b];
+                ix = pidx[b];
+            }
+        im_s = ix;
+        piv[j] = ix;
+        if (ix != j) vv[ix] = vv[j];
+    }
+    __syncthreads();
+    const int im = im_s;
+    if (im != j)
+        for (int k = c0 + threadIdx.x; k < c0 + nb; k += 64) {
+            zc *ck = A + (int64_t)k * n;
+            const zc t = ck[im];
+            ck[im] = ck[j];
+            ck[j] = t;
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        zc *colj = A + (int64_t)j * n;
+        if (zabs(colj[j]) == 0.0) colj[j] = cmk(1e-20, 0.0);
+        const zc p = colj[j];
+        const double den = p.re * p.re + p.im * p.im;
+        rcp[0] = cmk(p.re / den, -p.im / den);
+    }
+}
+
+__global__ void lu_colupdate_kernel(zc *A, int n, int c0, int nb, int j, const zc *rcp)
+{
+    const int i = j + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    zc *colj = A + (int64_t)j * n;
+    const zc l = cmul(colj[i], rcp[0]);
+    colj[i] = l;
+    for (int k = j + 1; k < c0 + nb; k++) {
+        zc *ck = A + (int64_t)k * n;
+        ck[i] = csub(ck[i], cmul(l, ck[j]));
+    }
+}
+
 // the panel's row swaps on columns [k0, k1)
 __global__ void lu_swap_kernel(zc *A, int n, int c0, int nb, const int *piv, int k0, int k1)
 {
@@ -273,6 +366,64 @@ __global__ __launch_bounds__(LU_SOLVE_THREADS) void lu_solve_kernel(const zc *LU
     }
 }
 
+// Blocked substitution on 32-row blocks.  lower: unit L forward, block
+// rows [r0, r0+nb); else U backward.  The block's triangle in one workgroup
+// (thread = right-hand side x row chunk), then the other rows updated by many
+// workgroups.
+__global__ __launch_bounds__(256) void lu_tri_block_kernel(const zc *LU, int n, int r0, int nb, double *b, int64_t ld,
+                                                           int nrhs, int lower)
+{
+    const int tid = threadIdx.x;
+    for (int r = 0; r < nrhs; r++) {
+        zc *x = reinterpret_cast<zc *>(b + 2 * ld * r);
+        if (lower) {
+            for (int j = r0; j < r0 + nb; j++) {
+                const zc xj = x[j];
+                const zc *cj = LU + (int64_t)j * n;
+                for (int i = j + 1 + tid; i < r0 + nb; i += 256) x[i] = csub(x[i], cmul(cj[i], xj));
+                __syncthreads();
+            }
+        } else {
+            for (int j = r0 + nb - 1; j >= r0; j--) {
+                const zc *cj = LU + (int64_t)j * n;
+                const zc xj = cdiv(x[j], cj[j]);
+                for (int i = r0 + tid; i < j; i += 256) x[i] = csub(x[i], cmul(cj[i], xj));
+                __syncthreads();
+                if (tid == 0) x[j] = xj;
+                __syncthreads();
+            }
+        }
+    }
+}
+
+__global__ void lu_tri_update_kernel(const zc *LU, int n, int r0, int nb, double *b, int64_t ld, int nrhs, int lower)
+{
+    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = lower ? r0 + nb + i0 : i0;
+    if (lower ? i >= n : i >= r0) return;
+    for (int r = 0; r < nrhs; r++) {
+        zc *x = reinterpret_cast<zc *>(b + 2 * ld * r);
+        zc s = x[i];
+        for (int j = r0; j < r0 + nb; j++) s = csub(s, cmul(LU[(int64_t)j * n + i], x[j]));
+        x[i] = s;
+    }
+}
+
+__global__ void lu_permute_kernel(const int *piv, int n, double *b, int64_t ld, int nrhs)
+{
+    const int r = threadIdx.x;
+    if (r >= nrhs) return;
+    zc *x = reinterpret_cast<zc *>(b + 2 * ld * r);
+    for (int i = 0; i < n; i++) {
+        const int p = piv[i];
+        if (p != i) {
+            const zc t = x[p];
+            x[p] = x[i];
+            x[i] = t;
+        }
+    }
+}
+
 // r = A x - p (one thread per row)
 __global__ void lu_resid_kernel(const zc *A, int n, const double *x, const double *p, double *r)
 {
@@ -285,7 +436,105 @@ __global__ void lu_resid_kernel(const zc *A, int n, const double *x, const doubl
     r[2 * i + 1] = s.im - p[2 * i + 1];
 }
 
+// unpack a model-2 address e of a mode block of order L into (m, n)
+__device__ inline void lu_unpack(int e, int L, int &m, int &n)
+{
+    const int row = e % (L + 2), colm = e / (L + 2) + 1;
+    if (row <= colm) {
+        m = row;
+        n = colm;
+    } else {
+        m = -colm;
+        n = row - 1;
+    }
+}
+
+// Direct build of A = I - T H for spheres of one order L (offsets 2 NB i):
+// one workgroup per unordered pair (i, j) of the chunk, from its FP64 list
+// table (target i, source j, full layout).  Element of the forward block
+//   H_ij[(k,t),(k',q)] = e^{-ik phi} e^{ik' phi} sum_m d^t(m,k) A_m(t,q) d^q(m,k')
+// and of the reverse block (the shift form of the same table)
+//   H_ji[(k,t),(k',q)] = (-1)^{k+k'} e^{-ik phi} e^{ik' phi} sum_m d^t(m,-k) A_m(q,t) d^q(m,-k');
+// H is diagonal in the mode, and A takes -T_target(t)[p][mode] H for both
+// target modes p (multmiecoeffmult of the column, mstm-modules-33.f90:5976-5981).
+__global__ __launch_bounds__(256) void lu_expand_kernel(const double *__restrict__ tab, int slot_elems,
+                                                        const int2 *__restrict__ pairs, int L, int n,
+                                                        const zc *__restrict__ an, zc *__restrict__ A)
+{
+    extern __shared__ double sh[];
+    const int q = blockIdx.x;
+    const double *grp = tab + (int64_t)(q / MSTM_SLOTS) * slot_elems * MSTM_SLOTS;
+    const int ks = q % MSTM_SLOTS;
+    for (int e = threadIdx.x; e < slot_elems; e += blockDim.x) sh[e] = grp[ilv(e, ks, 2)];
+    __syncthreads();
+    const int2 pr = pairs[q];
+    const int NB = L * (L + 2);
+    const int po = phi_off(L, L, 2);
+    const double phi = atan2(sh[po + 1], sh[po]);
+    for (int e = threadIdx.x; e < 4 * NB * NB; e += blockDim.x) {
+        const int sc = e % NB, tr = (e / NB) % NB, mode = (e / (NB * NB)) & 1, dir = e / (2 * NB * NB);
+        int kt, t, kq, qd;
+        lu_unpack(tr, L, kt, t);
+        lu_unpack(sc, L, kq, qd);
+        const int mm = imn(t, qd);
+        double sr = 0.0, si = 0.0;
+        for (int m = -mm; m <= mm; m++) {
+            const int b = (m == 0) ? 0 : (m < 0 ? -2 * m - 1 : 2 * m);
+            const int m1 = imx(1, iab(m));
+            const int e0 = sec_off(b, L, L, 2);
+            const int ND = L - m1 + 1;
+            const int ea = e0 + pad_to(rot_len(m, L, L), 2) + mode * pad_to(ax_len(m, L, L), 2);
+            double dt, ds;
+            int ia;
+            if (!dir) {
+                dt = sh[e0 + t * t - m1 * m1 + kt + t];
+                ds = sh[e0 + qd * qd - m1 * m1 + kq + qd];
+                ia = ea + 2 * ((t - m1) + ND * (qd - m1));
+            } else {
+                dt = sh[e0 + t * t - m1 * m1 - kt + t];
+                ds = sh[e0 + qd * qd - m1 * m1 - kq + qd];
+                ia = ea + 2 * ((qd - m1) + ND * (t - m1));
+            }
+            const double f = dt * ds;
+            sr = fma(f, sh[ia], sr);
+            si = fma(f, sh[ia + 1], si);
+        }
+        double sn, cs;
+        sincos((double)(kq - kt) * phi, &sn, &cs);
+        zc h = cmk(sr * cs - si * sn, sr * sn + si * cs);
+        if (dir && ((kt + kq) & 1)) h = cmk(-h.re, -h.im);
+        const int it = dir ? pr.y : pr.x, js = dir ? pr.x : pr.y;
+        const zc *tt = an + ((int64_t)it * L + t - 1) * 4;  // T[p][q] = tt[2q + p]
+        const int64_t col = (int64_t)js * 2 * NB + mode * NB + sc;
+        for (int p = 0; p < 2; p++) {
+            const zc v = cmul(tt[2 * mode + p], h);
+            A[col * n + (int64_t)it * 2 * NB + p * NB + tr] = cmk(-v.re, -v.im);
+        }
+    }
+}
+
+__global__ void lu_identity_kernel(zc *A, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) A[(int64_t)i * n + i] = cmk(1.0, 0.0);
+}
+
 }  // namespace
+
+hipError_t launch_lu_expand(const double *tab, int slot_elems, const int2 *pairs, int npairs, int L, int n,
+                            const double *an, double *A, hipStream_t st)
+{
+    if (npairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(lu_expand_kernel, dim3(npairs), dim3(256), sizeof(double) * slot_elems, st, tab, slot_elems,
+                       pairs, L, n, reinterpret_cast<const zc *>(an), reinterpret_cast<zc *>(A));
+    return hipGetLastError();
+}
+
+hipError_t launch_lu_identity(double *A, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(lu_identity_kernel, dim3((n + 255) / 256), dim3(256), 0, st, reinterpret_cast<zc *>(A), n);
+    return hipGetLastError();
+}
 
 hipError_t launch_lu_unit(double *x, int64_t ld, int n, int j0, int ncol, hipStream_t st)
 {
@@ -302,14 +551,34 @@ hipError_t launch_lu_store(double *A, int n, int j0, int ncol, const double *th,
     return hipGetLastError();
 }
 
-hipError_t launch_lu_factor(double *LUm, int n, double *vv, int *piv, int *err, hipStream_t st)
+// scratch: pval[LU_PIV_BLOCKS] doubles, pidx[LU_PIV_BLOCKS] ints, rcp one complex
+size_t lu_factor_scratch_bytes() { return LU_PIV_BLOCKS * (sizeof(double) + sizeof(int)) + 2 * sizeof(double); }
+
+hipError_t launch_lu_factor(double *LUm, int n, double *vv, int *piv, int *err, void *scratch, hipStream_t st)
 {
     zc *A = reinterpret_cast<zc *>(LUm);
+    double *pval = reinterpret_cast<double *>(scratch);
+    zc *rcp = reinterpret_cast<zc *>(pval + LU_PIV_BLOCKS);
+    int *pidx = reinterpret_cast<int *>(pval + LU_PIV_BLOCKS + 2);
     hipLaunchKernelGGL(lu_rowscale_kernel, dim3((n + 255) / 256), dim3(256), 0, st, A, n, vv, err);
     hipError_t e = hipGetLastError();
     for (int c0 = 0; c0 < n && e == hipSuccess; c0 += LU_NB) {
         const int nb = n - c0 < LU_NB ? n - c0 : LU_NB;
-        hipLaunchKernelGGL(lu_panel_kernel, dim3(1), dim3(LU_PANEL_THREADS), 0, st, A, n, c0, nb, vv, piv);
+        if (n - c0 <= 4096) {
+            // short panels: one workgroup does all its columns
+            hipLaunchKernelGGL(lu_panel_kernel, dim3(1), dim3(LU_PANEL_THREADS), 0, st, A, n, c0, nb, vv, piv);
+        } else {
+            for (int jj = 0; jj < nb; jj++) {
+                const int j = c0 + jj, rows = n - j;
+                const int nbk = imn(LU_PIV_BLOCKS, (rows + 255) / 256);
+                hipLaunchKernelGGL(lu_pivot_kernel, dim3(nbk), dim3(256), 0, st, A, n, j, vv, pval, pidx);
+                hipLaunchKernelGGL(lu_pick_kernel, dim3(1), dim3(64), 0, st, A, n, c0, nb, j, vv, piv, pval, pidx,
+                                   nbk, rcp);
+                if (rows > 1)
+                    hipLaunchKernelGGL(lu_colupdate_kernel, dim3((rows - 1 + 255) / 256), dim3(256), 0, st, A, n, c0,
+                                       nb, j, rcp);
+            }
+        }
         if (c0 > 0) hipLaunchKernelGGL(lu_swap_kernel, dim3((c0 + 255) / 256), dim3(256), 0, st, A, n, c0, nb, piv, 0, c0);
         const int rest = n - c0 - nb;
         if (rest > 0) {
@@ -326,8 +595,27 @@ hipError_t launch_lu_factor(double *LUm, int n, double *vv, int *piv, int *err, 
 hipError_t launch_lu_solve(const double *LUm, int n, const int *piv, double *b, int64_t ld, int nrhs, hipStream_t st)
 {
     if (nrhs < 1 || nrhs > 4) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(lu_solve_kernel, dim3(1), dim3(LU_SOLVE_THREADS), 0, st, reinterpret_cast<const zc *>(LUm), n,
-                       piv, b, ld, nrhs);
+    const zc *LU = reinterpret_cast<const zc *>(LUm);
+    if (n <= 2048) {  // small systems: one workgroup does it all
+        hipLaunchKernelGGL(lu_solve_kernel, dim3(1), dim3(LU_SOLVE_THREADS), 0, st, LU, n, piv, b, ld, nrhs);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(lu_permute_kernel, dim3(1), dim3(64), 0, st, piv, n, b, ld, nrhs);
+    for (int r0 = 0; r0 < n; r0 += LU_NB) {
+        const int nb = n - r0 < LU_NB ? n - r0 : LU_NB, rest = n - r0 - nb;
+        hipLaunchKernelGGL(lu_tri_block_kernel, dim3(1), dim3(256), 0, st, LU, n, r0, nb, b, ld, nrhs, 1);
+        if (rest > 0)
+            hipLaunchKernelGGL(lu_tri_update_kernel, dim3((rest + 255) / 256), dim3(256), 0, st, LU, n, r0, nb, b, ld,
+                               nrhs, 1);
+    }
+    const int nblk = (n + LU_NB - 1) / LU_NB;
+    for (int bk = nblk - 1; bk >= 0; bk--) {
+        const int r0 = bk * LU_NB, nb = n - r0 < LU_NB ? n - r0 : LU_NB;
+        hipLaunchKernelGGL(lu_tri_block_kernel, dim3(1), dim3(256), 0, st, LU, n, r0, nb, b, ld, nrhs, 0);
+        if (r0 > 0)
+            hipLaunchKernelGGL(lu_tri_update_kernel, dim3((r0 + 255) / 256), dim3(256), 0, st, LU, n, r0, nb, b, ld,
+                               nrhs, 0);
+    }
     return hipGetLastError();
 }
 
