@@ -125,10 +125,13 @@ template <typename T, int L> constexpr int tile_lds_elems()
     return tile_staged<T, L>() ? direct_slot_len(L, sizeof(T)) * MSTM_SLOTS : 1;
 }
 // pipelined form: the tile's two source blocks (16 spheres x 2 columns x 2
-// modes x NB complex each) also sit in LDS for the whole tile
+// modes x NB complex each) also sit in LDS for the whole tile; each sphere's
+// 4 NB values are followed by SRC_PAD unused ones, so that the 16 slots of a
+// wave, reading 16 different spheres, spread over the LDS banks
+#define SRC_PAD 2
 template <typename T, int L, int NR> constexpr int tile_src_elems()
 {
-    return tile_src_staged<T, L, NR>() ? 2 * MSTM_SLOTS * 4 * L * (L + 2) * 2 : 1;
+    return tile_src_staged<T, L, NR>() ? 2 * MSTM_SLOTS * (4 * L * (L + 2) + SRC_PAD) * 2 : 1;
 }
 // copy table group s of a tile into LDS, all threads of the workgroup
 template <typename T, int L, int NR>
@@ -181,7 +184,7 @@ __device__ __forceinline__ void tile_body_sym(int2 bt, int tile, const T *__rest
         }
         const cplx<T> *xs = xin + ((int64_t)(rhs * npad + src) * 2 + mode) * NB;
         // staged copy: block 0 = bt.y (forward sources), 1 = bt.x (reverse sources)
-        const cplx<T> *xl = srcl + (((DIR * S + (src % S)) * 2 + (rhs & 1)) * 2 + mode) * NB;
+        const cplx<T> *xl = srcl + (DIR * S + (src % S)) * (4 * NB + SRC_PAD) + ((rhs & 1) * 2 + mode) * NB;
         __syncthreads();
         cplx<T> xv[PIPE ? NB : 1];
         if constexpr (PIPE) {
@@ -387,7 +390,7 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
         }
         const cplx<T> *xs = xin + ((int64_t)(rhs * npad + src) * 2 + mode) * NB;
         // staged copy: block 0 = bt.y (forward sources), 1 = bt.x (reverse sources)
-        const cplx<T> *xl = srcl + (((DIR * S + (src % S)) * 2 + (rhs & 1)) * 2 + mode) * NB;
+        const cplx<T> *xl = srcl + (DIR * S + (src % S)) * (4 * NB + SRC_PAD) + ((rhs & 1) * 2 + mode) * NB;
         // all waves of the tile read the same table group in step s: keep them
         // in step so that the group is fetched from HBM once (and, staged,
         // copied to LDS once)
@@ -542,7 +545,7 @@ __global__ __launch_bounds__(64 * NR) __attribute__((amdgpu_waves_per_eu(tile_wa
             const int b = e / (2 * BLK), rem = e - b * 2 * BLK, r = rem / BLK, o = rem - r * BLK;
             const int blk = b == 0 ? bt.y : bt.x;
             const int i = o / (2 * NB), mo = o - i * 2 * NB;  // sphere in block, (mode, coefficient)
-            srcl[((b * MSTM_SLOTS + i) * 2 + r) * 2 * NB + mo] =
+            srcl[(b * MSTM_SLOTS + i) * (4 * NB + SRC_PAD) + r * 2 * NB + mo] =
                 xin[((int64_t)(r * npad + blk * MSTM_SLOTS + i) * 2) * NB + mo];
         }
     }
