@@ -210,3 +210,36 @@ def test_stalled_refinement_falls_back_to_plain_fp64(gpu_ctx, solver):
     # the handover starts from b, as the reference does: same iterates
     assert r["iterations"] == g["iterations"]
     assert rel(r["amnp"], g["amnp"]) < FP64_TOL
+
+
+@pytest.mark.parametrize("nsph,x", [(1, 2.0), (2, 0.8), (17, 1.3)])
+def test_small_and_ragged_clusters_match_oracle(gpu_ctx, nsph, x):
+    """One sphere (a lone diagonal tile of padding and a self pair), two, and
+    17 (a full 16-sphere block plus a block of one): FP64 / FP32 products,
+    the FP64 solve (same iterations) and the direct solution against the
+    oracle."""
+    from mstm_amd import geometry as G
+    from mstm_amd.solver import FP32, FP64, SOLVE_LU, SOLVE_PLAIN
+    from oracle import oracle as O
+
+    c = G.rsa_in_sphere(nsph, x, 0.15, seed=60 + nsph, ri=1.45 + 0.02j)
+    gpu_ctx.set_spheres(c.pos, c.radius, c.ri, 1.0, 1e-6)
+    oc = O.OracleCluster(c.pos, c.radius, c.ri)
+    ne = oc.neqns
+    rng = np.random.default_rng(nsph)
+    a = rng.normal(size=(2, ne)) + 1j * rng.normal(size=(2, ne))
+    ref = oc.sphereinteraction(a, [0, 1], [1, 1])
+    out64 = gpu_ctx.sphereinteraction(a, [False, True], [True, True], precision=FP64)
+    out32 = gpu_ctx.sphereinteraction(a, [False, True], [True, True], precision=FP32)
+    if nsph == 1:
+        assert np.abs(out64).max() == 0.0 and np.abs(out32).max() == 0.0
+    else:
+        assert rel(out64, ref) < 1e-12
+        assert rel(out32, ref) < 2e-5
+    r = gpu_ctx.fixedorsoln(eps=1e-10, niter=1000, solver=SOLVE_PLAIN)
+    ro = oc.fixedorsoln(eps=1e-10, niter=1000)
+    assert r["iterations"] == ro["iterations"] and rel(r["amnp"], ro["amnp"]) < 1e-10
+    rl = gpu_ctx.fixedorsoln(eps=1e-10, niter=1000, solver=SOLVE_LU)
+    pw = oc.planewave()
+    x = oc.direct_solve(np.stack([oc.mie_mult(pw[0]), oc.mie_mult(pw[1])]))
+    assert rel(rl["amnp"], x) < 1e-10
