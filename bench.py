@@ -419,7 +419,7 @@ def main():
         return bench_lu(args)
 
     from mstm_amd import geometry as G
-    from mstm_amd.solver import FP32, FP64, SOLVE_LEARNED, SOLVE_MIXED, SOLVE_PLAIN, HipContext, qtotcalc
+    from mstm_amd.solver import FP32, FP64, SOLVE_FGMRES, SOLVE_LEARNED, SOLVE_MIXED, SOLVE_PLAIN, HipContext, qtotcalc
 
     t0 = time.time()
     cl = G.rsa_in_sphere(args.nsphere, 2.0, 0.30, seed=5, ri=1.31 + 0.0j)
@@ -467,6 +467,15 @@ def main():
     ob_mixed = observe(ctx)
     print(f"[bench] mixed (no network): {mixed_s:.3f} s, iters {rm['iterations']}", file=sys.stderr, flush=True)
 
+    # ---- flexible GMRES with the learned first preconditioner (once) ---------
+    ctx.clear(clear_tables=False)
+    t0 = time.perf_counter()
+    rf = ctx.fixedorsoln(eps=1e-6, niter=10000, solver=SOLVE_FGMRES)
+    fg_s = time.perf_counter() - t0
+    st_fg = ctx.stats()
+    q_fg = qtotcalc(cl.radius, rf["qeff"])
+    print(f"[bench] flexible GMRES: {fg_s:.3f} s, steps {st_fg['fgmres_steps']:.0f}", file=sys.stderr, flush=True)
+
     # ---- plain FP64 path (once) ----------------------------------------------
     plain = None
     if not args.no_plain:
@@ -496,6 +505,7 @@ def main():
         qerr = {}
         runs = (("learned", q_lig, ob_lig), ("mixed", q_mixed, ob_mixed)) + \
             ((("plain", q_plain, ob_plain),) if plain else ())
+        qerr["fgmres"] = dict(qext=abs(q_fg[0, 0] - q_ref[0, 0]) / abs(q_ref[0, 0]))
         for name, q, ob in runs:
             qerr[name] = dict(qext=abs(q[0, 0] - q_ref[0, 0]) / abs(q_ref[0, 0]),
                               qsca=abs(q[0, 2] - q_ref[0, 2]) / abs(q_ref[0, 2]),
@@ -593,6 +603,8 @@ def main():
                       iters_fp64_matvecs=st_mixed["matvecs_fp64"] / len(mtimes),
                       refinements=st_mixed["refinements"] / len(mtimes),
                       speedup_vs_plain_fp64=round(plain["s_per_solve"] / mixed_s, 3) if plain else None),
+        "fgmres": dict(s_per_solve=round(fg_s, 4), steps=st_fg["fgmres_steps"], iters_fp32=st_fg["iters_fp32"],
+                       maxerr=rf["maxerr"], qext=q_fg[0, 0], fallbacks=st_fg["lig_fallbacks"]),
         "q_error_vs_tight_fp64": qerr,
         "observables": dict(t_matrix_order=ob_lig["T"], g=ob_lig["g"], ms=ob_lig["ms"],
                             note="post-solve, on the GPU: translation orders (host), merge to the common origin, "
