@@ -588,7 +588,8 @@ def main():
                    "nsphere": cl.n, "order": L, "number_eqns": ne, "pairs": npairs},
         "plain_fp64": plain,
         "speedup_vs_plain_fp64": round(plain["s_per_solve"] / lig_s, 3) if plain else None,
-        "learned": dict(iterations_last_pol=res["iterations"], maxerr=res["maxerr"], qext=q_lig[0, 0],
+        "learned": dict(iterations_last_pol=res["iterations"], maxerr=res["maxerr"],
+                        fp64_rel_residual=float(np.sqrt(res["maxerr"])), qext=q_lig[0, 0],
                         qabs=q_lig[0, 1], iters_fp32=st_lig["iters_fp32"] / args.steps,
                         iters_fp64_matvecs=st_lig["matvecs_fp64"] / args.steps,
                         refinements=st_lig["refinements"] / args.steps,
@@ -599,6 +600,7 @@ def main():
                                                                  configs=meta.get("configs"),
                                                                  loss=meta.get("loss_final"))),
         "mixed": dict(s_per_solve=round(mixed_s, 4), iterations_last_pol=rm["iterations"], maxerr=rm["maxerr"],
+                      fp64_rel_residual=float(np.sqrt(rm["maxerr"])),
                       qext=q_mixed[0, 0], qabs=q_mixed[0, 1], iters_fp32=st_mixed["iters_fp32"] / len(mtimes),
                       iters_fp64_matvecs=st_mixed["matvecs_fp64"] / len(mtimes),
                       refinements=st_mixed["refinements"] / len(mtimes),
