@@ -41,7 +41,7 @@ def direct_sym(L: int, elem: int) -> bool:
     return L <= (5 if elem == 4 else 4)
 
 
-def survey_table_bytes_per_pair(L: int, elem: int) -> int:
+def full_table_bytes_per_pair(L: int, elem: int) -> int:
     """Full rotation + axial table of one unordered pair (equal orders L):
     sum_{n<=L}(2n+1)^2 reals + 2 sum_{|m|<=L} (L-m1+1)^2 complex."""
     rot = sum((2 * n + 1) ** 2 for n in range(1, L + 1))
@@ -63,10 +63,10 @@ def table_bytes_per_pair(L: int, elem: int) -> int:
     if direct_sym(L, elem):
         rot, ax = sym_counts(L)
         return rot * elem + ax * 2 * elem
-    return survey_table_bytes_per_pair(L, elem)
+    return full_table_bytes_per_pair(L, elem)
 
 
-def survey_tile_flops_per_pair(L: int, nrhs: int = 2) -> int:
+def full_tile_flops_per_pair(L: int, nrhs: int = 2) -> int:
     """Flops of both directions of one pair for nrhs columns with full
     rotations: 16 (S_rot + S_ax) per direction and column (equal orders)."""
     s_rot = sum((2 * n + 1) ** 2 for n in range(1, L + 1))
@@ -80,7 +80,7 @@ def tile_flops_per_pair(L: int, nrhs: int = 2, elem: int = 4) -> int:
     lengths multiply-adds per rotation instead of S_rot; axial work is
     unchanged: 16 (R + S_ax) per direction and column."""
     if not direct_sym(L, elem):
-        return survey_tile_flops_per_pair(L, nrhs)
+        return full_tile_flops_per_pair(L, nrhs)
     rot, _ = sym_counts(L)
     s_ax = sum((L - max(1, abs(m)) + 1) ** 2 for m in range(-L, L + 1))
     return 2 * nrhs * 16 * (rot + s_ax)
@@ -544,7 +544,7 @@ def main():
         rec = json.load(open(pmc)).get("by_columns", {}).get(str(ncol))
         traffic = rec["hbm_bytes_per_launch"] if rec else None
     fl32 = npairs * tile_flops_per_pair(L, ncol, 4)
-    surv32 = npairs * survey_table_bytes_per_pair(L, 4) + vec_bytes
+    surv32 = npairs * full_table_bytes_per_pair(L, 4) + vec_bytes
     roof = dict(kernel=f"direct_tile_kernel<float,{L},{ncol}>", bound="hbm", achieved=round(ach32, 1),
                 peak=HBM_PEAK_GBPS, unit="GB/s", frac=round(ach32 / HBM_PEAK_GBPS, 4), traffic=traffic,
                 algorithmic_bytes=alg32, columns=ncol, avg_launch_ms=round(t32, 4),
