@@ -9,8 +9,9 @@
 //       maximizing vv(i) |a(i,j)| (vv(i) = 1 / max_k |a(i,k)| of the input
 //       rows, moved with the row swaps; ties go to the last row), a zero
 //       pivot becomes 1e-20.  Here the same pivot rule drives a blocked
-//       right-looking factorization (panels of LU_NB columns, row swaps,
-//       triangular solve of the block row, trailing update by GEMM);
+//       factorization (panels of LU_NB columns, left-looking inside a group
+//       of LU_GROUP panels, then the group's row swaps, block row and one
+//       rank-LU_GROUP*LU_NB trailing update on the FP64 matrix cores);
 //   lu_backsubstitution          mstm-solver-8.f90:1022-1052
 //   direct_solver                mstm-solver-8.f90:571-642
 //       solve, then niter refinement steps r = A x - p, x -= LU^{-1} r.
@@ -21,6 +22,7 @@
 #include "kernels.hpp"
 
 #define LU_NB 32
+#define LU_GROUP 4  // panels per trailing update
 #define LU_PANEL_THREADS 1024
 #define LU_SOLVE_THREADS 1024
 
@@ -249,9 +251,10 @@ __global__ void lu_swap_kernel(zc *A, int n, int c0, int nb, const int *piv, int
     }
 }
 
-// block row U12 = L11^{-1} A12 (L11 unit lower, LU_NB x LU_NB: only full
-// panels have a block row to their right), one column per thread
-__global__ __launch_bounds__(256) void lu_trsm_kernel(zc *A, int n, int c0)
+// block row U12 = L11^{-1} A12 for the columns [kb, ke) (L11 unit lower,
+// LU_NB x LU_NB at (c0, c0): only full panels have a block row to their
+// right), one column per thread
+__global__ __launch_bounds__(256) void lu_trsm_kernel(zc *A, int n, int c0, int kb, int ke)
 {
     __shared__ zc l11[LU_NB][LU_NB];
     for (int e = threadIdx.x; e < LU_NB * LU_NB; e += blockDim.x) {
@@ -259,8 +262,8 @@ __global__ __launch_bounds__(256) void lu_trsm_kernel(zc *A, int n, int c0)
         l11[r][c] = A[(int64_t)(c0 + c) * n + c0 + r];
     }
     __syncthreads();
-    const int k = c0 + LU_NB + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+    const int k = kb + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ke) return;
     zc *ck = A + (int64_t)k * n + c0;
     zc x[LU_NB];
 #pragma unroll
@@ -276,52 +279,80 @@ __global__ __launch_bounds__(256) void lu_trsm_kernel(zc *A, int n, int c0)
     for (int r = 0; r < LU_NB; r++) ck[r] = x[r];
 }
 
-// trailing update A22 -= L21 U12: 64x64 output tiles, 256 threads x 4x4,
-// K = nb staged in LDS in halves of 16
-__global__ __launch_bounds__(256) void lu_gemm_kernel(zc *A, int n, int c0, int nb)
+// update A[rs:re, kb:ke] -= A[rs:re, c0:c0+kd] A[c0:c0+kd, kb:ke] (the
+// trailing update A22 -= L21 U12) on the FP64 matrix cores: a 64x64 complex
+// output tile per workgroup, each of the 4 waves a 32x32 quarter as 2x2
+// v_mfma_f64_16x16x4f64 blocks with real and imaginary accumulators (four
+// real products per complex one: re += Lr Ur - Li Ui, im += Lr Ui + Li Ur).
+// The instruction forms D[i][j] (i = lane/16 + 4*item, j = lane%16) from
+// A[i][k] and B[k][j] held as lane (i or j) + 16*k; it is applied to the
+// transposed product U12^T L21^T so that j runs down the rows of A22 and a
+// wave's read-modify-write of its block is 16 consecutive rows per column.
+typedef double lu_d4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void lu_gemm_kernel(zc *A, int n, int c0, int kd, int rs, int re, int kb, int ke)
 {
     constexpr int TK = 16;
     __shared__ zc Ls[TK][64 + 1];
     __shared__ zc Us[TK][64 + 1];
-    const int r0 = c0 + nb + blockIdx.x * 64, k0 = c0 + nb + blockIdx.y * 64;
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    zc acc[4][4];
+    const int r0 = rs + blockIdx.x * 64, k0 = kb + blockIdx.y * 64;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = (w & 1) * 32, wc = (w >> 1) * 32, li = lane & 15, lk = lane >> 4;
+    lu_d4 cr[2][2], ci[2][2];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = cmk(0.0, 0.0);
-    for (int kk = 0; kk < nb; kk += TK) {
+        for (int b = 0; b < 2; b++) {
+            cr[a][b] = lu_d4{0.0, 0.0, 0.0, 0.0};
+            ci[a][b] = lu_d4{0.0, 0.0, 0.0, 0.0};
+        }
+    for (int kk = 0; kk < kd; kk += TK) {
         for (int e = threadIdx.x; e < TK * 64; e += 256) {
             const int q = e / 64, rr = e % 64;
             const int row = r0 + rr, col = k0 + rr;
-            Ls[q][rr] = (row < n && kk + q < nb) ? A[(int64_t)(c0 + kk + q) * n + row] : cmk(0.0, 0.0);
-            Us[q][rr] = (col < n && kk + q < nb) ? A[(int64_t)col * n + c0 + kk + q] : cmk(0.0, 0.0);
+            Ls[q][rr] = (row < re && kk + q < kd) ? A[(int64_t)(c0 + kk + q) * n + row] : cmk(0.0, 0.0);
+            Us[q][rr] = (col < ke && kk + q < kd) ? A[(int64_t)col * n + c0 + kk + q] : cmk(0.0, 0.0);
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < TK; q++) {
-            zc lv[4], uv[4];
+        for (int q = 0; q < TK; q += 4) {
+            double ur[2], ui[2], lr[2], lm[2], lim[2];
 #pragma unroll
-            for (int a = 0; a < 4; a++) lv[a] = Ls[q][ty + 16 * a];
+            for (int a = 0; a < 2; a++) {  // a: column block (MFMA rows i)
+                const zc u = Us[q + lk][wc + 16 * a + li];
+                ur[a] = u.re;
+                ui[a] = u.im;
+            }
 #pragma unroll
-            for (int b = 0; b < 4; b++) uv[b] = Us[q][tx + 16 * b];
+            for (int b = 0; b < 2; b++) {  // b: row block (MFMA columns j)
+                const zc l = Ls[q + lk][wr + 16 * b + li];
+                lr[b] = l.re;
+                lm[b] = l.im;
+                lim[b] = -l.im;
+            }
 #pragma unroll
-            for (int a = 0; a < 4; a++)
+            for (int a = 0; a < 2; a++)
 #pragma unroll
-                for (int b = 0; b < 4; b++) cfma(acc[a][b], lv[a], uv[b]);
+                for (int b = 0; b < 2; b++) {
+                    cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[a], lr[b], cr[a][b], 0, 0, 0);
+                    cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ui[a], lim[b], cr[a][b], 0, 0, 0);
+                    ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[a], lm[b], ci[a][b], 0, 0, 0);
+                    ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ui[a], lr[b], ci[a][b], 0, 0, 0);
+                }
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
-            const int row = r0 + ty + 16 * a, col = k0 + tx + 16 * b;
-            if (row < n && col < n) {
-                zc &o = A[(int64_t)col * n + row];
-                o = csub(o, acc[a][b]);
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int col = k0 + wc + 16 * a + lk + 4 * t, row = r0 + wr + 16 * b + li;
+                if (row < re && col < ke) {
+                    zc &o = A[(int64_t)col * n + row];
+                    o = cmk(o.re - cr[a][b][t], o.im - ci[a][b][t]);
+                }
             }
-        }
 }
 
 // LU x = P b for nrhs columns of b (2*ld doubles apart), one workgroup:
@@ -562,30 +593,66 @@ hipError_t launch_lu_factor(double *LUm, int n, double *vv, int *piv, int *err, 
     int *pidx = reinterpret_cast<int *>(pval + LU_PIV_BLOCKS + 2);
     hipLaunchKernelGGL(lu_rowscale_kernel, dim3((n + 255) / 256), dim3(256), 0, st, A, n, vv, err);
     hipError_t e = hipGetLastError();
-    for (int c0 = 0; c0 < n && e == hipSuccess; c0 += LU_NB) {
-        const int nb = n - c0 < LU_NB ? n - c0 : LU_NB;
+    // one panel of nb columns at c0: pivot search, row swaps and elimination
+    // inside the panel's own columns
+    auto panel = [&](int c0, int nb) {
         if (n - c0 <= 4096) {
             // short panels: one workgroup does all its columns
             hipLaunchKernelGGL(lu_panel_kernel, dim3(1), dim3(LU_PANEL_THREADS), 0, st, A, n, c0, nb, vv, piv);
-        } else {
-            for (int jj = 0; jj < nb; jj++) {
-                const int j = c0 + jj, rows = n - j;
-                const int nbk = imn(LU_PIV_BLOCKS, (rows + 255) / 256);
-                hipLaunchKernelGGL(lu_pivot_kernel, dim3(nbk), dim3(256), 0, st, A, n, j, vv, pval, pidx);
-                hipLaunchKernelGGL(lu_pick_kernel, dim3(1), dim3(64), 0, st, A, n, c0, nb, j, vv, piv, pval, pidx,
-                                   nbk, rcp);
-                if (rows > 1)
-                    hipLaunchKernelGGL(lu_colupdate_kernel, dim3((rows - 1 + 255) / 256), dim3(256), 0, st, A, n, c0,
-                                       nb, j, rcp);
-            }
+            return;
         }
-        if (c0 > 0) hipLaunchKernelGGL(lu_swap_kernel, dim3((c0 + 255) / 256), dim3(256), 0, st, A, n, c0, nb, piv, 0, c0);
-        const int rest = n - c0 - nb;
-        if (rest > 0) {
-            hipLaunchKernelGGL(lu_swap_kernel, dim3((rest + 255) / 256), dim3(256), 0, st, A, n, c0, nb, piv, c0 + nb, n);
-            hipLaunchKernelGGL(lu_trsm_kernel, dim3((rest + 255) / 256), dim3(256), 0, st, A, n, c0);
-            const int g = (rest + 63) / 64;
-            hipLaunchKernelGGL(lu_gemm_kernel, dim3(g, g), dim3(256), 0, st, A, n, c0, nb);
+        for (int jj = 0; jj < nb; jj++) {
+            const int j = c0 + jj, rows = n - j;
+            const int nbk = imn(LU_PIV_BLOCKS, (rows + 255) / 256);
+            hipLaunchKernelGGL(lu_pivot_kernel, dim3(nbk), dim3(256), 0, st, A, n, j, vv, pval, pidx);
+            hipLaunchKernelGGL(lu_pick_kernel, dim3(1), dim3(64), 0, st, A, n, c0, nb, j, vv, piv, pval, pidx, nbk,
+                               rcp);
+            if (rows > 1)
+                hipLaunchKernelGGL(lu_colupdate_kernel, dim3((rows - 1 + 255) / 256), dim3(256), 0, st, A, n, c0, nb,
+                                   j, rcp);
+        }
+    };
+    auto swap = [&](int p0, int np, int k0, int k1) {  // pivots [p0, p0+np) on columns [k0, k1)
+        if (k1 > k0)
+            hipLaunchKernelGGL(lu_swap_kernel, dim3((k1 - k0 + 255) / 256), dim3(256), 0, st, A, n, p0, np, piv, k0,
+                               k1);
+    };
+    auto trsm = [&](int c0, int k0, int k1) {
+        if (k1 > k0) hipLaunchKernelGGL(lu_trsm_kernel, dim3((k1 - k0 + 255) / 256), dim3(256), 0, st, A, n, c0, k0, k1);
+    };
+    auto gemm = [&](int c0, int kd, int rs, int re, int k0, int k1) {
+        if (re > rs && k1 > k0)
+            hipLaunchKernelGGL(lu_gemm_kernel, dim3((re - rs + 63) / 64, (k1 - k0 + 63) / 64), dim3(256), 0, st, A, n,
+                               c0, kd, rs, re, k0, k1);
+    };
+    // LU_GROUP panels per step, so that the update of the trailing matrix is
+    // of rank LU_GROUP LU_NB: each panel of the group is first brought up to
+    // date with the ones before it (their row swaps, its part of the block
+    // row, one update of its own columns) and factored; then the rest of the
+    // matrix takes the group's row swaps, its block row and one update
+    for (int c0 = 0; c0 < n && e == hipSuccess; c0 += LU_GROUP * LU_NB) {
+        const int cE = imn(n, c0 + LU_GROUP * LU_NB);
+        for (int cg = c0; cg < cE; cg += LU_NB) {
+            const int ce = imn(cE, cg + LU_NB);
+            if (cg > c0) {  // the panels before this one are full
+                swap(c0, cg - c0, cg, ce);
+                for (int ch = c0; ch < cg; ch += LU_NB) {
+                    trsm(ch, cg, ce);
+                    gemm(ch, LU_NB, ch + LU_NB, cg, cg, ce);
+                }
+                gemm(c0, cg - c0, cg, n, cg, ce);
+            }
+            panel(cg, ce - cg);
+            if (cg > c0) swap(cg, ce - cg, c0, cg);
+        }
+        swap(c0, cE - c0, 0, c0);
+        if (cE < n) {  // every panel of the group is full here
+            swap(c0, cE - c0, cE, n);
+            for (int ch = c0; ch < cE; ch += LU_NB) {
+                trsm(ch, cE, n);
+                gemm(ch, LU_NB, ch + LU_NB, cE, cE, n);
+            }
+            gemm(c0, cE - c0, cE, n, cE, n);
         }
         e = hipGetLastError();
     }
