@@ -24,6 +24,7 @@
 #define LU_NB 32
 #define LU_GROUP 4  // panels per trailing update
 #define LU_PANEL_THREADS 1024
+#define LU_PANEL_ONE_WG_ROWS 4096  // longer panels: pivot/pick/update launches per column
 #define LU_SOLVE_THREADS 1024
 
 namespace {
@@ -110,6 +111,7 @@ __global__ __launch_bounds__(LU_PANEL_THREADS) void lu_panel_kernel(zc *A, int n
                     v = sval[w];
                     ix = sidx[w];
                 }
+            if (ix < j) ix = j;  // no finite candidate: keep row j
             imax_s = ix;
             piv[j] = ix;
             if (ix != j) vv[ix] = vv[j];
@@ -199,6 +201,7 @@ __global__ __launch_bounds__(64) void lu_pick_kernel(zc *A, int n, int c0, int n
                 v = pval[b];
                 ix = pidx[b];
             }
+        if (ix < j) ix = j;  // no finite candidate: keep row j
         im_s = ix;
         piv[j] = ix;
         if (ix != j) vv[ix] = vv[j];
@@ -222,16 +225,53 @@ __global__ __launch_bounds__(64) void lu_pick_kernel(zc *A, int n, int c0, int n
     }
 }
 
-__global__ void lu_colupdate_kernel(zc *A, int n, int c0, int nb, int j, const zc *rcp)
+// When column j+1 is in the panel, the update also leaves the pivot
+// partials of column j+1 over its rows (one per workgroup, as the pivot
+// kernel would): the search of the next column needs no launch of its own.
+__global__ __launch_bounds__(256) void lu_colupdate_kernel(zc *A, int n, int c0, int nb, int j, const zc *rcp,
+                                                           const double *vv, double *pval, int *pidx)
 {
-    const int i = j + 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    zc *colj = A + (int64_t)j * n;
-    const zc l = cmul(colj[i], rcp[0]);
-    colj[i] = l;
-    for (int k = j + 1; k < c0 + nb; k++) {
-        zc *ck = A + (int64_t)k * n;
-        ck[i] = csub(ck[i], cmul(l, ck[j]));
+    __shared__ double sv[4];
+    __shared__ int si[4];
+    const int i = j + 1 + blockIdx.x * 256 + threadIdx.x;
+    const bool ahead = j + 1 < c0 + nb;
+    double bv = -1.0;
+    int bi = -1;
+    if (i < n) {
+        zc *colj = A + (int64_t)j * n;
+        const zc l = cmul(colj[i], rcp[0]);
+        colj[i] = l;
+        for (int k = j + 1; k < c0 + nb; k++) {
+            zc *ck = A + (int64_t)k * n;
+            ck[i] = csub(ck[i], cmul(l, ck[j]));
+        }
+        if (ahead) {
+            bv = vv[i] * zabs(A[(int64_t)(j + 1) * n + i]);
+            bi = i;
+        }
+    }
+    if (!ahead) return;
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_down(bv, off, 64);
+        const int oi = __shfl_down(bi, off, 64);
+        if (ov > bv || (ov == bv && oi > bi)) {
+            bv = ov;
+            bi = oi;
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sv[threadIdx.x >> 6] = bv;
+        si[threadIdx.x >> 6] = bi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; w++)
+            if (sv[w] > bv || (sv[w] == bv && si[w] > bi)) {
+                bv = sv[w];
+                bi = si[w];
+            }
+        pval[blockIdx.x] = bv;
+        pidx[blockIdx.x] = bi;
     }
 }
 
@@ -587,6 +627,7 @@ size_t lu_factor_scratch_bytes() { return LU_PIV_BLOCKS * (sizeof(double) + size
 
 hipError_t launch_lu_factor(double *LUm, int n, double *vv, int *piv, int *err, void *scratch, hipStream_t st)
 {
+    if (n < 1 || n > LU_PIV_BLOCKS * 256) return hipErrorInvalidValue;
     zc *A = reinterpret_cast<zc *>(LUm);
     double *pval = reinterpret_cast<double *>(scratch);
     zc *rcp = reinterpret_cast<zc *>(pval + LU_PIV_BLOCKS);
@@ -596,20 +637,23 @@ hipError_t launch_lu_factor(double *LUm, int n, double *vv, int *piv, int *err, 
     // one panel of nb columns at c0: pivot search, row swaps and elimination
     // inside the panel's own columns
     auto panel = [&](int c0, int nb) {
-        if (n - c0 <= 4096) {
+        if (n - c0 <= LU_PANEL_ONE_WG_ROWS) {
             // short panels: one workgroup does all its columns
             hipLaunchKernelGGL(lu_panel_kernel, dim3(1), dim3(LU_PANEL_THREADS), 0, st, A, n, c0, nb, vv, piv);
             return;
         }
         for (int jj = 0; jj < nb; jj++) {
             const int j = c0 + jj, rows = n - j;
-            const int nbk = imn(LU_PIV_BLOCKS, (rows + 255) / 256);
-            hipLaunchKernelGGL(lu_pivot_kernel, dim3(nbk), dim3(256), 0, st, A, n, j, vv, pval, pidx);
+            // partials of column j: its own search for the panel's first
+            // column, else those the update of column j-1 left (one per
+            // workgroup of that launch, (rows + 255) / 256 <= LU_PIV_BLOCKS)
+            const int nbk = jj == 0 ? imn(LU_PIV_BLOCKS, (rows + 255) / 256) : (rows + 255) / 256;
+            if (jj == 0) hipLaunchKernelGGL(lu_pivot_kernel, dim3(nbk), dim3(256), 0, st, A, n, j, vv, pval, pidx);
             hipLaunchKernelGGL(lu_pick_kernel, dim3(1), dim3(64), 0, st, A, n, c0, nb, j, vv, piv, pval, pidx, nbk,
                                rcp);
             if (rows > 1)
                 hipLaunchKernelGGL(lu_colupdate_kernel, dim3((rows - 1 + 255) / 256), dim3(256), 0, st, A, n, c0, nb,
-                                   j, rcp);
+                                   j, rcp, vv, pval, pidx);
         }
     };
     auto swap = [&](int p0, int np, int k0, int k1) {  // pivots [p0, p0+np) on columns [k0, k1)
