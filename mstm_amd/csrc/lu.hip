@@ -24,7 +24,7 @@
 #define LU_NB 32
 #define LU_GROUP 4  // panels per trailing update
 #define LU_PANEL_THREADS 1024
-#define LU_PANEL_ONE_WG_ROWS 4096  // longer panels: pivot/pick/update launches per column
+#define LU_PANEL_ONE_WG_ROWS 512  // longer panels: pivot/pick/update launches per column
 #define LU_SOLVE_THREADS 1024
 
 namespace {
@@ -192,36 +192,49 @@ __global__ __launch_bounds__(256) void lu_pivot_kernel(const zc *A, int n, int j
 __global__ __launch_bounds__(64) void lu_pick_kernel(zc *A, int n, int c0, int nb, int j, double *vv, int *piv,
                                                      const double *pval, const int *pidx, int nblk, zc *rcp)
 {
-    __shared__ int im_s;
-    if (threadIdx.x == 0) {
-        double v = pval[0];
-        int ix = pidx[0];
-        for (int b = 1; b < nblk; b++)
-            if (pval[b] > v || (pval[b] == v && pidx[b] > ix)) {
-                v = pval[b];
-                ix = pidx[b];
-            }
-        if (ix < j) ix = j;  // no finite candidate: keep row j
-        im_s = ix;
+    const int t = threadIdx.x;
+    double v = -1.0;
+    int ix = -1;  // no candidate
+    for (int b = t; b < nblk; b += 64) {
+        const double pv = pval[b];
+        const int pi = pidx[b];
+        if (pi >= 0 && (ix < 0 || pv > v || (pv == v && pi > ix))) {
+            v = pv;
+            ix = pi;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_down(v, off, 64);
+        const int oi = __shfl_down(ix, off, 64);
+        if (oi >= 0 && (ix < 0 || ov > v || (ov == v && oi > ix))) {
+            v = ov;
+            ix = oi;
+        }
+    }
+    ix = __shfl(ix, 0, 64);
+    if (ix < j) ix = j;  // no finite candidate: keep row j
+    if (t == 0) {
         piv[j] = ix;
         if (ix != j) vv[ix] = vv[j];
     }
-    __syncthreads();
-    const int im = im_s;
-    if (im != j)
-        for (int k = c0 + threadIdx.x; k < c0 + nb; k += 64) {
-            zc *ck = A + (int64_t)k * n;
-            const zc t = ck[im];
-            ck[im] = ck[j];
-            ck[j] = t;
+    // rows j and ix swap across the panel (nb <= 64 columns, one per lane);
+    // the lane of column j keeps the pivot and its reciprocal
+    const int k = c0 + t;
+    if (t < nb) {
+        zc *ck = A + (int64_t)k * n;
+        zc a = ck[ix];
+        if (ix != j) {
+            ck[ix] = ck[j];
+            ck[j] = a;
         }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        zc *colj = A + (int64_t)j * n;
-        if (zabs(colj[j]) == 0.0) colj[j] = cmk(1e-20, 0.0);
-        const zc p = colj[j];
-        const double den = p.re * p.re + p.im * p.im;
-        rcp[0] = cmk(p.re / den, -p.im / den);
+        if (k == j) {
+            if (zabs(a) == 0.0) {
+                a = cmk(1e-20, 0.0);
+                ck[j] = a;
+            }
+            const double den = a.re * a.re + a.im * a.im;
+            rcp[0] = cmk(a.re / den, -a.im / den);
+        }
     }
 }
 
@@ -233,20 +246,29 @@ __global__ __launch_bounds__(256) void lu_colupdate_kernel(zc *A, int n, int c0,
 {
     __shared__ double sv[4];
     __shared__ int si[4];
-    const int i = j + 1 + blockIdx.x * 256 + threadIdx.x;
-    const bool ahead = j + 1 < c0 + nb;
+    __shared__ zc urow[LU_NB];  // row j of the columns after j in the panel
+    const int i = j + 1 + blockIdx.x * 256 + threadIdx.x, nk = c0 + nb - j - 1;
+    const bool ahead = nk > 0;
+    if (threadIdx.x < nk) urow[threadIdx.x] = A[(int64_t)(j + 1 + threadIdx.x) * n + j];
+    __syncthreads();
     double bv = -1.0;
     int bi = -1;
     if (i < n) {
         zc *colj = A + (int64_t)j * n;
         const zc l = cmul(colj[i], rcp[0]);
         colj[i] = l;
-        for (int k = j + 1; k < c0 + nb; k++) {
-            zc *ck = A + (int64_t)k * n;
-            ck[i] = csub(ck[i], cmul(l, ck[j]));
-        }
+        zc x[LU_NB - 1];  // every load of the row before any store
+#pragma unroll
+        for (int t = 0; t < LU_NB - 1; t++)
+            if (t < nk) x[t] = A[(int64_t)(j + 1 + t) * n + i];
+#pragma unroll
+        for (int t = 0; t < LU_NB - 1; t++)
+            if (t < nk) {
+                x[t] = csub(x[t], cmul(l, urow[t]));
+                A[(int64_t)(j + 1 + t) * n + i] = x[t];
+            }
         if (ahead) {
-            bv = vv[i] * zabs(A[(int64_t)(j + 1) * n + i]);
+            bv = vv[i] * zabs(x[0]);
             bi = i;
         }
     }
