@@ -248,19 +248,22 @@ __global__ __launch_bounds__(256) void lu_colupdate_kernel(zc *A, int n, int c0,
     __shared__ int si[4];
     __shared__ zc urow[LU_NB];  // row j of the columns after j in the panel
     const int i = j + 1 + blockIdx.x * 256 + threadIdx.x, nk = c0 + nb - j - 1;
-    const bool ahead = nk > 0;
+    const bool ahead = nk > 0, mine = i < n;
+    // every load of the row goes out before the barrier and before any store
+    // (no thread here writes row j)
+    zc x[LU_NB - 1], l = cmk(0.0, 0.0);
+    if (mine) {
+#pragma unroll
+        for (int t = 0; t < LU_NB - 1; t++)
+            if (t < nk) x[t] = A[(int64_t)(j + 1 + t) * n + i];
+        l = cmul(A[(int64_t)j * n + i], rcp[0]);
+    }
     if (threadIdx.x < nk) urow[threadIdx.x] = A[(int64_t)(j + 1 + threadIdx.x) * n + j];
     __syncthreads();
     double bv = -1.0;
     int bi = -1;
-    if (i < n) {
-        zc *colj = A + (int64_t)j * n;
-        const zc l = cmul(colj[i], rcp[0]);
-        colj[i] = l;
-        zc x[LU_NB - 1];  // every load of the row before any store
-#pragma unroll
-        for (int t = 0; t < LU_NB - 1; t++)
-            if (t < nk) x[t] = A[(int64_t)(j + 1 + t) * n + i];
+    if (mine) {
+        A[(int64_t)j * n + i] = l;
 #pragma unroll
         for (int t = 0; t < LU_NB - 1; t++)
             if (t < nk) {
