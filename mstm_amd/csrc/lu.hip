@@ -370,14 +370,29 @@ __global__ __launch_bounds__(256) void lu_gemm_kernel(zc *A, int n, int c0, int 
             cr[a][b] = lu_d4{0.0, 0.0, 0.0, 0.0};
             ci[a][b] = lu_d4{0.0, 0.0, 0.0, 0.0};
         }
-    for (int kk = 0; kk < kd; kk += TK) {
-        for (int e = threadIdx.x; e < TK * 64; e += 256) {
-            const int q = e / 64, rr = e % 64;
+    // the next K chunk is fetched into registers while this one is used (the
+    // block updated never overlaps the L and U panels read)
+    constexpr int PF = TK * 64 / 256;
+    zc pl[PF], pu[PF];
+    auto fetch = [&](int kk) {
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const int e = threadIdx.x + 256 * u, q = e / 64, rr = e % 64;
             const int row = r0 + rr, col = k0 + rr;
-            Ls[q][rr] = (row < re && kk + q < kd) ? A[(int64_t)(c0 + kk + q) * n + row] : cmk(0.0, 0.0);
-            Us[q][rr] = (col < ke && kk + q < kd) ? A[(int64_t)col * n + c0 + kk + q] : cmk(0.0, 0.0);
+            pl[u] = (row < re && kk + q < kd) ? A[(int64_t)(c0 + kk + q) * n + row] : cmk(0.0, 0.0);
+            pu[u] = (col < ke && kk + q < kd) ? A[(int64_t)col * n + c0 + kk + q] : cmk(0.0, 0.0);
+        }
+    };
+    fetch(0);
+    for (int kk = 0; kk < kd; kk += TK) {
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const int e = threadIdx.x + 256 * u;
+            Ls[e / 64][e % 64] = pl[u];
+            Us[e / 64][e % 64] = pu[u];
         }
         __syncthreads();
+        if (kk + TK < kd) fetch(kk + TK);
 #pragma unroll
         for (int q = 0; q < TK; q += 4) {
             double ur[2], ui[2], lr[2], lm[2], lim[2];
