@@ -477,47 +477,70 @@ __global__ __launch_bounds__(LU_SOLVE_THREADS) void lu_solve_kernel(const zc *LU
     }
 }
 
-// Blocked substitution on 32-row blocks.  lower: unit L forward, block
-// rows [r0, r0+nb); else U backward.  The block's triangle in one workgroup
-// (thread = right-hand side x row chunk), then the other rows updated by many
-// workgroups.
-__global__ __launch_bounds__(256) void lu_tri_block_kernel(const zc *LU, int n, int r0, int nb, double *b, int64_t ld,
+// Blocked substitution on LU_NB-row blocks: lower = unit L forward over the
+// block rows [r0, r0+nb), else U backward.
+//
+// One diagonal block (nb <= LU_NB rows) of the substitution: the triangle in
+// LDS, one wave per two right-hand sides (lane = 32 * (rhs & 1) + row), the
+// solved values passed between lanes by shuffles, so no barrier per column.
+__global__ __launch_bounds__(128) void lu_tri_block_kernel(const zc *LU, int n, int r0, int nb, double *b, int64_t ld,
                                                            int nrhs, int lower)
 {
-    const int tid = threadIdx.x;
-    for (int r = 0; r < nrhs; r++) {
-        zc *x = reinterpret_cast<zc *>(b + 2 * ld * r);
-        if (lower) {
-            for (int j = r0; j < r0 + nb; j++) {
-                const zc xj = x[j];
-                const zc *cj = LU + (int64_t)j * n;
-                for (int i = j + 1 + tid; i < r0 + nb; i += 256) x[i] = csub(x[i], cmul(cj[i], xj));
-                __syncthreads();
-            }
-        } else {
-            for (int j = r0 + nb - 1; j >= r0; j--) {
-                const zc *cj = LU + (int64_t)j * n;
-                const zc xj = cdiv(x[j], cj[j]);
-                for (int i = r0 + tid; i < j; i += 256) x[i] = csub(x[i], cmul(cj[i], xj));
-                __syncthreads();
-                if (tid == 0) x[j] = xj;
-                __syncthreads();
-            }
+    __shared__ zc T[LU_NB][LU_NB + 1];
+    for (int e = threadIdx.x; e < LU_NB * LU_NB; e += 128) {
+        const int ti = e % LU_NB, tj = e / LU_NB;
+        T[ti][tj] = (ti < nb && tj < nb) ? LU[(int64_t)(r0 + tj) * n + r0 + ti] : cmk(0.0, 0.0);
+    }
+    const int lane = threadIdx.x & 63, i = lane & 31, base = lane & 32, r = (threadIdx.x >> 6) * 2 + (lane >> 5);
+    const bool act = r < nrhs && i < nb;
+    zc *x = reinterpret_cast<zc *>(b + 2 * ld * (r < nrhs ? r : 0));
+    zc xi = act ? x[r0 + i] : cmk(0.0, 0.0);
+    __syncthreads();
+    if (lower) {  // unit lower
+        for (int j = 0; j < nb; j++) {
+            const zc xj = cmk(__shfl(xi.re, base + j, 64), __shfl(xi.im, base + j, 64));
+            if (i > j) xi = csub(xi, cmul(T[i][j], xj));
+        }
+    } else {
+        for (int j = nb - 1; j >= 0; j--) {
+            zc xj = cmk(__shfl(xi.re, base + j, 64), __shfl(xi.im, base + j, 64));
+            xj = cdiv(xj, T[j][j]);
+            if (i < j)
+                xi = csub(xi, cmul(T[i][j], xj));
+            else if (i == j)
+                xi = xj;
         }
     }
+    if (act) x[r0 + i] = xi;
 }
 
-__global__ void lu_tri_update_kernel(const zc *LU, int n, int r0, int nb, double *b, int64_t ld, int nrhs, int lower)
+// The other rows take the block's solved values: each LU element is loaded
+// once for every right-hand side (nrhs <= 4), the block's values sit in LDS.
+__global__ __launch_bounds__(256) void lu_tri_update_kernel(const zc *LU, int n, int r0, int nb, double *b, int64_t ld,
+                                                            int nrhs, int lower)
 {
+    __shared__ zc xs[4][LU_NB];
+    for (int e = threadIdx.x; e < 4 * LU_NB; e += 256) {
+        const int r = e / LU_NB, j = e % LU_NB;
+        xs[r][j] = (r < nrhs && j < nb) ? reinterpret_cast<const zc *>(b + 2 * ld * r)[r0 + j] : cmk(0.0, 0.0);
+    }
+    __syncthreads();
     const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = lower ? r0 + nb + i0 : i0;
     if (lower ? i >= n : i >= r0) return;
-    for (int r = 0; r < nrhs; r++) {
-        zc *x = reinterpret_cast<zc *>(b + 2 * ld * r);
-        zc s = x[i];
-        for (int j = r0; j < r0 + nb; j++) s = csub(s, cmul(LU[(int64_t)j * n + i], x[j]));
-        x[i] = s;
+    zc s[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) s[r] = r < nrhs ? reinterpret_cast<const zc *>(b + 2 * ld * r)[i] : cmk(0.0, 0.0);
+#pragma unroll 8
+    for (int j = 0; j < nb; j++) {
+        const zc lij = LU[(int64_t)(r0 + j) * n + i];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            if (r < nrhs) s[r] = csub(s[r], cmul(lij, xs[r][j]));
     }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+        if (r < nrhs) reinterpret_cast<zc *>(b + 2 * ld * r)[i] = s[r];
 }
 
 __global__ void lu_permute_kernel(const int *piv, int n, double *b, int64_t ld, int nrhs)
@@ -754,7 +777,7 @@ hipError_t launch_lu_solve(const double *LUm, int n, const int *piv, double *b, 
     hipLaunchKernelGGL(lu_permute_kernel, dim3(1), dim3(64), 0, st, piv, n, b, ld, nrhs);
     for (int r0 = 0; r0 < n; r0 += LU_NB) {
         const int nb = n - r0 < LU_NB ? n - r0 : LU_NB, rest = n - r0 - nb;
-        hipLaunchKernelGGL(lu_tri_block_kernel, dim3(1), dim3(256), 0, st, LU, n, r0, nb, b, ld, nrhs, 1);
+        hipLaunchKernelGGL(lu_tri_block_kernel, dim3(1), dim3(128), 0, st, LU, n, r0, nb, b, ld, nrhs, 1);
         if (rest > 0)
             hipLaunchKernelGGL(lu_tri_update_kernel, dim3((rest + 255) / 256), dim3(256), 0, st, LU, n, r0, nb, b, ld,
                                nrhs, 1);
@@ -762,7 +785,7 @@ hipError_t launch_lu_solve(const double *LUm, int n, const int *piv, double *b, 
     const int nblk = (n + LU_NB - 1) / LU_NB;
     for (int bk = nblk - 1; bk >= 0; bk--) {
         const int r0 = bk * LU_NB, nb = n - r0 < LU_NB ? n - r0 : LU_NB;
-        hipLaunchKernelGGL(lu_tri_block_kernel, dim3(1), dim3(256), 0, st, LU, n, r0, nb, b, ld, nrhs, 0);
+        hipLaunchKernelGGL(lu_tri_block_kernel, dim3(1), dim3(128), 0, st, LU, n, r0, nb, b, ld, nrhs, 0);
         if (r0 > 0)
             hipLaunchKernelGGL(lu_tri_update_kernel, dim3((r0 + 255) / 256), dim3(256), 0, st, LU, n, r0, nb, b, ld,
                                nrhs, 0);
