@@ -531,13 +531,16 @@ __global__ __launch_bounds__(256) void lu_tri_update_kernel(const zc *LU, int n,
     zc s[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) s[r] = r < nrhs ? reinterpret_cast<const zc *>(b + 2 * ld * r)[i] : cmk(0.0, 0.0);
-#pragma unroll 8
-    for (int j = 0; j < nb; j++) {
-        const zc lij = LU[(int64_t)(r0 + j) * n + i];
+    zc lij[LU_NB];  // the row's loads all in flight at once
 #pragma unroll
-        for (int r = 0; r < 4; r++)
-            if (r < nrhs) s[r] = csub(s[r], cmul(lij, xs[r][j]));
-    }
+    for (int j = 0; j < LU_NB; j++)
+        if (j < nb) lij[j] = LU[(int64_t)(r0 + j) * n + i];
+#pragma unroll
+    for (int j = 0; j < LU_NB; j++)
+        if (j < nb)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                if (r < nrhs) s[r] = csub(s[r], cmul(lij[j], xs[r][j]));
 #pragma unroll
     for (int r = 0; r < 4; r++)
         if (r < nrhs) reinterpret_cast<zc *>(b + 2 * ld * r)[i] = s[r];
