@@ -54,12 +54,20 @@ __global__ void lu_store_kernel(zc *A, int n, int j0, int ncol, const double *th
 }
 
 // vv(i) = 1 / max_k |a(i,k)|; a zero row sets *err
-__global__ void lu_rowscale_kernel(const zc *A, int n, double *vv, int *err)
+// 64 rows per workgroup; the four waves take every fourth column each
+__global__ __launch_bounds__(256) void lu_rowscale_kernel(const zc *A, int n, double *vv, int *err)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    __shared__ double sm[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i = blockIdx.x * 64 + lane;
     double m = 0.0;
-    for (int k = 0; k < n; k++) m = fmax(m, zabs(A[(int64_t)k * n + i]));
+    if (i < n) {
+#pragma unroll 8
+        for (int k = w; k < n; k += 4) m = fmax(m, zabs(A[(int64_t)k * n + i]));
+    }
+    sm[w][lane] = m;
+    __syncthreads();
+    if (w != 0 || i >= n) return;
+    m = fmax(fmax(sm[0][lane], sm[1][lane]), fmax(sm[2][lane], sm[3][lane]));
     if (m == 0.0) {
         err[0] = 1;  // same value from every zero row
         vv[i] = 0.0;
@@ -698,7 +706,7 @@ hipError_t launch_lu_factor(double *LUm, int n, double *vv, int *piv, int *err, 
     double *pval = reinterpret_cast<double *>(scratch);
     zc *rcp = reinterpret_cast<zc *>(pval + LU_PIV_BLOCKS);
     int *pidx = reinterpret_cast<int *>(pval + LU_PIV_BLOCKS + 2);
-    hipLaunchKernelGGL(lu_rowscale_kernel, dim3((n + 255) / 256), dim3(256), 0, st, A, n, vv, err);
+    hipLaunchKernelGGL(lu_rowscale_kernel, dim3((n + 63) / 64), dim3(256), 0, st, A, n, vv, err);
     hipError_t e = hipGetLastError();
     // one panel of nb columns at c0: pivot search, row swaps and elimination
     // inside the panel's own columns
