@@ -3,7 +3,9 @@
 // Replaces the direct solution branch of MSTM:
 //   general_interaction_matrix   mstm-modules-33.f90:5854-5985
 //       A = I - T H, column by column (multmiecoeffmult of each column of H);
-//       here every column is the FP64 operator applied to a unit vector;
+//       here the blocks come straight from the pair tables when all spheres
+//       have one order, else every column is the FP64 operator applied to a
+//       unit vector;
 //   lu_decomposition             mstm-solver-8.f90:951-1020
 //       Crout LU with implicit row scaling: the pivot of column j is the row
 //       maximizing vv(i) |a(i,j)| (vv(i) = 1 / max_k |a(i,k)| of the input
@@ -152,11 +154,12 @@ __global__ __launch_bounds__(LU_PANEL_THREADS) void lu_panel_kernel(zc *A, int n
     }
 }
 
-// The same panel factorization spread over the chip, three launches per
-// column j: (1) block partial maxima of vv(i)|a(i,j)| over rows >= j, (2) one
-// workgroup picks the pivot (ties to the larger row), swaps rows j and imax
-// across the panel and keeps 1/a(j,j), (3) scaling of column j and the
-// rank-1 update of the panel's remaining columns, one thread per row.
+// The same panel factorization spread over the chip, per column j: (1) block
+// partial maxima of vv(i)|a(i,j)| over rows >= j (a launch of its own only
+// for a panel's first column: the update of column j-1 leaves them for the
+// others), (2) one wave picks the pivot (ties to the larger row), swaps rows
+// j and imax across the panel and keeps 1/a(j,j), (3) scaling of column j
+// and the rank-1 update of the panel's remaining columns, one thread per row.
 #define LU_PIV_BLOCKS 256
 __global__ __launch_bounds__(256) void lu_pivot_kernel(const zc *A, int n, int j, const double *vv, double *pval,
                                                        int *pidx)
@@ -696,7 +699,7 @@ hipError_t launch_lu_store(double *A, int n, int j0, int ncol, const double *th,
     return hipGetLastError();
 }
 
-// scratch: pval[LU_PIV_BLOCKS] doubles, pidx[LU_PIV_BLOCKS] ints, rcp one complex
+// scratch: pval[LU_PIV_BLOCKS] doubles, rcp one complex, pidx[LU_PIV_BLOCKS] ints
 size_t lu_factor_scratch_bytes() { return LU_PIV_BLOCKS * (sizeof(double) + sizeof(int)) + 2 * sizeof(double); }
 
 hipError_t launch_lu_factor(double *LUm, int n, double *vv, int *piv, int *err, void *scratch, hipStream_t st)
