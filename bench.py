@@ -360,7 +360,8 @@ def bench_lu(args):
           {"number_eqns": ne, "build_ms": st["lu_build_ms"], "factor_ms": st["lu_factor_ms"],
            "solve_ms": st["lu_solve_ms"], "plain_fp64_bicg_s": round(t_plain, 4),
            "lu_vs_bicg_1e-6_rel": d,
-           "roofline": {"kernel": "lu_factor (panel + swaps + TRSM + complex FP64 GEMM)", "bound": "fp64 vector",
+           "roofline": {"kernel": "lu_factor (panels + swaps + TRSM + complex FP64 GEMM on the matrix cores)",
+                        "bound": "fp64 (matrix cores and vector: both 78.6 TF/s)",
                         "achieved": round(tf, 2), "peak": 78.6, "unit": "TFLOP/s", "frac": round(tf / 78.6, 4),
                         "traffic": None, "flops": fl},
            "cpu_baseline": None})
