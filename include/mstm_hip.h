@@ -75,12 +75,28 @@ int mstm_hip_get_mie(const mstm_hip_ctx *ctx, double *an_mie);
  * MSTM_HIP_FP64 or MSTM_HIP_FP32; both may coexist. */
 int mstm_hip_setup_direct(mstm_hip_ctx *ctx, int precision);
 
-/* Node grid of the FFT operator: node_selection + node_order rule
- * (fft_translation-5.f90:654-825, mstm-input-37.f90:1238-1262).  fva <= 0
- * selects the volume fraction automatically; node_order <= 0 gives
- * -node_order + ceiling(d_cell).  Outputs cell_dim[3], d_cell, node order. */
-int mstm_hip_node_selection(mstm_hip_ctx *ctx, double fva, int node_order, int neighbor_node_model, int *cell_dim,
-                            double *d_cell, int *node_order_out);
+/* Node grid of the FFT operator: node_selection(fva, target_min, target_max,
+ * d_specified, local_host) (fft_translation-5.f90:654-825) followed by the
+ * node_order rule of main_calling_program (mstm-input-37.f90:1238-1262).
+ *   fva          in/out: <= 0 selects the volume fraction automatically
+ *                (clamped to [0.02, 1]); set to the fraction used unless
+ *                d_specified
+ *   target_min/_max  the target box (3 each); NULL: the bounds of the sphere
+ *                centres (sphere_min/max_position)
+ *   d_specified, d_cell  d_specified != 0 takes *d_cell as the cell size;
+ *                *d_cell returns the final cell size (d_cell of the module)
+ *   local_host   must be 0 (spheres inside a host sphere are not on this path)
+ *   node_order   <= 0 gives -node_order + ceiling(d_cell)
+ *   neighbor_node_model  0..3 (stencil of 1, 7, 19 or 27 cells)
+ * Outputs cell_dim[3] and the node order.  Replaces any earlier grid. */
+int mstm_hip_node_selection(mstm_hip_ctx *ctx, double *fva, const double *target_min, const double *target_max,
+                            int d_specified, double *d_cell, int local_host, int node_order, int neighbor_node_model,
+                            int *cell_dim, int *node_order_out);
+/* The private grid state node_selection keeps (fft_translation-5.f90:22-27):
+ * cell_boundary[3], sphere_node[3 nsphere] (1-based cell indices per sphere)
+ * and number_neighbor_nodes; each output may be NULL.  A Fortran host can
+ * compare it with its own grid. */
+int mstm_hip_fft_grid(const mstm_hip_ctx *ctx, double *cell_boundary, int *sphere_node, int *number_neighbor_nodes);
 /* Build the FFT operator (fft_translation_initialization, :901-974, plus the
  * sphere<->node and near-field tables) at the given precision. */
 int mstm_hip_setup_fft(mstm_hip_ctx *ctx, int precision);
@@ -89,25 +105,41 @@ int mstm_hip_setup_fft(mstm_hip_ctx *ctx, int precision);
  * for external spheres.  ain/aout: (number_eqns, nrhs) complex, host memory.
  * con_tran[r] != 0 selects conj(H^T T^T conj(.)), mie_mult[r] != 0 applies
  * the T-matrix.  use_fft selects fft_external_to_external_expansion
- * (fft_translation-5.f90:72-336) instead of the direct pair sum. */
+ * (fft_translation-5.f90:72-336) instead of the direct pair sum.
+ * rhs_list (NULL = all) and skip_external_translation as the reference's
+ * optional arguments: a column left out of the list, or every column when
+ * skip_external_translation != 0, gets no translation, i.e. a zero column
+ * (external spheres have no host-sphere or plane-boundary part). */
 int mstm_hip_sphereinteraction(mstm_hip_ctx *ctx, int nrhs, const double *ain, double *aout, const int *con_tran,
-                               const int *mie_mult, int use_fft, int precision);
+                               const int *mie_mult, int use_fft, int precision, const int *rhs_list,
+                               int skip_external_translation);
 
 /* cbicg (mstm-solver-8.f90:654-949): solve (I - T H) anp = pnp with the
  * reference's complex biconjugate gradient, FP64, starting from anp.
  * Stops when |r|^2/|b|^2 < eps.  iter: iterations done (niter+1 when not
- * converged, as the Fortran do-variable), errmax: last |r|^2/|b|^2. */
+ * converged, as the Fortran do-variable), errmax: last |r|^2/|b|^2.
+ * niter < 0: the order-of-scattering series (:785-825), anp += c_k with
+ * c_1 = T H anp, c_{k+1} = T H c_k, until |c_k|^2/|pnp|^2 < eps (errmax =
+ * that ratio, iter = k) or -niter terms (iter = -niter + 1, errmax = 0).
+ * niter = 0 (:768-781): anp becomes the host-sphere part of the product,
+ * zero for external spheres.  pnp = 0 returns with anp unchanged. */
 int mstm_hip_cbicg(mstm_hip_ctx *ctx, int niter, double eps, const double *pnp, double *anp, int use_fft, int *iter,
                    double *errmax);
 
-/* fixedorsoln (mstm-solver-8.f90:350-569), iterative branch: plane-wave RHS
- * for incidence (alpha, sin beta, dir=1), both polarizations, solve, and the
- * efficiencies of qefficiencyfactors (mstm-scatprops-26.f90:782-870).
- * amnp(number_eqns,2) complex, qeff(3,3,nsphere).  As in the reference,
- * maxerr/maxiter/istat report the second polarization's solve.
- * solver: MSTM_HIP_SOLVE_*. */
+/* fixedorsoln (mstm-solver-8.f90:350-569): plane-wave RHS for incidence
+ * (alpha, sin beta, dir = 1 from below / 2 from above), both polarizations,
+ * solve, and the efficiencies of qefficiencyfactors
+ * (mstm-scatprops-26.f90:782-870).  amnp(number_eqns,2) complex,
+ * qeff(3,3,nsphere) (may be NULL).  As in the reference, maxerr/maxiter/istat
+ * report the second polarization's solve.  solver: MSTM_HIP_SOLVE_*.
+ * niter = 0: no solve, amnp = T f (:513-522); niter < 0: cbicg's
+ * order-of-scattering series from amnp = T f for every solver but
+ * MSTM_HIP_SOLVE_LU (istat is then 1, as in the reference).
+ * excited_spheres (NULL = all): nsphere flags; a sphere with flag 0 gets no
+ * incident field (sphereplanewavecoef, mstm-scatprops-26.f90:217-284). */
 int mstm_hip_fixedorsoln(mstm_hip_ctx *ctx, double alpha, double sinc, int dir, double eps, int niter, int solver,
-                         int use_fft, double *amnp, double *qeff, double *maxerr, int *maxiter, int *istat);
+                         int use_fft, double *amnp, double *qeff, double *maxerr, int *maxiter, int *istat,
+                         const int *excited_spheres);
 
 /* Inner solver of the mixed refinement (MSTM_HIP_SOLVE_MIXED / _LEARNED; no
  * counterpart in MSTM): 0 = BiCG (the reference's recurrences in FP32),

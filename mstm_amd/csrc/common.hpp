@@ -11,6 +11,10 @@
 #include <stdint.h>
 
 #define MSTM_MAX_L 10  // largest sphere / node order with a compiled tile kernel
+// largest order of a generated translation table (build_slot's per-thread
+// arrays are sized by it): sphere orders of the pair tables and the node order
+// of the FFT path's sphere<->node tables
+#define MSTM_TAB_L 12
 #define MSTM_SLOTS 16  // pair slots per tile step (and spheres per block)
 
 #define MSTM_HD __host__ __device__ __forceinline__

@@ -87,6 +87,8 @@ struct mstm_hip_ctx {
     double *d_pos = nullptr, *d_radius = nullptr, *d_mre = nullptr, *d_mim = nullptr;
     int *d_order = nullptr, *d_offset = nullptr, *d_mieoff = nullptr;
     double *d_an = nullptr;  // complex, (npad*Lmax*4), sphere-uniform stride 4*Lmax
+    int *d_excited = nullptr;  // excited_spheres mask of the last fixedorsoln (nsph)
+    double *d_qeff = nullptr;  // per-sphere efficiencies of fixedorsoln (9 nsph)
     int2 *d_tiles = nullptr;
     int ntiles = 0, nblocks = 0;
     double *d_vcc = nullptr;
@@ -289,6 +291,8 @@ static void release_spheres(mstm_hip_ctx *c)
     dfree(c->d_offset);
     dfree(c->d_mieoff);
     dfree(c->d_an);
+    dfree(c->d_excited);
+    dfree(c->d_qeff);
     dfree(c->d_tiles);
     dfree(c->d_kv);
     dfree(c->d_fg);
@@ -420,6 +424,8 @@ int mstm_hip_set_spheres(mstm_hip_ctx *c, int n, const double *pos, const double
     HIPCHK(dalloc(&c->d_mre, sizeof(double) * n));
     HIPCHK(dalloc(&c->d_mim, sizeof(double) * n));
     HIPCHK(dalloc(&c->d_order, sizeof(int) * n));
+    HIPCHK(dalloc(&c->d_excited, sizeof(int) * n));
+    HIPCHK(dalloc(&c->d_qeff, sizeof(double) * 9 * n));
     HIPCHK(hipMemcpy(c->d_pos, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_radius, radius, sizeof(double) * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_mre, mre.data(), sizeof(double) * n, hipMemcpyHostToDevice));
@@ -443,12 +449,16 @@ int mstm_hip_set_mie(mstm_hip_ctx *c, const int *order, const double *an_mie)
 {
     if (!c || !c->nsph || !order || !an_mie) return fail(c, MSTM_HIP_ESTATE, "set_mie: no spheres");
     int n = c->nsph;
+    // everything built from the old orders or Mie matrices goes: operator
+    // tables, the FFT grid, the LU factors and the held solution
+    release_lu(c);
     release_direct(c, 0);
     release_direct(c, 1);
     if (c->fft) {
         fft_destroy(c->fft);
         c->fft = nullptr;
     }
+    c->obs.x_valid = false;
     c->order.assign(order, order + n);
     int rc = finish_sphere_layout(c);
     if (rc) return rc;
@@ -521,11 +531,17 @@ int mstm_hip_setup_direct(mstm_hip_ctx *c, int prec)
     return 0;
 }
 
-int mstm_hip_node_selection(mstm_hip_ctx *c, double fva, int node_order, int neighbor_node_model, int *cell_dim,
-                            double *d_cell, int *node_order_out)
+int mstm_hip_node_selection(mstm_hip_ctx *c, double *fva, const double *target_min, const double *target_max,
+                            int d_specified, double *d_cell, int local_host, int node_order, int neighbor_node_model,
+                            int *cell_dim, int *node_order_out)
 {
     if (!c || !c->nsph) return fail(c, MSTM_HIP_ESTATE, "node_selection: no spheres");
-    if (!cell_dim || !d_cell || !node_order_out) return fail(c, MSTM_HIP_EARG, "node_selection: null output");
+    if (!fva || !cell_dim || !d_cell || !node_order_out) return fail(c, MSTM_HIP_EARG, "node_selection: null argument");
+    if (local_host != 0)
+        return fail(c, MSTM_HIP_EUNSUP, "node_selection: local_host %d (spheres inside a host sphere) is not on the "
+                                        "GPU path", local_host);
+    if (neighbor_node_model < 0 || neighbor_node_model > 3)
+        return fail(c, MSTM_HIP_EARG, "node_selection: neighbor_node_model %d", neighbor_node_model);
     HIPCHK(hipSetDevice(c->device));
     if (c->fft) {
         fft_destroy(c->fft);
@@ -535,8 +551,17 @@ int mstm_hip_node_selection(mstm_hip_ctx *c, double fva, int node_order, int nei
                   c->offset.data(), c->rimed,   c->d_pos,   c->d_order,    c->d_offset,      c->d_an,
                   c->d_vcc,         c->vcc_LM};
     int err = 0;
-    c->fft = fft_create(g, fva, node_order, neighbor_node_model, cell_dim, d_cell, node_order_out, &err);
-    if (!c->fft) return fail(c, err ? err : MSTM_HIP_EHIP, "node_selection failed");
+    FftOp *f = fft_create(g, fva, target_min, target_max, d_specified, node_order, neighbor_node_model, cell_dim,
+                          d_cell, node_order_out, &err);
+    if (!f) return fail(c, err ? err : MSTM_HIP_EHIP, "node_selection failed");
+    c->fft = f;
+    return 0;
+}
+
+int mstm_hip_fft_grid(const mstm_hip_ctx *c, double *cell_boundary, int *sphere_node, int *number_neighbor_nodes)
+{
+    if (!c || !c->fft) return MSTM_HIP_ESTATE;
+    fft_grid(c->fft, cell_boundary, sphere_node, number_neighbor_nodes, nullptr);
     return 0;
 }
 
@@ -798,6 +823,69 @@ static int bicg2_dev(mstm_hip_ctx *c, int ns, int niter, const double *eps, cons
     return 0;
 }
 
+// Order-of-scattering solution, cbicg with niter < 0 (mstm-solver-8.f90:
+// 785-825), for ns = 1 or 2 systems at once: starting from x_s, the series
+//   c_1 = T H x_s,  c_{k+1} = T H c_k,  x_s += c_k
+// stops when |c_k|^2/|b_s|^2 < eps[s] (err[s] = that ratio, it[s] = k) or
+// after nord terms (it[s] = nord + 1 and err[s] = 0, as the Fortran loop
+// leaves them).  Each term of the active systems is one FP64 product.
+static int oos2_dev(mstm_hip_ctx *c, int ns, int nord, const double *eps, const double *b, double *x, int use_fft,
+                    int *it, double *err)
+{
+    const int n = c->neqns;
+    const size_t V = 2 * c->kv_n;
+    double *buf[2] = {c->d_kv + V * KV_PW, c->d_kv + V * KV_CAPS};  // two blocks of ns terms, alternating
+    double enorm[2] = {0, 0};
+    bool act[2] = {false, false};
+    for (int s = 0; s < ns; s++) {
+        it[s] = 0;
+        err[s] = 0.0;
+        HIPCHK(launch_zdot(b + V * s, b + V * s, n, 1, pregion(c, s), nullptr, c->st));
+    }
+    int rc = fetch_partials(c, 2 * krylov_grid() * ns);
+    if (rc) return rc;
+    for (int s = 0; s < ns; s++) {
+        double v[2];
+        reduce_region(c, pregion_off(s), 1, v);
+        enorm[s] = v[0];
+        act[s] = v[0] != 0.0;  // cbicg returns at once for pnp = 0
+    }
+    HIPCHK(hipMemcpyAsync(buf[0], x, sizeof(double) * V * ns, hipMemcpyDeviceToDevice, c->st));
+    int f2[2] = {2, 2};
+    int cur = 0, k;
+    for (k = 1; k <= nord; k++) {
+        if (!act[0] && !(ns == 2 && act[1])) break;
+        double *in = buf[cur], *out = buf[cur ^ 1];
+        if (ns == 2 && act[0] && act[1]) rc = matvec(c, 2, in, out, f2, MSTM_HIP_FP64, use_fft);
+        else
+            for (int s = 0; s < ns && !rc; s++)
+                if (act[s]) rc = matvec(c, 1, in + V * s, out + V * s, f2, MSTM_HIP_FP64, use_fft);
+        if (rc) return rc;
+        for (int s = 0; s < ns; s++) {
+            if (!act[s]) continue;
+            HIPCHK(launch_zaxpby(1.0, 0.0, out + V * s, 1.0, 0.0, x + V * s, n, c->st));
+            HIPCHK(launch_zdot(out + V * s, out + V * s, n, 1, pregion(c, s), nullptr, c->st));
+        }
+        rc = fetch_partials(c, 2 * krylov_grid() * ns);
+        if (rc) return rc;
+        for (int s = 0; s < ns; s++) {
+            if (!act[s]) continue;
+            double v[2];
+            reduce_region(c, pregion_off(s), 1, v);
+            c->stats.iters_fp64++;
+            if (v[0] / enorm[s] < eps[s]) {
+                err[s] = v[0] / enorm[s];
+                it[s] = k;
+                act[s] = false;
+            }
+        }
+        cur ^= 1;
+    }
+    for (int s = 0; s < ns; s++)
+        if (act[s]) it[s] = k;  // ran out: nord + 1
+    return 0;
+}
+
 // GMRES(m) (Saad & Schultz, SIAM J. Sci. Stat. Comput. 7, 856 (1986)) for
 // ns = 1 or 2 systems A d_s = b_s at once, from d_s = 0.  Arnoldi vectors
 // are FP64 and orthogonalised by classical Gram-Schmidt applied twice (one
@@ -988,10 +1076,13 @@ static int gmres2_dev(mstm_hip_ctx *c, int ns, int niter, const double *eps, con
 // Mixed precision for ns = 1 or 2 systems: x_s <- x_s + d_s with A d_s = r_s
 // solved by FP32 BiCG, r_s = b_s - A x_s evaluated with the FP64 operator;
 // a system stops on the reference criterion |r|^2/|b|^2 < eps measured in
-// FP64.  r_in (optional): the residual block of the starting x, already
-// evaluated in FP64.  iters[s] counts inner FP32 iterations.
+// FP64.  r_in (optional): the residual block of the starting x.  When
+// r_in_exact is false it came from an FP32 product: it only steers the first
+// inner solve, and a system it shows below eps gets no inner solve but one
+// more pass whose FP64 residual decides.  iters[s] counts inner FP32
+// iterations.
 static int refine2_dev(mstm_hip_ctx *c, int ns, int niter, double eps, const double *b, double *x, int use_fft,
-                       int *iters, double *errmax, const double *r_in = nullptr)
+                       int *iters, double *errmax, const double *r_in = nullptr, bool r_in_exact = true)
 {
     const int n = c->neqns;
     const size_t V = 2 * c->kv_n;
@@ -1010,8 +1101,12 @@ static int refine2_dev(mstm_hip_ctx *c, int ns, int niter, double eps, const dou
         errmax[s] = 0.0;
     }
     int f2[2] = {2, 2};
-    const int max_ref = c->max_ref;
-    for (int ref = 0; ref <= max_ref; ref++) {
+    // a system closes on an FP64 residual only: below eps, after max_ref
+    // inner solves, or with its iteration budget spent.  An inexact starting
+    // residual never closes one (pass 0 then only decides on an inner solve).
+    const bool inexact = r_in && !r_in_exact;
+    int nsolve[2] = {0, 0};
+    for (int ref = 0;; ref++) {
         if (!open[0] && !(ns == 2 && open[1])) break;
         if (ref == 0 && r_in) {
             for (int s = 0; s < ns; s++)
@@ -1042,10 +1137,13 @@ static int refine2_dev(mstm_hip_ctx *c, int ns, int niter, double eps, const dou
             reduce_region(c, pregion_off(s), 1, v);
             rr[s] = v[0];
             errmax[s] = rr[s] / bb[s];
-            if (errmax[s] < eps || ref == max_ref || iters[s] > niter) {
+            const bool done = errmax[s] < eps || nsolve[s] >= c->max_ref || iters[s] >= niter;
+            if (done && ref == 0 && inexact) continue;  // no inner solve; the FP64 residual of pass 1 decides
+            if (done) {
                 open[s] = false;
                 continue;
             }
+            nsolve[s]++;
             // inner FP32 solve of A d = r from d = r, to half the outer
             // target: the FP32 operator's relative error (~2e-7) is far below
             // the 1e-3 relative residual asked for, so one refinement step
@@ -1098,8 +1196,9 @@ static int lig_predict(mstm_hip_ctx *c)
 // Starting vectors of the learned solver for both polarizations: the
 // network's prediction xl_p is checked against the zero start (residual b,
 // |r|^2/|b|^2 = 1) with one two-column FP32 product; the better one goes to
-// x_p and its residual to r_p (the refinement re-evaluates every residual in
-// FP64 before it accepts an answer).  used[p]: prediction kept.
+// x_p and its residual to r_p.  That residual is an FP32 product: the
+// refinement uses it to steer its first inner solve and accepts an answer
+// only on an FP64 residual.  used[p]: prediction kept.
 static int lig_start2(mstm_hip_ctx *c, const double *b, double *x, double *r, int use_fft, bool *used)
 {
     const int n = c->neqns;
@@ -1427,21 +1526,38 @@ int mstm_hip_direct_solver(mstm_hip_ctx *c, const double *pnp, double *anp, int 
 }
 
 int mstm_hip_sphereinteraction(mstm_hip_ctx *c, int nrhs, const double *ain, double *aout, const int *con_tran,
-                               const int *mie_mult, int use_fft, int prec)
+                               const int *mie_mult, int use_fft, int prec, const int *rhs_list,
+                               int skip_external_translation)
 {
     if (!c || !c->nsph) return fail(c, MSTM_HIP_ESTATE, "sphereinteraction: no spheres");
     if (nrhs < 1 || !ain || !aout || !con_tran || !mie_mult) return fail(c, MSTM_HIP_EARG, "sphereinteraction: args");
     HIPCHK(hipSetDevice(c->device));
     const size_t V = 2 * (size_t)c->neqns;
+    // A column left out of rhs_list gets no translation (mstm-modules-33.f90:
+    // 6507; fft_translation-5.f90:457, 609, 619), and skip_external_translation
+    // drops the external translation of every column (mstm-scatprops-26.f90:
+    // 101): with no host spheres or plane boundaries those columns are zero
+    // after the Mie or conjugation post-step.
+    std::vector<int> cols;
+    for (int r = 0; r < nrhs; r++) {
+        if (!skip_external_translation && (!rhs_list || rhs_list[r])) cols.push_back(r);
+        else memset(aout + V * r, 0, sizeof(double) * V);
+    }
     double *din = c->d_kv + V * KV_ST, *dout = c->d_kv + V * KV_PW;  // up to four columns per product
-    for (int r0 = 0; r0 < nrhs; r0 += 4) {
-        int nr = std::min(4, nrhs - r0);
+    for (size_t q0 = 0; q0 < cols.size(); q0 += 4) {
+        const int nr = (int)std::min<size_t>(4, cols.size() - q0);
         int flags[4] = {0, 0, 0, 0};
-        for (int q = 0; q < nr; q++) flags[q] = (con_tran[r0 + q] ? 1 : 0) | (mie_mult[r0 + q] ? 2 : 0);
-        HIPCHK(hipMemcpyAsync(din, ain + V * r0, sizeof(double) * V * nr, hipMemcpyHostToDevice, c->st));
+        for (int q = 0; q < nr; q++) {
+            const int r = cols[q0 + q];
+            flags[q] = (con_tran[r] ? 1 : 0) | (mie_mult[r] ? 2 : 0);
+            HIPCHK(hipMemcpyAsync(din + V * q, ain + V * r, sizeof(double) * V, hipMemcpyHostToDevice, c->st));
+        }
         int rc = matvec(c, nr, din, dout, flags, prec, use_fft);
         if (rc) return rc;
-        HIPCHK(hipMemcpyAsync(aout + V * r0, dout, sizeof(double) * V * nr, hipMemcpyDeviceToHost, c->st));
+        for (int q = 0; q < nr; q++)
+            HIPCHK(hipMemcpyAsync(aout + V * cols[q0 + q], dout + V * q, sizeof(double) * V, hipMemcpyDeviceToHost,
+                                  c->st));
+        HIPCHK(hipStreamSynchronize(c->st));  // din is refilled next
     }
     HIPCHK(hipStreamSynchronize(c->st));
     collect_events(c);
@@ -1452,14 +1568,28 @@ int mstm_hip_cbicg(mstm_hip_ctx *c, int niter, double eps, const double *pnp, do
                    double *errmax)
 {
     if (!c || !c->nsph) return fail(c, MSTM_HIP_ESTATE, "cbicg: no spheres");
-    if (niter <= 0) return fail(c, MSTM_HIP_EUNSUP, "cbicg: niter <= 0 (order-of-scattering) not supported");
+    if (!pnp || !anp || !iter || !errmax) return fail(c, MSTM_HIP_EARG, "cbicg: null argument");
     HIPCHK(hipSetDevice(c->device));
     c->obs.x_valid = false;  // the solution slot is reused
     const size_t V = 2 * (size_t)c->neqns;
     double *db = c->d_kv + V * KV_B, *dx = c->d_kv + V * KV_X;
     HIPCHK(hipMemcpyAsync(db, pnp, sizeof(double) * V, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(dx, anp, sizeof(double) * V, hipMemcpyHostToDevice, c->st));
-    int rc = bicg2_dev(c, 1, niter, &eps, db, dx, MSTM_HIP_FP64, use_fft, iter, errmax);
+    int rc = 0;
+    if (niter > 0) {
+        rc = bicg2_dev(c, 1, niter, &eps, db, dx, MSTM_HIP_FP64, use_fft, iter, errmax);
+    } else if (niter < 0) {
+        rc = oos2_dev(c, 1, -niter, &eps, db, dx, use_fft, iter, errmax);  // order of scattering (:785-825)
+    } else {
+        // niter = 0 (:768-781): anp becomes the host-sphere part of the
+        // product, which is zero for external spheres; pnp = 0 returns first
+        *iter = 0;
+        *errmax = 0.0;
+        double bb[2];
+        HIPCHK(launch_zdot(db, db, c->neqns, 1, c->d_partials, nullptr, c->st));
+        rc = sum_partials(c, 1, bb);
+        if (!rc && bb[0] != 0.0) HIPCHK(hipMemsetAsync(dx, 0, sizeof(double) * V, c->st));
+    }
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(anp, dx, sizeof(double) * V, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
@@ -1468,11 +1598,12 @@ int mstm_hip_cbicg(mstm_hip_ctx *c, int niter, double eps, const double *pnp, do
 }
 
 int mstm_hip_fixedorsoln(mstm_hip_ctx *c, double alpha, double sinc, int dir, double eps, int niter, int solver,
-                         int use_fft, double *amnp, double *qeff, double *maxerr, int *maxiter, int *istat)
+                         int use_fft, double *amnp, double *qeff, double *maxerr, int *maxiter, int *istat,
+                         const int *excited_spheres)
 {
     if (!c || !c->nsph) return fail(c, MSTM_HIP_ESTATE, "fixedorsoln: no spheres");
     if (dir != 1 && dir != 2) return fail(c, MSTM_HIP_EARG, "fixedorsoln: dir must be 1 or 2");
-    if (niter <= 0) return fail(c, MSTM_HIP_EUNSUP, "fixedorsoln: niter <= 0 not supported");
+    if (!maxerr || !maxiter || !istat) return fail(c, MSTM_HIP_EARG, "fixedorsoln: null status argument");
     if (solver != MSTM_HIP_SOLVE_PLAIN && solver != MSTM_HIP_SOLVE_MIXED && solver != MSTM_HIP_SOLVE_LEARNED &&
         solver != MSTM_HIP_SOLVE_FGMRES && solver != MSTM_HIP_SOLVE_LU)
         return fail(c, MSTM_HIP_EUNSUP, "fixedorsoln: solver %d not available", solver);
@@ -1487,10 +1618,22 @@ int mstm_hip_fixedorsoln(mstm_hip_ctx *c, double alpha, double sinc, int dir, do
     const size_t V = 2 * (size_t)n;
     double *pmnp = c->d_kv + V * KV_PM, *x = c->d_kv + V * KV_X, *b = c->d_kv + V * KV_B, *r = c->d_kv + V * KV_R;
     auto t0 = std::chrono::steady_clock::now();
-    HIPCHK(launch_planewave(pmnp, 2 * n, c->nsph, c->d_pos, c->d_order, c->d_offset, alpha, sinc, dir, c->rimed, c->st));
-    // the network is trained for incidence along +z (beta = 0)
+    // excited_spheres (:373-377, 468): spheres left out get no incident field
+    bool all_excited = true;
+    if (excited_spheres)
+        for (int i = 0; i < c->nsph; i++) all_excited = all_excited && excited_spheres[i] != 0;
+    if (!all_excited) {
+        std::vector<int> m(excited_spheres, excited_spheres + c->nsph);
+        for (auto &v : m) v = v != 0;
+        HIPCHK(hipMemcpyAsync(c->d_excited, m.data(), sizeof(int) * c->nsph, hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));  // m lives on the host stack
+    }
+    HIPCHK(launch_planewave(pmnp, 2 * n, c->nsph, c->d_pos, c->d_order, c->d_offset, alpha, sinc, dir, c->rimed, c->st,
+                            all_excited ? nullptr : c->d_excited));
+    // the network is trained for incidence along +z (beta = 0) on fully
+    // excited clusters; niter < 0 is the order-of-scattering series
     const bool lig_on = (solver == MSTM_HIP_SOLVE_LEARNED || solver == MSTM_HIP_SOLVE_FGMRES) && c->lig.ready &&
-                        sinc == 0.0 && dir == 1;
+                        sinc == 0.0 && dir == 1 && all_excited && niter > 0;
     if (lig_on) {
         auto tl = std::chrono::steady_clock::now();
         int rc = lig_predict(c);
@@ -1507,7 +1650,14 @@ int mstm_hip_fixedorsoln(mstm_hip_ctx *c, double alpha, double sinc, int dir, do
     int it[2] = {0, 0}, rc = 0;
     double err[2] = {0, 0};
     const double eps2[2] = {eps, eps};
-    if (solver == MSTM_HIP_SOLVE_PLAIN) {
+    if (niter == 0 && solver != MSTM_HIP_SOLVE_LU) {
+        // no solve (:513-522): amnp = T f, iterations and error 0
+    } else if (niter < 0 && solver != MSTM_HIP_SOLVE_LU) {
+        // cbicg's order-of-scattering series from x = b (:785-825), both
+        // polarizations together; the solution_method letters of this
+        // library other than the direct one all take it
+        rc = oos2_dev(c, 2, -niter, eps2, b, x, use_fft, it, err);
+    } else if (solver == MSTM_HIP_SOLVE_PLAIN) {
         rc = bicg2_dev(c, 2, niter, eps2, b, x, MSTM_HIP_FP64, use_fft, it, err);
     } else if (solver == MSTM_HIP_SOLVE_LU) {
         // direct_solver with number_iterations = 0 (:491-493): LU of A once,
@@ -1542,7 +1692,9 @@ int mstm_hip_fixedorsoln(mstm_hip_ctx *c, double alpha, double sinc, int dir, do
         }
         for (int p = 0; p < 2; p++)
             if (!used[p]) c->stats.lig_fallbacks++;
-        rc = refine2_dev(c, 2, niter, eps, b, x, use_fft, it, err, lig_on ? r : b);
+        // the learned start's residual is an FP32 product: it steers the first
+        // inner solve only, and FP64 residuals decide every answer
+        rc = refine2_dev(c, 2, niter, eps, b, x, use_fft, it, err, lig_on ? r : b, !lig_on);
         for (int p = 0; p < 2 && !rc; p++)
             if (err[p] >= eps && it[p] < niter) {
                 c->stats.lig_fallbacks++;
@@ -1574,13 +1726,9 @@ int mstm_hip_fixedorsoln(mstm_hip_ctx *c, double alpha, double sinc, int dir, do
     HIPCHK(hipMemcpyAsync(amnp, x, sizeof(double) * 2 * V, hipMemcpyDeviceToHost, c->st));
     c->obs.x_valid = true;
     if (qeff) {
-        double *dq = nullptr;
-        HIPCHK(dalloc(&dq, sizeof(double) * 9 * c->nsph));
         HIPCHK(launch_qeff(x, pmnp, 2 * n, c->nsph, c->d_radius, c->d_order, c->d_offset, c->d_mieoff, c->d_an,
-                           c->rimed, dq, c->st));
-        HIPCHK(hipMemcpyAsync(qeff, dq, sizeof(double) * 9 * c->nsph, hipMemcpyDeviceToHost, c->st));
-        HIPCHK(hipStreamSynchronize(c->st));
-        dfree(dq);
+                           c->rimed, c->d_qeff, c->st));
+        HIPCHK(hipMemcpyAsync(qeff, c->d_qeff, sizeof(double) * 9 * c->nsph, hipMemcpyDeviceToHost, c->st));
     }
     HIPCHK(hipStreamSynchronize(c->st));
     collect_events(c);

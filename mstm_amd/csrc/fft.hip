@@ -22,7 +22,7 @@
 #include "kernels.hpp"
 #include "specfun.hpp"
 
-#define LMAX_LIST 12  // largest order in a list translation (node order)
+#define LMAX_LIST MSTM_TAB_L  // largest order in a list translation (node order): the tables' limit
 
 // ===========================================================================
 // Generic directed translations over a slot list (runtime orders).
@@ -649,7 +649,8 @@ static bool is235(int n)
     return n == 1;
 }
 
-FftOp *fft_create(const FftGeometry &g, double fva, int node_order, int neighbor_model, int *cell_dim, double *d_cell,
+FftOp *fft_create(const FftGeometry &g, double *fva, const double *target_min, const double *target_max,
+                  int d_specified, int node_order, int neighbor_model, int *cell_dim, double *d_cell,
                   int *node_order_out, int *err)
 {
     FftOp *f = new FftOp();
@@ -660,13 +661,19 @@ FftOp *fft_create(const FftGeometry &g, double fva, int node_order, int neighbor
     f->pos.assign(g.pos, g.pos + 3 * g.nsph);
     f->radius.assign(g.radius, g.radius + g.nsph);
     const int n = g.nsph;
-    // node_selection (fft_translation-5.f90:654-825), target box = centre bounds
+    // node_selection (fft_translation-5.f90:654-825).  Target box: the
+    // caller's, else the bounds of the sphere centres (sphere_min_position /
+    // sphere_max_position, mstm-modules-33.f90:5237-5244)
     double tmin[3] = {1e10, 1e10, 1e10}, tmax[3] = {-1e10, -1e10, -1e10};
     for (int i = 0; i < n; i++)
         for (int m = 0; m < 3; m++) {
             tmin[m] = std::min(tmin[m], f->pos[3 * i + m]);
             tmax[m] = std::max(tmax[m], f->pos[3 * i + m]);
         }
+    for (int m = 0; m < 3; m++) {
+        if (target_min) tmin[m] = target_min[m];
+        if (target_max) tmax[m] = target_max[m];
+    }
     double amean = 0, svol = 0;
     for (int i = 0; i < n; i++) {
         amean += f->radius[i];
@@ -677,11 +684,27 @@ FftOp *fft_create(const FftGeometry &g, double fva, int node_order, int neighbor
     double tvol = 1.0;
     for (int m = 0; m < 3; m++) tvol *= std::max(tmax[m] - tmin[m], amean);
     const double pi4 = (double)3.14159265358979323846f;  // the reference's real(4) pi
-    double fv = fva;
-    if (fva <= 0.0) fv = std::max(std::min(4.0 * pi4 / 3.0 * svol * svol * svol / tvol, 1.0), 0.02);
-    double d = pow(4.0 * pi4 / 3.0 / fv / (double)n, 1.0 / 3.0) * svol;
+    double d = *d_cell;  // d_specified: the caller's cell size (:708-719 skipped)
+    if (!d_specified) {
+        double fv = *fva;
+        if (fv <= 0.0) fv = std::max(std::min(4.0 * pi4 / 3.0 * svol * svol * svol / tvol, 1.0), 0.02);
+        *fva = fv;  // fva is intent(inout) in the reference
+        d = pow(4.0 * pi4 / 3.0 / fv / (double)n, 1.0 / 3.0) * svol;
+    }
+    if (!(d > 0.0)) {
+        *err = 1;
+        f->err = "node_selection: cell size must be positive";
+        delete f;
+        return nullptr;
+    }
     for (int m = 0; m < 3; m++) {
         int nd = (int)ceil((tmax[m] - tmin[m]) / d);
+        if (nd < 1) {  // correctn235 would not end on 0 (fft_translation-5.f90:1068-1076)
+            *err = 1;
+            f->err = "node_selection: the target box has no extent along an axis";
+            delete f;
+            return nullptr;
+        }
         while (!is235(nd)) nd++;
         f->cd[m] = nd;
     }
@@ -750,6 +773,21 @@ FftOp *fft_create(const FftGeometry &g, double fva, int node_order, int neighbor
     *node_order_out = f->LG;
     *err = 0;
     return f;
+}
+
+void fft_grid(const FftOp *f, double *cell_boundary, int *sphere_node, int *number_neighbor_nodes, int *npairs)
+{
+    if (cell_boundary)
+        for (int m = 0; m < 3; m++) cell_boundary[m] = f->cb[m];
+    if (sphere_node)
+        for (int i = 0; i < f->nsph; i++) {
+            const int c = f->snode[i];
+            sphere_node[3 * i] = c % f->cd[0] + 1;
+            sphere_node[3 * i + 1] = (c / f->cd[0]) % f->cd[1] + 1;
+            sphere_node[3 * i + 2] = c / (f->cd[0] * f->cd[1]) + 1;
+        }
+    if (number_neighbor_nodes) *number_neighbor_nodes = f->nnb - 1;
+    if (npairs) *npairs = f->npairs;
 }
 
 void fft_bind(FftOp *f, const FftGeometry &g)

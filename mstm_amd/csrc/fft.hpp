@@ -22,8 +22,15 @@ struct FftGeometry {
     const double *d_vcc;
     int vcc_LM;
 };
-FftOp *fft_create(const FftGeometry &g, double fva, int node_order, int neighbor_model, int *cell_dim, double *d_cell,
+// node_selection: fva in/out (<= 0: automatic), target box (null: sphere
+// centre bounds), d_cell in (d_specified) / out
+FftOp *fft_create(const FftGeometry &g, double *fva, const double *target_min, const double *target_max,
+                  int d_specified, int node_order, int neighbor_model, int *cell_dim, double *d_cell,
                   int *node_order_out, int *err);
+// the grid of node_selection: cell_boundary[3] (the target box minimum),
+// sphere_node[3 nsph] (1-based cell indices), number of stencil cells - 1
+// (number_neighbor_nodes) and near-field pairs
+void fft_grid(const FftOp *f, double *cell_boundary, int *sphere_node, int *number_neighbor_nodes, int *npairs);
 // (re)binds the context's device pointers (vcc may have been regrown)
 void fft_bind(FftOp *f, const FftGeometry &g);
 int fft_setup(FftOp *f, int prec, hipStream_t st);

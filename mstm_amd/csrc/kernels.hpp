@@ -33,7 +33,8 @@ hipError_t launch_mie(const double *x, const double *mre, const double *mim, dou
 hipError_t launch_mie_mult(const double *ain, double *aout, int nsph, const int *order, const int *offset,
                            const int *mieoff, const double *an, int idir, int inverse, int Lmax, hipStream_t st);
 hipError_t launch_planewave(double *pmnp, int ld, int nsph, const double *pos, const int *order, const int *offset,
-                            double alpha, double sinc, int dir, double rimed, hipStream_t st);
+                            double alpha, double sinc, int dir, double rimed, hipStream_t st,
+                            const int *excited = nullptr);
 hipError_t launch_qeff(const double *amnp, const double *pmnp, int ld, int nsph, const double *radius, const int *order,
                        const int *offset, const int *mieoff, const double *an, double rimed, double *qeff,
                        hipStream_t st);

@@ -172,13 +172,20 @@ hipError_t launch_mie_mult(const double *ain, double *aout, int nsph, const int 
 // Plane-wave VSWF coefficients (genplanewavecoef with lr_tran, :1453-1520)
 // times the zero-boundary phase of layerplanewavecoef (:3901-3965), one
 // thread per sphere.  pmnp(:,pol) columns of length ld (complex).
+// excited (may be null): spheres with excited[i] == 0 get zero blocks, as
+// sphereplanewavecoef's excited_spheres (mstm-scatprops-26.f90:262-264).
 __global__ void planewave_kernel(cplx<double> *__restrict__ pmnp, int ld, int nsph, const double *__restrict__ pos,
                                  const int *__restrict__ order, const int *__restrict__ offset, double alpha,
-                                 double sinc, int dir, double rimed)
+                                 double sinc, int dir, double rimed, const int *__restrict__ excited)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nsph) return;
     int L = order[i], NB = L * (L + 2);
+    if (excited && !excited[i]) {
+        cplx<double> *q0 = pmnp + offset[i], *q1 = pmnp + (int64_t)ld + offset[i];
+        for (int k = 0; k < 2 * NB; k++) q0[k] = q1[k] = cmk(0.0, 0.0);
+        return;
+    }
     // dir = 1: incidence from below (cos beta > 0), dir = 2: from above (:3934)
     double cb = sqrt((1.0 - sinc / rimed) * (1.0 + sinc / rimed)) * (double)(3 - 2 * dir);
     const double *x = pos + 3 * i;
@@ -247,12 +254,12 @@ __global__ void planewave_kernel(cplx<double> *__restrict__ pmnp, int ld, int ns
 }
 
 hipError_t launch_planewave(double *pmnp, int ld, int nsph, const double *pos, const int *order, const int *offset,
-                            double alpha, double sinc, int dir, double rimed, hipStream_t st)
+                            double alpha, double sinc, int dir, double rimed, hipStream_t st, const int *excited)
 {
     int bs = 64;
     hipLaunchKernelGGL(planewave_kernel, dim3((nsph + bs - 1) / bs), dim3(bs), 0, st,
                        reinterpret_cast<cplx<double> *>(pmnp), ld / 2, nsph, pos, order, offset, alpha, sinc, dir,
-                       rimed);
+                       rimed, excited);
     return hipGetLastError();
 }
 

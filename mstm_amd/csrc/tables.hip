@@ -28,10 +28,10 @@ __device__ void build_slot(T *__restrict__ grp, int k, int LT, int LS, int itype
     // ---- Wigner d rows, degree by degree --------------------------------
     double sbe = (fabs(ct) >= 1.0) ? 0.0 : sqrt(fabs((1.0 + ct) * (1.0 - ct)));
     double cbe2 = 0.5 * (1.0 + ct), sbe2 = 0.5 * (1.0 - ct);
-    double dk0[2 * MSTM_MAX_L + 3], dk01[2 * MSTM_MAX_L + 3];
-    double rowp[2 * MSTM_MAX_L + 1], rowc[2 * MSTM_MAX_L + 1];
-    const int o0 = MSTM_MAX_L + 1;
-    for (int q = 0; q < 2 * MSTM_MAX_L + 3; q++) dk0[q] = dk01[q] = 0.0;
+    double dk0[2 * MSTM_TAB_L + 3], dk01[2 * MSTM_TAB_L + 3];
+    double rowp[2 * MSTM_TAB_L + 1], rowc[2 * MSTM_TAB_L + 1];
+    const int o0 = MSTM_TAB_L + 1;
+    for (int q = 0; q < 2 * MSTM_TAB_L + 3; q++) dk0[q] = dk01[q] = 0.0;
     dk0[o0] = 1.0;
     int sgn = 1;
     double sben = 1.0, bc = 1.0;  // bc = bcof(n,n) = sqrt((2n)!/(n!)^2)
@@ -101,18 +101,18 @@ __device__ void build_slot(T *__restrict__ grp, int k, int LT, int LS, int itype
         }
     }
     // ---- axial coefficients ---------------------------------------------
-    cplx<double> hw[2 * MSTM_MAX_L + 1];
+    cplx<double> hw[2 * MSTM_TAB_L + 1];
     const int nw = LT + LS;
     double z = r * rimed;
     if (itype == 3) {
         sph_hankel_up(nw, z, hw);
     } else {
-        double jw[2 * MSTM_MAX_L + 1];
+        double jw[2 * MSTM_TAB_L + 1];
         sph_bessel_j(nw, z, jw);
         for (int w = 0; w <= nw; w++) hw[w] = cmk(jw[w], 0.0);
     }
-    cplx<double> act[(MSTM_MAX_L + 1) * (MSTM_MAX_L + 1)][2];
-#define ACT(n, l) act[(n) * (MSTM_MAX_L + 1) + (l)]
+    cplx<double> act[(MSTM_TAB_L + 1) * (MSTM_TAB_L + 1)][2];
+#define ACT(n, l) act[(n) * (MSTM_TAB_L + 1) + (l)]
     for (int b = 0; b < 2 * nmin + 1; b++) {
         int m = mblock_m(b);
         if (sym && m < 0) continue;  // A_{-m} of one mode is A_m of the other
@@ -197,6 +197,7 @@ template <typename T>
 hipError_t launch_direct_tables(T *tab, const int2 *tiles, int ntiles, int L, int nsph, const double *pos, double rimed,
                                 const double *vcc, int LM, hipStream_t st)
 {
+    if (L < 1 || L > MSTM_TAB_L) return hipErrorInvalidValue;  // build_slot's arrays
     int64_t se = direct_slot_len(L, (int)sizeof(T));
     int64_t nq = (int64_t)ntiles * MSTM_SLOTS * MSTM_SLOTS;
     int bs = 128;
@@ -235,6 +236,7 @@ template <typename T>
 hipError_t launch_list_tables(T *tab, const double *rtran, int nslots, int LT, int LS, int itype, double rimed,
                               const double *vcc, int LM, hipStream_t st)
 {
+    if (LT < 1 || LS < 1 || LT > MSTM_TAB_L || LS > MSTM_TAB_L) return hipErrorInvalidValue;  // build_slot's arrays
     const int VW = 16 / (int)sizeof(T);
     int64_t se = slot_len(LT, LS, VW);
     int bs = 128;

@@ -47,27 +47,35 @@
          type(c_ptr), value :: ctx
          integer(c_int), value :: precision
          end function mstm_hip_setup_direct
-         integer(c_int) function mstm_hip_node_selection(ctx,fva,node_order,model,cell_dim,d_cell, &
-            node_order_out) bind(c,name='mstm_hip_node_selection')
+         integer(c_int) function mstm_hip_node_selection(ctx,fva,target_min,target_max,d_specified, &
+            d_cell,local_host,node_order,model,cell_dim,node_order_out) bind(c,name='mstm_hip_node_selection')
          import :: c_ptr, c_int, c_double
          type(c_ptr), value :: ctx
-         real(c_double), value :: fva
-         integer(c_int), value :: node_order,model
+         real(c_double) :: fva,d_cell
+         type(c_ptr), value :: target_min,target_max
+         integer(c_int), value :: d_specified,local_host,node_order,model
          integer(c_int) :: cell_dim(3),node_order_out
-         real(c_double) :: d_cell
          end function mstm_hip_node_selection
+         integer(c_int) function mstm_hip_fft_grid(ctx,cell_boundary,sphere_node,number_neighbor_nodes) &
+            bind(c,name='mstm_hip_fft_grid')
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double) :: cell_boundary(3)
+         integer(c_int) :: sphere_node(*),number_neighbor_nodes
+         end function mstm_hip_fft_grid
          integer(c_int) function mstm_hip_setup_fft(ctx,precision) bind(c,name='mstm_hip_setup_fft')
          import :: c_ptr, c_int
          type(c_ptr), value :: ctx
          integer(c_int), value :: precision
          end function mstm_hip_setup_fft
          integer(c_int) function mstm_hip_sphereinteraction(ctx,nrhs,ain,aout,con_tran,mie_mult, &
-            use_fft,precision) bind(c,name='mstm_hip_sphereinteraction')
+            use_fft,precision,rhs_list,skip_external_translation) bind(c,name='mstm_hip_sphereinteraction')
          import :: c_ptr, c_int, c_double_complex
          type(c_ptr), value :: ctx
-         integer(c_int), value :: nrhs,use_fft,precision
+         integer(c_int), value :: nrhs,use_fft,precision,skip_external_translation
          complex(c_double_complex) :: ain(*),aout(*)
          integer(c_int) :: con_tran(*),mie_mult(*)
+         type(c_ptr), value :: rhs_list
          end function mstm_hip_sphereinteraction
          integer(c_int) function mstm_hip_cbicg(ctx,niter,eps,pnp,anp,use_fft,iter,errmax) &
             bind(c,name='mstm_hip_cbicg')
@@ -80,7 +88,7 @@
          real(c_double) :: errmax
          end function mstm_hip_cbicg
          integer(c_int) function mstm_hip_fixedorsoln(ctx,alpha,sinc,dir,eps,niter,solver,use_fft, &
-            amnp,qeff,maxerr,maxiter,istat) bind(c,name='mstm_hip_fixedorsoln')
+            amnp,qeff,maxerr,maxiter,istat,excited_spheres) bind(c,name='mstm_hip_fixedorsoln')
          import :: c_ptr, c_int, c_double, c_double_complex
          type(c_ptr), value :: ctx
          real(c_double), value :: alpha,sinc,eps
@@ -88,6 +96,7 @@
          complex(c_double_complex) :: amnp(*)
          real(c_double) :: qeff(*),maxerr
          integer(c_int) :: maxiter,istat
+         type(c_ptr), value :: excited_spheres
          end function mstm_hip_fixedorsoln
          integer(c_int) function mstm_hip_set_learned(ctx,params,nparams,hidden,nlayers,lnet) &
             bind(c,name='mstm_hip_set_learned')

@@ -15,11 +15,19 @@
 !   hip_asymmetry           <- s00(1,1,0:1) of fosmexpansion (mstm-scatprops-26.f90:1820)
 !   hip_tmatrix_solution    <- tmatrix_solution    (mstm-solver-8.f90:22-25)
 !   hip_direct_solver       <- direct_solver       (mstm-solver-8.f90:571-572)
+!   hip_node_selection      <- node_selection      (fft_translation-5.f90:654)
+!   hip_clear_fft_matrix    <- clear_fft_matrix    (fft_translation-5.f90:42)
+!   hip_fft_external_to_external_expansion
+!                           <- fft_external_to_external_expansion
+!                                                  (fft_translation-5.f90:72-74)
 ! The GPU context is created on first use and refilled when
 ! initialize_solver / initial_run is true (geometry, orders and an_mie are
 ! taken from the MSTM modules, so the Mie data are the host's own).
-! fft_translation_option selects the node-grid operator; its grid is the
-! one node_selection left in the fft_translation module variables.
+! fft_translation_option selects the node-grid operator.  Its grid is made
+! from the arguments of the last hip_node_selection call (the reference's
+! node_selection runs as well, so the host's module state is its own); the
+! GPU grid must equal the cell_dim / d_cell of the fft_translation module,
+! or the run stops with a message.
 !
       module mstm_hip_solver
       use iso_c_binding
@@ -33,6 +41,12 @@
       type(c_ptr), save :: hip_ctx=c_null_ptr
       logical, save :: hip_loaded=.false.
       integer, save :: hip_precision=0
+! arguments of the last hip_node_selection (fft_translation-5.f90:654)
+      logical, save :: hip_grid_args=.false.,hip_grid_dspec=.false.,hip_grid_box=.false.
+      real(8), target, save :: hip_grid_tmin(3),hip_grid_tmax(3)
+      real(8), save :: hip_grid_fva=0.d0,hip_grid_d=0.d0
+! firstrun defaults of fixedorsoln and direct_solver (mstm-solver-8.f90:383-387, 590-594)
+      logical, save :: hip_fo_first=.true.,hip_ds_first=.true.
 
       contains
 
@@ -70,13 +84,93 @@
          call hip_check(rc,'set_spheres')
          call hip_check(mstm_hip_set_mie(hip_ctx,sphere_order,an_mie),'set_mie')
          if(fft_translation_option) then
-            call hip_check(mstm_hip_node_selection(hip_ctx,cell_volume_fraction,node_order, &
-               neighbor_node_model,cd,dc,no),'node_selection')
+            call hip_grid(cd,dc,no)
+            if(any(cd.ne.cell_dim).or.abs(dc-d_cell).gt.1.d-12*abs(d_cell)) then
+               write(run_print_unit,'('' mstm_hip: GPU node grid'',3i5,es16.8,'' differs from the host grid'', &
+                  & 3i5,es16.8,'': call hip_node_selection in place of node_selection'')') cd,dc,cell_dim,d_cell
+               stop
+            endif
             call hip_check(mstm_hip_setup_fft(hip_ctx,int(hip_precision,c_int)),'setup_fft')
          endif
          deallocate(ri)
          hip_loaded=.true.
          end subroutine hip_load_state
+!
+! the GPU node grid from the last hip_node_selection arguments (or the
+! defaults of node_selection: the host's fva, the sphere-centre box, no
+! specified d) and the node_order rule of main_calling_program
+! (mstm-input-37.f90:1256-1260; input_node_order is the module's node_order
+! as main_calling_program leaves it, so it is passed as a positive order)
+!
+         subroutine hip_grid(cd,dc,no)
+         integer(c_int) :: cd(3),no,ds
+         real(c_double) :: dc,fv
+         type(c_ptr) :: pmin,pmax
+         fv=cell_volume_fraction
+         dc=d_cell
+         ds=0
+         pmin=c_null_ptr
+         pmax=c_null_ptr
+         if(hip_grid_args) then
+            fv=hip_grid_fva
+            if(hip_grid_dspec) then
+               ds=1
+               dc=hip_grid_d
+            endif
+            if(hip_grid_box) then
+               pmin=c_loc(hip_grid_tmin)
+               pmax=c_loc(hip_grid_tmax)
+            endif
+         endif
+         call hip_check(mstm_hip_node_selection(hip_ctx,fv,pmin,pmax,ds,dc,0_c_int,int(node_order,c_int), &
+            int(neighbor_node_model,c_int),cd,no),'node_selection')
+         end subroutine hip_grid
+!
+! node_selection (fft_translation-5.f90:654-825) with the reference's
+! argument list: runs the reference routine for the host's module state and
+! keeps the arguments for the GPU grid (made at the next state load).
+! local_host must be 0: spheres inside a host sphere are not on the GPU path.
+!
+         subroutine hip_node_selection(fva,target_min,target_max,d_specified,local_host)
+         real(8) :: fva
+         real(8), optional :: target_min(3),target_max(3)
+         logical, optional :: d_specified
+         integer, optional :: local_host
+         integer :: lh
+         lh=0
+         if(present(local_host)) lh=local_host
+         if(lh.ne.0) then
+            write(run_print_unit,'('' mstm_hip: node_selection with a local host sphere is not on the GPU path'')')
+            stop
+         endif
+         hip_grid_fva=fva
+         hip_grid_dspec=.false.
+         if(present(d_specified)) hip_grid_dspec=d_specified
+         hip_grid_d=d_cell
+         hip_grid_box=.true.
+         hip_grid_tmin=sphere_min_position
+         hip_grid_tmax=sphere_max_position
+         if(present(target_min)) hip_grid_tmin=target_min
+         if(present(target_max)) hip_grid_tmax=target_max
+         hip_grid_args=.true.
+         call node_selection(fva,target_min=hip_grid_tmin,target_max=hip_grid_tmax, &
+            d_specified=hip_grid_dspec,local_host=lh)
+         hip_loaded=.false.
+         end subroutine hip_node_selection
+!
+! clear_fft_matrix (fft_translation-5.f90:42-70): the reference routine for
+! the host's state; the GPU state is reloaded at the next call, and with
+! clear_h its operator tables and node-kernel spectrum are released now
+!
+         subroutine hip_clear_fft_matrix(clear_h)
+         logical, optional :: clear_h
+         logical :: ch
+         ch=.false.
+         if(present(clear_h)) ch=clear_h
+         call clear_fft_matrix(clear_h=ch)
+         if(ch.and.c_associated(hip_ctx)) call hip_check(mstm_hip_clear(hip_ctx,1_c_int),'clear')
+         hip_loaded=.false.
+         end subroutine hip_clear_fft_matrix
 !
 ! cbicg (mstm-solver-8.f90:654-949) on the GPU, FP64, iteration-identical
 !
@@ -135,9 +229,19 @@
          real(8), allocatable :: q(:,:,:)
          logical :: init
          integer(c_int) :: solver,mi,ist,ufft
-         init=.true.
+         integer(c_int), allocatable, target :: ex(:)
+         type(c_ptr) :: pex
+         init=hip_fo_first
          if(present(initialize_solver)) init=initialize_solver
+         hip_fo_first=.false.
          if(init.or.(.not.hip_loaded)) call hip_load_state()
+         pex=c_null_ptr
+         if(present(excited_spheres)) then
+            allocate(ex(number_spheres))
+            ex=0
+            where(excited_spheres) ex=1
+            pex=c_loc(ex)
+         endif
          solver=mstm_hip_solve_plain
          if(present(solution_method)) then
             if(solution_method.eq.'m') then
@@ -154,7 +258,8 @@
          if(fft_translation_option) ufft=1
          allocate(q(3,3,number_spheres))
          call hip_check(mstm_hip_fixedorsoln(hip_ctx,alpha,sinc,int(dir,c_int),eps,int(niter,c_int), &
-            solver,ufft,amnp,q,maxerr,mi,ist),'fixedorsoln')
+            solver,ufft,amnp,q,maxerr,mi,ist,pex),'fixedorsoln')
+         if(allocated(ex)) deallocate(ex)
          maxiter=mi
          istat=ist
          if(qeffdim.eq.1) then
@@ -174,7 +279,9 @@
          logical, optional :: initial_run,rhs_list(nrhs),con_tran(nrhs),mie_mult(nrhs), &
             fft_option,store_matrix_option,skip_external_translation
          complex(8) :: ain(neqns,nrhs),aout(neqns,nrhs)
-         integer(c_int) :: ct(nrhs),mm(nrhs),ufft
+         integer(c_int) :: ct(nrhs),mm(nrhs),ufft,skip
+         integer(c_int), target :: rl(nrhs)
+         type(c_ptr) :: prl
          integer :: r
          if(present(initial_run)) then
             if(initial_run) call hip_load_state()
@@ -198,9 +305,72 @@
             ufft=0
             if(fft_option) ufft=1
          endif
+         prl=c_null_ptr
+         if(present(rhs_list)) then
+            rl=0
+            where(rhs_list) rl=1
+            prl=c_loc(rl)
+         endif
+         skip=0
+         if(present(skip_external_translation)) then
+            if(skip_external_translation) skip=1
+         endif
          call hip_check(mstm_hip_sphereinteraction(hip_ctx,int(nrhs,c_int),ain,aout,ct,mm,ufft, &
-            int(hip_precision,c_int)),'sphereinteraction')
+            int(hip_precision,c_int),prl,skip),'sphereinteraction')
          end subroutine hip_sphereinteraction
+!
+! fft_external_to_external_expansion (fft_translation-5.f90:72-336): gout = H ain
+! on the node-grid operator (no Mie multiplication), H^T ain for con_tran
+! columns.  The C entry returns conj(H^T conj(x)) for a con_tran column
+! without Mie multiplication, so those columns go in and come out conjugated.
+!
+         subroutine hip_fft_external_to_external_expansion(neqns,nrhs,ain,gout, &
+            store_matrix_option,initial_run,rhs_list,mpi_comm,con_tran)
+         integer :: neqns,nrhs
+         integer, optional :: mpi_comm
+         logical, optional :: store_matrix_option,initial_run,rhs_list(nrhs),con_tran(nrhs)
+         complex(8) :: ain(neqns,nrhs),gout(neqns,nrhs)
+         complex(8), allocatable :: at(:,:)
+         integer(c_int) :: ct(nrhs),mm(nrhs)
+         integer(c_int), target :: rl(nrhs)
+         type(c_ptr) :: prl
+         integer :: r
+         if(.not.fft_translation_option) then
+            write(run_print_unit,'('' mstm_hip: fft_external_to_external_expansion needs fft_translation_option'')')
+            stop
+         endif
+         if(present(initial_run)) then
+            if(initial_run) call hip_load_state()
+         endif
+         if(.not.hip_loaded) call hip_load_state()
+         ct=0
+         if(present(con_tran)) then
+            do r=1,nrhs
+               if(con_tran(r)) ct(r)=1
+            enddo
+         endif
+         mm=0
+         prl=c_null_ptr
+         if(present(rhs_list)) then
+            rl=0
+            where(rhs_list) rl=1
+            prl=c_loc(rl)
+         endif
+         allocate(at(neqns,nrhs))
+         do r=1,nrhs
+            if(ct(r).ne.0) then
+               at(:,r)=conjg(ain(:,r))
+            else
+               at(:,r)=ain(:,r)
+            endif
+         enddo
+         call hip_check(mstm_hip_sphereinteraction(hip_ctx,int(nrhs,c_int),at,gout,ct,mm,1_c_int, &
+            int(hip_precision,c_int),prl,0_c_int),'fft_external_to_external_expansion')
+         do r=1,nrhs
+            if(ct(r).ne.0) gout(:,r)=conjg(gout(:,r))
+         enddo
+         deallocate(at)
+         end subroutine hip_fft_external_to_external_expansion
 !
 ! merge_to_common_origin (mstm-scatprops-26.f90:447-559) of both
 ! polarizations in one call, with the host's translation_order (spheredata)
@@ -343,10 +513,14 @@
          complex(8) :: pnp(number_eqns),anp(number_eqns)
          integer(c_int) :: init,niter
          real(c_double) :: serr
-         if(.not.hip_loaded) call hip_load_state()
-         init=1
-         if(present(initialize_solver)) then
-            if(.not.initialize_solver) init=0
+         logical :: linit
+         linit=hip_ds_first
+         if(present(initialize_solver)) linit=initialize_solver
+         if(linit.or.(.not.hip_loaded)) call hip_load_state()
+         init=0
+         if(linit) then
+            init=1
+            hip_ds_first=.false.
          endif
          niter=3
          if(present(number_iterations)) niter=number_iterations

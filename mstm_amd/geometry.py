@@ -1,11 +1,14 @@
 """Seeded cluster geometries and the MSTM position-file format.
 
-The reference reads sphere data as whitespace-separated lines
-``x y z [r [m_re m_im [m2_re m2_im]]]`` (read_sphere_data_input_file,
-mstm-input-37.f90:2871-2924); missing radius -> 1, missing index -> 1+0i.
-Its own random generator is unseeded (random_configuration-10.f90:77-80), so
-the workloads here come from these seeded samplers instead.  Lengths are in
-units of 1/k (size parameter units), as with length_scale_factor = 1.
+The reference reads sphere data as lines ``x y z [r [m [m2]]]``
+(read_sphere_data_input_file, mstm-input-37.f90:2871-2924), each complex
+index either as two numbers or as ``(re,im)``; missing radius -> 1, missing
+index -> 1+0i, missing second (chiral) index -> the first.  Positions and
+radii are then scaled by length_scale_factor and both indices by
+ref_index_scale_factor.  Its own random generator is unseeded
+(random_configuration-10.f90:77-80), so the workloads here come from these
+seeded samplers instead.  Lengths are in units of 1/k (size parameter units),
+as with length_scale_factor = 1.
 """
 from __future__ import annotations
 
@@ -33,17 +36,44 @@ def write_pos(path: str, c: Cluster) -> None:
             f.write(f"{p[0]:.17e} {p[1]:.17e} {p[2]:.17e} {r:.17e} {m.real:.17e} {m.imag:.17e}\n")
 
 
-def read_pos(path: str, default_ri: complex = 1.0 + 0.0j) -> Cluster:
-    pos, rad, ri = [], [], []
+def _pos_fields(line: str) -> list[float]:
+    """The numbers of one position-file line, a parenthesised complex
+    ``(re,im)`` counting as its two parts, as list-directed input reads it."""
+    out = []
+    for tok in line.replace("(", " ").replace(")", " ").replace(",", " ").split():
+        out.append(float(tok.replace("d", "e").replace("D", "e")))
+    return out
+
+
+def read_pos(path: str, number_spheres: int | None = None, length_scale_factor: float = 1.0,
+             ref_index_scale_factor: complex = 1.0, with_chiral: bool = False):
+    """read_sphere_data_input_file (mstm-input-37.f90:2871-2924): up to
+    number_spheres lines (all when None) of ``x y z [r [m_re m_im [m2_re
+    m2_im]]]``.  A line gives the radius from 4 numbers on, the index from 6
+    on and the second index from 8 on; otherwise radius 1, index 1+0i and the
+    second index equal to the first.  A line with fewer than three numbers,
+    or fewer lines than number_spheres, is an error, as in the reference.
+    Returns a Cluster (and the (N,) second indices when with_chiral)."""
+    pos, rad, ri, ri2 = [], [], [], []
     with open(path) as f:
         for line in f:
-            v = line.split()
-            if len(v) < 3:
+            if number_spheres is not None and len(pos) == number_spheres:
+                break
+            v = _pos_fields(line)
+            if number_spheres is None and not v:
                 continue
-            pos.append([float(v[0]), float(v[1]), float(v[2])])
-            rad.append(float(v[3]) if len(v) >= 4 else 1.0)
-            ri.append(complex(float(v[4]), float(v[5])) if len(v) >= 6 else default_ri)
-    return Cluster(np.array(pos), np.array(rad), np.array(ri, dtype=complex))
+            if len(v) < 3:
+                raise ValueError(f"read error in sphere data input file {path}, line {len(pos) + 1}")
+            pos.append(v[:3])
+            rad.append(v[3] if len(v) >= 4 else 1.0)
+            m1 = complex(v[4], v[5]) if len(v) >= 6 else 1.0 + 0.0j
+            ri.append(m1)
+            ri2.append(complex(v[6], v[7]) if len(v) >= 8 else m1)
+    if number_spheres is not None and len(pos) < number_spheres:
+        raise ValueError(f"insufficient data in input file: {len(pos)} lines, need {number_spheres}")
+    ls, rs = float(length_scale_factor), complex(ref_index_scale_factor)
+    c = Cluster(np.array(pos, dtype=float) * ls, np.array(rad, dtype=float) * ls, np.array(ri, dtype=complex) * rs)
+    return (c, np.array(ri2, dtype=complex) * rs) if with_chiral else c
 
 
 class _Grid:

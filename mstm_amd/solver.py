@@ -105,27 +105,49 @@ class HipContext:
     def setup_direct(self, precision: int = FP64):
         self._chk(self.L.mstm_hip_setup_direct(self.h, precision), "setup_direct")
 
-    def node_selection(self, cell_volume_fraction: float = 0.0, node_order: int = -1, neighbor_node_model: int = 2):
+    def node_selection(self, cell_volume_fraction: float = 0.0, node_order: int = -1, neighbor_node_model: int = 2,
+                       target_min=None, target_max=None, d_specified: bool = False, d_cell: float = 0.0,
+                       local_host: int = 0):
+        """node_selection(fva, target_min, target_max, d_specified, local_host)
+        (fft_translation-5.f90:654-825) and the node_order rule
+        (mstm-input-37.f90:1256-1260).  Returns the grid: cell_dim, d_cell,
+        node_order, the volume fraction used (fva), cell_boundary,
+        sphere_node (nsphere, 3; 1-based) and number_neighbor_nodes."""
         cd = np.zeros(3, np.int32)
-        d = np.zeros(1)
+        d = np.array([float(d_cell)])
+        fv = np.array([float(cell_volume_fraction)])
         no = np.zeros(1, np.int32)
-        self._chk(self.L.mstm_hip_node_selection(self.h, cell_volume_fraction, node_order, neighbor_node_model, _i(cd),
-                                                 _d(d), _i(no)), "node_selection")
-        return dict(cell_dim=cd, d_cell=float(d[0]), node_order=int(no[0]))
+        tmn = None if target_min is None else np.ascontiguousarray(target_min, np.float64)
+        tmx = None if target_max is None else np.ascontiguousarray(target_max, np.float64)
+        self._chk(self.L.mstm_hip_node_selection(self.h, _d(fv), None if tmn is None else _d(tmn),
+                                                 None if tmx is None else _d(tmx), int(bool(d_specified)), _d(d),
+                                                 int(local_host), int(node_order), int(neighbor_node_model), _i(cd),
+                                                 _i(no)), "node_selection")
+        cb = np.zeros(3)
+        sn = np.zeros((self.nsphere, 3), np.int32)
+        nn = np.zeros(1, np.int32)
+        self._chk(self.L.mstm_hip_fft_grid(self.h, _d(cb), _i(sn), _i(nn)), "fft_grid")
+        return dict(cell_dim=cd, d_cell=float(d[0]), node_order=int(no[0]), fva=float(fv[0]), cell_boundary=cb,
+                    sphere_node=sn, number_neighbor_nodes=int(nn[0]))
 
     def setup_fft(self, precision: int = FP64):
         self._chk(self.L.mstm_hip_setup_fft(self.h, precision), "setup_fft")
 
-    def sphereinteraction(self, ain, con_tran, mie_mult=None, fft_option=False, precision: int = FP64):
-        """aout = A ain per column (sphereinteraction, mstm-scatprops-26.f90:25-180)."""
+    def sphereinteraction(self, ain, con_tran, mie_mult=None, fft_option=False, precision: int = FP64,
+                          rhs_list=None, skip_external_translation: bool = False):
+        """aout = A ain per column (sphereinteraction, mstm-scatprops-26.f90:25-180).
+        Columns left out of rhs_list, or all with skip_external_translation,
+        come back zero (no translation)."""
         ain = np.ascontiguousarray(np.atleast_2d(ain), np.complex128)
         nrhs = ain.shape[0]
         ct = np.array([1 if x else 0 for x in con_tran], np.int32)
         mm = np.array([1 if x else 0 for x in (mie_mult if mie_mult is not None else [True] * nrhs)], np.int32)
+        rl = None if rhs_list is None else np.array([1 if x else 0 for x in rhs_list], np.int32)
         out = np.zeros_like(ain)
         self._chk(self.L.mstm_hip_sphereinteraction(self.h, nrhs, _d(ain.view(np.float64)), _d(out.view(np.float64)),
-                                                    _i(ct), _i(mm), int(bool(fft_option)), precision),
-                  "sphereinteraction")
+                                                    _i(ct), _i(mm), int(bool(fft_option)), precision,
+                                                    None if rl is None else _i(rl),
+                                                    int(bool(skip_external_translation))), "sphereinteraction")
         return out
 
     # -- solver (cbicg / fixedorsoln, mstm-solver-8.f90) ----------------------
@@ -139,16 +161,22 @@ class HipContext:
         return x, int(it[0]), float(err[0])
 
     def fixedorsoln(self, alpha=0.0, sinc=0.0, eps=1e-6, niter=10000, solver=SOLVE_PLAIN, fft_option=False,
-                    direction=1):
+                    direction=1, excited_spheres=None):
+        """fixedorsoln (mstm-solver-8.f90:350-569).  niter < 0 runs the
+        order-of-scattering series, niter = 0 no solve; excited_spheres
+        (nsphere flags, None = all) switches the incident field per sphere."""
         ne = self.number_eqns
         amnp = np.zeros((2, ne), np.complex128)
         qeff = np.zeros((self.nsphere, 3, 3))
         maxerr = np.zeros(1)
         maxiter = np.zeros(1, np.int32)
         istat = np.zeros(1, np.int32)
+        ex = None if excited_spheres is None else np.array([1 if v else 0 for v in excited_spheres], np.int32)
+        if ex is not None and len(ex) != self.nsphere:
+            raise ValueError("excited_spheres needs one flag per sphere")
         self._chk(self.L.mstm_hip_fixedorsoln(self.h, alpha, sinc, direction, eps, niter, solver, int(bool(fft_option)),
                                               _d(amnp.view(np.float64)), _d(qeff), _d(maxerr), _i(maxiter),
-                                              _i(istat)), "fixedorsoln")
+                                              _i(istat), None if ex is None else _i(ex)), "fixedorsoln")
         return dict(amnp=amnp, qeff=qeff, maxerr=float(maxerr[0]), iterations=int(maxiter[0]), istat=int(istat[0]))
 
     # -- post-solve observables of a single-origin expansion ------------------

@@ -877,6 +877,13 @@ void orc_sphereinteraction(const orc_cluster *c, int nrhs, const int *contran, c
  * pmnp(neqns, 2), column = incident polarization.                      */
 void orc_planewave(const orc_cluster *c, double alpha, double sinc, int dir, zc *pmnp)
 {
+    orc_planewave_ex(c, alpha, sinc, dir, NULL, pmnp);
+}
+
+/* the same with sphereplanewavecoef's excited_spheres (mstm-scatprops-26.f90:
+ * 262-264): spheres with excited[i] == 0 keep zero blocks; NULL = all    */
+void orc_planewave_ex(const orc_cluster *c, double alpha, double sinc, int dir, const int *excited, zc *pmnp)
+{
     int ne = c->neqns;
     double rim = creal(c->rimed);
     double cb = sqrt((1.0 - sinc / rim) * (1.0 + sinc / rim)) * (3 - 2 * dir);
@@ -886,6 +893,7 @@ void orc_planewave(const orc_cluster *c, double alpha, double sinc, int dir, zc 
     orc_rotcoef(cb, 1, Lmax, drot);
     memset(pmnp, 0, sizeof(zc) * ne * 2);
     for (int i = 0; i < c->n; i++) {
+        if (excited && !excited[i]) continue;
         int L = c->order[i], bl = L * (L + 2), o = c->offset[i];
         const double *x = c->pos + 3 * i;
         zc ph = cexp(CI * (sinc * (cos(alpha) * x[0] + sin(alpha) * x[1]) + rim * cb * x[2]));
@@ -925,6 +933,33 @@ int orc_cbicg(const orc_cluster *c, int use_fft, int niter, double eps, const zc
     int ne = c->neqns;
     *errmax = 0.0;
     if (creal(zdotc(ne, pnp, pnp)) == 0.0) return 0;
+    if (niter == 0) {
+        /* (:768-781): anp = the host-sphere part of the product with
+         * skip_external_translation, zero for external spheres */
+        memset(anp, 0, sizeof(zc) * ne);
+        return 0;
+    }
+    if (niter < 0) {
+        /* order of scattering (:785-825): cp = anp; per term cr = T H cp,
+         * anp += cr, cp = cr; stop when |cr|^2/|pnp|^2 < eps */
+        zc *cp = malloc(sizeof(zc) * ne), *cr = malloc(sizeof(zc) * ne);
+        double enorm = creal(zdotc(ne, pnp, pnp));
+        int f1[1] = {0}, t1[1] = {1}, iter;
+        memcpy(cp, anp, sizeof(zc) * ne);
+        for (iter = 1; iter <= -niter; iter++) {
+            orc_sphereinteraction(c, 1, f1, t1, use_fft, cp, cr);
+            for (int q = 0; q < ne; q++) anp[q] += cr[q];
+            memcpy(cp, cr, sizeof(zc) * ne);
+            double eerr = creal(zdotc(ne, cr, cr)) / enorm;
+            if (eerr < eps) {
+                *errmax = eerr;
+                break;
+            }
+        }
+        free(cp);
+        free(cr);
+        return iter;
+    }
     zc *cr = calloc(ne, sizeof(zc)), *cp = malloc(sizeof(zc) * ne), *cw = malloc(sizeof(zc) * ne);
     zc *cq = malloc(sizeof(zc) * ne), *ctin = malloc(sizeof(zc) * 2 * ne), *ctout = malloc(sizeof(zc) * 2 * ne);
     double enorm = creal(zdotc(ne, pnp, pnp));
@@ -1068,9 +1103,17 @@ void orc_qtot(const orc_cluster *c, const double *qeff, double *qtot)
 int orc_fixedorsoln(const orc_cluster *c, int use_fft, double alpha, double sinc, int dir, double eps, int niter,
                     zc *amnp, double *qeff, double *maxerr, int *maxiter)
 {
+    return orc_fixedorsoln_ex(c, use_fft, alpha, sinc, dir, eps, niter, NULL, amnp, qeff, maxerr, maxiter);
+}
+
+/* the same with excited_spheres (NULL = all); niter = 0 solves nothing
+ * (:513-522), niter < 0 is cbicg's order-of-scattering series        */
+int orc_fixedorsoln_ex(const orc_cluster *c, int use_fft, double alpha, double sinc, int dir, double eps, int niter,
+                       const int *excited, zc *amnp, double *qeff, double *maxerr, int *maxiter)
+{
     int ne = c->neqns, istat = 0;
     zc *pmnp = malloc(sizeof(zc) * 2 * ne), *pan = malloc(sizeof(zc) * ne);
-    orc_planewave(c, alpha, sinc, dir, pmnp);
+    orc_planewave_ex(c, alpha, sinc, dir, excited, pmnp);
     for (int p = 0; p < 2; p++) {
         /* istat, maxiter, maxerr are reset for each polarization (:471-475),
          * so they report the second solve only */
@@ -1080,7 +1123,7 @@ int orc_fixedorsoln(const orc_cluster *c, int use_fft, double alpha, double sinc
         orc_mie_mult(c, 1, 0, pmnp + p * ne, pan);
         memcpy(amnp + p * ne, pan, sizeof(zc) * ne);
         double serr = 0.0;
-        int iter = orc_cbicg(c, use_fft, niter, eps, pan, amnp + p * ne, &serr);
+        int iter = niter != 0 ? orc_cbicg(c, use_fft, niter, eps, pan, amnp + p * ne, &serr) : 0;
         if (iter > *maxiter) *maxiter = iter;
         if (serr > *maxerr) *maxerr = serr;
         if (iter > niter || serr > eps) istat = 1;
@@ -1108,12 +1151,28 @@ static int check235(int n)
 int orc_node_selection(orc_cluster *c, double fva, int node_order_in, int neighbor_model, int *cell_dim,
                        double *d_cell, int *node_order)
 {
+    return orc_node_selection_ex(c, &fva, NULL, NULL, 0, d_cell, node_order_in, neighbor_model, cell_dim,
+                                 node_order);
+}
+
+/* the same with node_selection's optional arguments: target box (NULL:
+ * sphere centre bounds), d_specified (then *d_cell is the cell size in);
+ * fva in/out as in the reference (set to the fraction used)            */
+int orc_node_selection_ex(orc_cluster *c, double *fva_io, const double *target_min, const double *target_max,
+                          int d_specified, double *d_cell, int node_order_in, int neighbor_model, int *cell_dim,
+                          int *node_order)
+{
     double tmin[3] = {1e10, 1e10, 1e10}, tmax[3] = {-1e10, -1e10, -1e10};
     for (int i = 0; i < c->n; i++)
         for (int m = 0; m < 3; m++) {
             tmin[m] = fmin(tmin[m], c->pos[3 * i + m]);
             tmax[m] = fmax(tmax[m], c->pos[3 * i + m]);
         }
+    for (int m = 0; m < 3; m++) {
+        if (target_min) tmin[m] = target_min[m];
+        if (target_max) tmax[m] = target_max[m];
+    }
+    const double fva = *fva_io;
     double amean = 0.0, svol = 0.0;
     for (int i = 0; i < c->n; i++) {
         amean += c->rad[i];
@@ -1123,16 +1182,21 @@ int orc_node_selection(orc_cluster *c, double fva, int node_order_in, int neighb
     amean /= c->n;
     double tvol = 1.0;
     for (int m = 0; m < 3; m++) tvol *= fmax(tmax[m] - tmin[m], amean);
-    double fv;
-    if (fva <= 0.0) {
-        fv = 4.0 * PI_R4 / 3.0 * svol * svol * svol / tvol;
-        fv = fmin(fv, 1.0);
-        fv = fmax(fv, 0.02);
-    } else
-        fv = fva;
-    double d = pow(4.0 * PI_R4 / 3.0 / fv / (double)c->n, 1.0 / 3.0) * svol;
+    double d = *d_cell;
+    if (!d_specified) {
+        double fv;
+        if (fva <= 0.0) {
+            fv = 4.0 * PI_R4 / 3.0 * svol * svol * svol / tvol;
+            fv = fmin(fv, 1.0);
+            fv = fmax(fv, 0.02);
+        } else
+            fv = fva;
+        *fva_io = fv;
+        d = pow(4.0 * PI_R4 / 3.0 / fv / (double)c->n, 1.0 / 3.0) * svol;
+    }
     for (int m = 0; m < 3; m++) {
         int nd = (int)ceil((tmax[m] - tmin[m]) / d);
+        if (nd < 1) return -1; /* correctn235 would not end on 0 */
         while (!check235(nd)) nd++;
         c->cell_dim[m] = nd;
     }

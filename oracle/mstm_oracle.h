@@ -58,16 +58,22 @@ void orc_external_to_external(const orc_cluster *c, int nrhs, const int *contran
 void orc_sphereinteraction(const orc_cluster *c, int nrhs, const int *contran, const int *miemult,
                            int use_fft, const zc *ain, zc *aout);
 void orc_planewave(const orc_cluster *c, double alpha, double sinc, int dir, zc *pmnp);
+void orc_planewave_ex(const orc_cluster *c, double alpha, double sinc, int dir, const int *excited, zc *pmnp);
 int orc_cbicg(const orc_cluster *c, int use_fft, int niter, double eps, const zc *pnp, zc *anp,
               double *errmax);
 void orc_qeff(const orc_cluster *c, const zc *amnp, const zc *pmnp, double *qeff);
 void orc_qtot(const orc_cluster *c, const double *qeff, double *qtot);
 int orc_fixedorsoln(const orc_cluster *c, int use_fft, double alpha, double sinc, int dir,
                     double eps, int niter, zc *amnp, double *qeff, double *maxerr, int *maxiter);
+int orc_fixedorsoln_ex(const orc_cluster *c, int use_fft, double alpha, double sinc, int dir, double eps, int niter,
+                       const int *excited, zc *amnp, double *qeff, double *maxerr, int *maxiter);
 
 /* FFT node grid */
 int orc_node_selection(orc_cluster *c, double fva, int node_order_in, int neighbor_model,
                        int *cell_dim, double *d_cell, int *node_order);
+int orc_node_selection_ex(orc_cluster *c, double *fva_io, const double *target_min, const double *target_max,
+                          int d_specified, double *d_cell, int node_order_in, int neighbor_model, int *cell_dim,
+                          int *node_order);
 void orc_fft_external_to_external(const orc_cluster *c, int nrhs, const int *contran,
                                   const zc *ain, zc *gout);
 void orc_node_scatter(const orc_cluster *c, const zc *asphere, zc *anode);

@@ -48,6 +48,10 @@ def lib():
         L.orc_fixedorsoln.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, vp, dp, dp,
                                       ip]
         L.orc_node_selection.argtypes = [vp, C.c_double, C.c_int, C.c_int, ip, dp, ip]
+        L.orc_node_selection_ex.argtypes = [vp, dp, dp, dp, C.c_int, dp, C.c_int, C.c_int, ip, ip]
+        L.orc_planewave_ex.argtypes = [vp, C.c_double, C.c_double, C.c_int, ip, vp]
+        L.orc_fixedorsoln_ex.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, ip, vp,
+                                         dp, dp, ip]
         L.orc_fft_external_to_external.argtypes = [vp, C.c_int, ip, vp, vp]
         L.orc_local_pairs.argtypes = [vp, ip]
         L.orc_rotcoef.argtypes = [C.c_double, C.c_int, C.c_int, dp]
@@ -144,9 +148,10 @@ class OracleCluster:
         lib().orc_sphereinteraction(self.h, nrhs, _i(ct), _i(mm), int(use_fft), ain.ctypes.data, out.ctypes.data)
         return out
 
-    def planewave(self, alpha=0.0, sinc=0.0, direction=1):
+    def planewave(self, alpha=0.0, sinc=0.0, direction=1, excited=None):
         out = np.zeros((2, self.neqns), np.complex128)
-        lib().orc_planewave(self.h, alpha, sinc, direction, out.ctypes.data)
+        ex = None if excited is None else np.ascontiguousarray(excited, np.int32)
+        lib().orc_planewave_ex(self.h, alpha, sinc, direction, None if ex is None else _i(ex), out.ctypes.data)
         return out
 
     def cbicg(self, pnp, anp0, niter=10000, eps=1e-6, use_fft=False):
@@ -156,14 +161,15 @@ class OracleCluster:
         it = lib().orc_cbicg(self.h, int(use_fft), niter, eps, pnp.ctypes.data, anp.ctypes.data, _d(err))
         return anp, it, float(err[0])
 
-    def fixedorsoln(self, alpha=0.0, sinc=0.0, eps=1e-6, niter=10000, use_fft=False, direction=1):
+    def fixedorsoln(self, alpha=0.0, sinc=0.0, eps=1e-6, niter=10000, use_fft=False, direction=1, excited=None):
         amnp = np.zeros((2, self.neqns), np.complex128)
         qeff = np.zeros((self.n, 3, 3))
         maxerr = np.zeros(1)
         maxiter = np.zeros(1, np.int32)
-        istat = lib().orc_fixedorsoln(self.h, int(use_fft), alpha, sinc, direction, eps, niter, amnp.ctypes.data,
-                                      _d(qeff),
-                                      _d(maxerr), _i(maxiter))
+        ex = None if excited is None else np.ascontiguousarray(excited, np.int32)
+        istat = lib().orc_fixedorsoln_ex(self.h, int(use_fft), alpha, sinc, direction, eps, niter,
+                                         None if ex is None else _i(ex), amnp.ctypes.data, _d(qeff), _d(maxerr),
+                                         _i(maxiter))
         return dict(amnp=amnp, qeff=qeff, maxerr=float(maxerr[0]), iterations=int(maxiter[0]), istat=istat)
 
     def qeff(self, amnp, pmnp):
@@ -178,12 +184,20 @@ class OracleCluster:
         lib().orc_qtot(self.h, _d(np.ascontiguousarray(qeff)), _d(q))
         return q
 
-    def node_selection(self, fva=0.0, node_order=-1, neighbor_model=2):
+    def node_selection(self, fva=0.0, node_order=-1, neighbor_model=2, target_min=None, target_max=None,
+                       d_specified=False, d_cell=0.0):
         cd = np.zeros(3, np.int32)
-        d = np.zeros(1)
+        d = np.array([float(d_cell)])
+        fv = np.array([float(fva)])
         no = np.zeros(1, np.int32)
-        lib().orc_node_selection(self.h, fva, node_order, neighbor_model, _i(cd), _d(d), _i(no))
-        self.fft = dict(cell_dim=cd.copy(), d_cell=float(d[0]), node_order=int(no[0]))
+        tmn = None if target_min is None else np.ascontiguousarray(target_min, np.float64)
+        tmx = None if target_max is None else np.ascontiguousarray(target_max, np.float64)
+        rc = lib().orc_node_selection_ex(self.h, _d(fv), None if tmn is None else _d(tmn),
+                                         None if tmx is None else _d(tmx), int(bool(d_specified)), _d(d),
+                                         node_order, neighbor_model, _i(cd), _i(no))
+        if rc != 0:
+            raise ValueError("node_selection: the target box has no extent along an axis")
+        self.fft = dict(cell_dim=cd.copy(), d_cell=float(d[0]), node_order=int(no[0]), fva=float(fv[0]))
         return self.fft
 
     def translation_orders(self, origin=None, eps=1e-5, maxt=100):

@@ -49,6 +49,14 @@
       integer :: tcap
       character(len=1) :: method
       logical :: ranorient
+! excite_frac > 0: excited_spheres = distance from the mean position <=
+! excite_frac times the largest (an excitation_radius run,
+! mstm-input-37.f90:1264-1294); dspec / dcell_in: node_selection's
+! d_specified; tbox_pad > 0: target box = sphere-centre bounds widened by
+! tbox_pad on every side (a random-configuration target box)
+      real(8) :: excite_frac,dcell_in,tbox_pad,rex(3),rexmax
+      logical :: dspec
+      logical, allocatable :: exsph(:)
       character(len=30) :: tfile30
       character(len=128) :: tfile128
       real(8), allocatable :: rsm(:,:,:),rsmcf(:,:,:)
@@ -58,7 +66,8 @@
       real(8), allocatable :: qeff(:,:,:),qvabs(:,:)
       real(8) :: qtot(3,3)
       namelist /job/ mode,posfile,nsphere,mie_eps,use_fft,cell_vf,node_order_in,nbr_model, &
-         eps,niter,alpha,sinc,infile,outprefix,reps,store_tables,rmed,observe,teps,conveps,tcap,method,ranorient
+         eps,niter,alpha,sinc,infile,outprefix,reps,store_tables,rmed,observe,teps,conveps,tcap,method,ranorient, &
+         excite_frac,dspec,dcell_in,tbox_pad
 
       mode='setup'
       posfile='cluster.pos'
@@ -83,6 +92,10 @@
       tcap=0
       method='i'
       ranorient=.false.
+      excite_frac=0.d0
+      dspec=.false.
+      dcell_in=0.d0
+      tbox_pad=0.d0
       call get_command_argument(1,jobfile)
       open(10,file=trim(jobfile),status='old')
       read(10,nml=job)
@@ -114,12 +127,13 @@
       fft_translation_option=use_fft
       if(use_fft) then
          cell_volume_fraction=cell_vf
-         tmin=sphere_min_position
-         tmax=sphere_max_position
+         tmin=sphere_min_position-tbox_pad
+         tmax=sphere_max_position+tbox_pad
          call clear_fft_matrix(clear_h=.true.)
          neighbor_node_model=nbr_model
+         if(dspec) d_cell=dcell_in
          call node_selection(cell_volume_fraction,target_min=tmin,target_max=tmax, &
-            d_specified=.false.,local_host=0)
+            d_specified=dspec,local_host=0)
          if(node_order_in.le.0) then
             node_order=-node_order_in+ceiling(d_cell)
          else
@@ -190,7 +204,24 @@
             enddo
             call qefficiencyfactors(nsphere,2,amnp,pmnp0,qeff)
          else
-            call fixedorsoln(alpha,sinc,1,eps,niter,amnp,qeff,3,maxerr,maxiter,0,istat, &
+            allocate(exsph(nsphere))
+            exsph=.true.
+            if(excite_frac.gt.0.d0) then
+               rex=0.d0
+               do i=1,nsphere
+                  rex=rex+sphere_position(:,i)
+               enddo
+               rex=rex/dble(nsphere)
+               rexmax=0.d0
+               do i=1,nsphere
+                  rexmax=max(rexmax,sqrt(sum((sphere_position(:,i)-rex)**2)))
+               enddo
+               do i=1,nsphere
+                  exsph(i)=sqrt(sum((sphere_position(:,i)-rex)**2)).le.excite_frac*rexmax
+               enddo
+               write(20,'(a,i10)') 'excited ',count(exsph)
+            endif
+            call fixedorsoln(alpha,sinc,1,eps,niter,amnp,qeff,3,maxerr,maxiter,0,istat,excited_spheres=exsph, &
                solution_method=method,initialize_solver=.true.)
          endif
          t1=mstm_mpi_wtime()

@@ -4,7 +4,14 @@
 ! built by the reference's own setup calls (as main_calling_program does);
 ! then the reference CPU procedures and their hip_* replacements run on the
 ! same state and the results are compared.  Prints one line per check and
-! exits with status 1 on a mismatch.  Usage: host_integration <posfile> <n>
+! exits with status 1 on a mismatch.
+!   host_integration <posfile> <n>       direct operator: fixedorsoln (full
+!        and partial excitation, order of scattering), sphereinteraction
+!        (rhs_list, skip_external_translation), cbicg, direct_solver before
+!        and after a geometry change, observables, tmatrix_solution
+!   host_integration <posfile> <n> fft   node-grid operator: node_selection
+!        (automatic and specified d_cell), sphereinteraction,
+!        fft_external_to_external_expansion, cbicg, clear_fft_matrix
 !
       program host_integration
       use mpidefs
@@ -19,7 +26,7 @@
       use solver
       use mstm_hip_solver
       implicit none
-      character(len=256) :: posfile,arg
+      character(len=256) :: posfile,arg,mode
       integer :: n,i,iref,igpu,istat_r,istat_g,mi_r,mi_g,bad
       real(8) :: line(6),err_r,err_g,d1,d2,d3
       complex(8), allocatable :: amnp_r(:,:),amnp_g(:,:),ain(:,:),ao_r(:,:),ao_g(:,:),b(:),x_r(:),x_g(:)
@@ -27,6 +34,8 @@
       call get_command_argument(1,posfile)
       call get_command_argument(2,arg)
       read(arg,*) n
+      mode='direct'
+      if(command_argument_count().ge.3) call get_command_argument(3,mode)
       call mstm_mpi(mpi_command='init')
       number_spheres=n
       allocate(sphere_radius(n),sphere_position(3,n),sphere_ref_index(2,0:n),host_sphere(n), &
@@ -48,6 +57,15 @@
       call init(max_mie_order)
       call incident_field_initialization(0.d0,0.d0,1)
       bad=0
+      if(trim(mode).eq.'fft') then
+         call fft_checks()
+         if(bad.ne.0) then
+            write(*,'(a)') 'HOST INTEGRATION FAILED'
+            call exit(1)
+         endif
+         write(*,'(a)') 'HOST INTEGRATION OK'
+         stop
+      endif
       ! fixedorsoln: reference CPU vs GPU
       allocate(amnp_r(number_eqns,2),amnp_g(number_eqns,2),q_r(3,3,n),q_g(3,3,n))
       call fixedorsoln(0.d0,0.d0,1,1.d-6,10000,amnp_r,q_r,3,err_r,mi_r,0,istat_r, &
@@ -84,6 +102,10 @@
       ! merge / scatteringmatrix / fosmexpansion against the hip_* forms
       call observables_check()
       call tmatrix_check()
+      call excitation_check()
+      call order_of_scattering_check()
+      call rhs_list_check()
+      call direct_solver_check()
       if(bad.ne.0) then
          write(*,'(a)') 'HOST INTEGRATION FAILED'
          call exit(1)
@@ -191,4 +213,154 @@
             h_r(1),h_g(1),nv,e1,e2
          if(h_r(1).ne.h_g(1).or.nv.le.0.or.st_r.ne.st_g.or.e1.gt.1.d-9.or.e2.gt.1.d-10) bad=1
          end subroutine tmatrix_check
+!
+! fixedorsoln with excited_spheres from an excitation radius about the mean
+! position, as main_calling_program sets sphere_excitation_switch
+! (mstm-input-37.f90:1264-1294): radius 0.8 of the largest distance
+!
+         subroutine excitation_check()
+         logical :: ex(n)
+         real(8) :: r0(3),rt(n),e1,e2
+         integer :: j
+         r0=0.d0
+         do j=1,n
+            r0=r0+sphere_position(:,j)
+         enddo
+         r0=r0/dble(n)
+         do j=1,n
+            rt(j)=sqrt(sum((sphere_position(:,j)-r0)**2))
+         enddo
+         ex=rt.le.0.8d0*maxval(rt)
+         call fixedorsoln(0.3d0,0.4d0,1,1.d-6,10000,amnp_r,q_r,3,err_r,mi_r,0,istat_r, &
+            excited_spheres=ex,solution_method='i',initialize_solver=.true.)
+         call hip_fixedorsoln(0.3d0,0.4d0,1,1.d-6,10000,amnp_g,q_g,3,err_g,mi_g,0,istat_g, &
+            excited_spheres=ex,solution_method='i',initialize_solver=.false.)
+         e1=maxval(abs(amnp_r-amnp_g))/maxval(abs(amnp_r))
+         e2=maxval(abs(q_r-q_g))/maxval(abs(q_r))
+         write(*,'(a,i4,a,i4,2i6,2es12.4)') 'fixedorsoln excited ',count(ex),' of',n, &
+            mi_r,mi_g,e1,e2
+         if(mi_r.ne.mi_g.or.istat_r.ne.istat_g.or..not.(e1.le.1.d-10.and.e2.le.1.d-10).or.count(ex).eq.n) bad=1
+         end subroutine excitation_check
+!
+! max_iterations < 0: cbicg's order-of-scattering series, through
+! fixedorsoln and through cbicg itself (from a start other than b)
+!
+         subroutine order_of_scattering_check()
+         complex(8) :: p(number_eqns),xr(number_eqns),xg(number_eqns)
+         real(8) :: e1,e2
+         call fixedorsoln(0.d0,0.d0,1,1.d-8,-40,amnp_r,q_r,3,err_r,mi_r,0,istat_r, &
+            solution_method='i',initialize_solver=.true.)
+         call hip_fixedorsoln(0.d0,0.d0,1,1.d-8,-40,amnp_g,q_g,3,err_g,mi_g,0,istat_g, &
+            solution_method='i',initialize_solver=.false.)
+         e1=maxval(abs(amnp_r-amnp_g))/maxval(abs(amnp_r))
+         e2=abs(err_r-err_g)/max(abs(err_r),1.d-300)
+         write(*,'(a,4i6,2es12.4)') 'order of scattering fixedorsoln iterations(ref,gpu) istat(ref,gpu) rel_err: ', &
+            mi_r,mi_g,istat_r,istat_g,e1,e2
+         if(mi_r.ne.mi_g.or.istat_r.ne.istat_g.or..not.(e1.le.1.d-10.and.e2.le.1.d-8)) bad=1
+         call multmiecoeffmult(number_eqns,1,1,ain(:,2),p)
+         xr=ain(:,1)
+         xg=ain(:,1)
+         call cbicg(-7,1.d-30,p,xr,0,iref,err_r,initialize_solver=.true.)
+         call hip_cbicg(-7,1.d-30,p,xg,0,igpu,err_g,initialize_solver=.false.)
+         e1=maxval(abs(xr-xg))/maxval(abs(xr))
+         write(*,'(a,2i6,2es12.4,es12.4)') 'order of scattering cbicg iterations(ref,gpu) errmax(ref,gpu) rel_err: ', &
+            iref,igpu,err_r,err_g,e1
+         if(iref.ne.igpu.or.err_r.ne.0.d0.or.err_g.ne.0.d0.or..not.(e1.le.1.d-10)) bad=1
+         end subroutine order_of_scattering_check
+!
+! sphereinteraction's rhs_list and skip_external_translation
+!
+         subroutine rhs_list_check()
+         real(8) :: e1,e2
+         ao_r=(1.d0,1.d0)
+         ao_g=(1.d0,1.d0)
+         call sphereinteraction(number_eqns,2,ain,ao_r,rhs_list=(/.false.,.true./),con_tran=(/.false.,.true./))
+         call hip_sphereinteraction(number_eqns,2,ain,ao_g,rhs_list=(/.false.,.true./),con_tran=(/.false.,.true./))
+         e1=maxval(abs(ao_r-ao_g))/maxval(abs(ao_r))
+         call sphereinteraction(number_eqns,2,ain,ao_r,skip_external_translation=.true.)
+         call hip_sphereinteraction(number_eqns,2,ain,ao_g,skip_external_translation=.true.)
+         e2=maxval(abs(ao_r-ao_g))
+         write(*,'(a,2es12.4)') 'sphereinteraction rhs_list rel_err, skip_external_translation abs_err: ',e1,e2
+         if(.not.(e1.le.1.d-10).or.e2.ne.0.d0.or.maxval(abs(ao_r)).ne.0.d0) bad=1
+         end subroutine rhs_list_check
+!
+! direct_solver (LU of the dense interaction matrix) against hip_direct_solver,
+! then after a geometry change (sphere 1 moved outward, the host's Mie data
+! made again) with initialize_solver, and a reuse of the factors
+!
+         subroutine direct_solver_check()
+         complex(8) :: p(number_eqns),xr(number_eqns),xg(number_eqns)
+         real(8) :: e1,e2,e3,s_r,s_g
+         call multmiecoeffmult(number_eqns,1,1,ain(:,1),p)
+         call direct_solver(p,xr,initialize_solver=.true.,solution_error=s_r)
+         call hip_direct_solver(p,xg,initialize_solver=.true.,solution_error=s_g)
+         e1=maxval(abs(xr-xg))/maxval(abs(xr))
+         sphere_position(:,1)=sphere_position(:,1)*1.25d0
+         call sphere_layer_initialization()
+         call miecoefcalc(1.d-6)
+         call multmiecoeffmult(number_eqns,1,1,ain(:,2),p)
+         call direct_solver(p,xr,initialize_solver=.true.,solution_error=s_r)
+         call hip_direct_solver(p,xg,initialize_solver=.true.,solution_error=s_g)
+         e2=maxval(abs(xr-xg))/maxval(abs(xr))
+         call multmiecoeffmult(number_eqns,1,1,ain(:,1),p)
+         call direct_solver(p,xr,solution_error=s_r)
+         call hip_direct_solver(p,xg,solution_error=s_g)
+         e3=maxval(abs(xr-xg))/maxval(abs(xr))
+         write(*,'(a,3es12.4)') 'direct_solver rel_err (first, after geometry change, factors reused): ',e1,e2,e3
+         if(.not.(e1.le.1.d-10.and.e2.le.1.d-10.and.e3.le.1.d-10)) bad=1
+         end subroutine direct_solver_check
+!
+! the node-grid (FFT) operator through the shim: hip_node_selection in place
+! of node_selection (automatic cell size; then, after hip_clear_fft_matrix, a
+! specified d_cell 1.3 times larger).  The reference's serial FFT path sizes
+! its saved node arrays by the first call's nrhs (fft_translation-5.f90:
+! 190-194), so each grid starts with a 2-column initial call and the solves
+! run cbicg with initialize_solver=.false..  The reference keeps a complex(4)
+! node-kernel spectrum: products agree to 1e-5.
+!
+         subroutine fft_checks()
+         real(8) :: tmin(3),tmax(3),e1,e2,e3,dsp
+         integer :: pass,cd0(3)
+         allocate(ain(number_eqns,2),ao_r(number_eqns,2),ao_g(number_eqns,2))
+         allocate(b(number_eqns),x_r(number_eqns),x_g(number_eqns))
+         do i=1,number_eqns
+            ain(i,1)=dcmplx(sin(0.37d0*i),cos(0.11d0*i))
+            ain(i,2)=dcmplx(cos(0.23d0*i),sin(0.05d0*i))
+         enddo
+         fft_translation_option=.true.
+         neighbor_node_model=2
+         do pass=1,2
+            tmin=sphere_min_position
+            tmax=sphere_max_position
+            cell_volume_fraction=0.d0
+            call hip_clear_fft_matrix(clear_h=.true.)
+            if(pass.eq.1) then
+               call hip_node_selection(cell_volume_fraction,target_min=tmin,target_max=tmax, &
+                  d_specified=.false.,local_host=0)
+               cd0=cell_dim
+               dsp=1.3d0*d_cell
+            else
+               d_cell=dsp
+               call hip_node_selection(cell_volume_fraction,target_min=tmin,target_max=tmax, &
+                  d_specified=.true.,local_host=0)
+            endif
+            node_order=1+ceiling(d_cell)
+            call sphereinteraction(number_eqns,2,ain,ao_r,initial_run=.true.,con_tran=(/.false.,.true./))
+            call hip_sphereinteraction(number_eqns,2,ain,ao_g,initial_run=.true.,con_tran=(/.false.,.true./))
+            e1=maxval(abs(ao_r-ao_g))/maxval(abs(ao_r))
+            call fft_external_to_external_expansion(number_eqns,2,ain,ao_r,con_tran=(/.false.,.true./))
+            call hip_fft_external_to_external_expansion(number_eqns,2,ain,ao_g,con_tran=(/.false.,.true./))
+            e2=maxval(abs(ao_r-ao_g))/maxval(abs(ao_r))
+            call multmiecoeffmult(number_eqns,1,1,ain(:,1),b)
+            x_r=b
+            x_g=b
+            call cbicg(10000,1.d-6,b,x_r,0,iref,err_r,initialize_solver=.false.)
+            call hip_cbicg(10000,1.d-6,b,x_g,0,igpu,err_g,initialize_solver=.false.)
+            e3=maxval(abs(x_r-x_g))/maxval(abs(x_r))
+            write(*,'(a,i2,3i4,es14.6,i4,a,3es12.4,2i6)') 'fft pass',pass,cell_dim,d_cell,node_order, &
+               ' rel_err(sphereinteraction,fft_expansion,cbicg) iterations(ref,gpu): ',e1,e2,e3,iref,igpu
+            if(.not.(e1.le.1.d-5.and.e2.le.1.d-5.and.e3.le.1.d-5).or.iref.ne.igpu) bad=1
+         enddo
+         if(all(cell_dim.eq.cd0)) bad=1  ! the specified d_cell must give another grid
+         end subroutine fft_checks
       end program host_integration

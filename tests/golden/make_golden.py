@@ -35,6 +35,12 @@ Cases
   lu_cfg1, lu_poly30  the clusters of cfg1_direct and poly30 through the
                 direct (LU) branch of fixedorsoln (solution_method 'd'):
                 solution and efficiencies
+  boundary_cfg1 the cluster of cfg1_direct through fixedorsoln's optional
+                branches: order of scattering (max_iterations -40, converged;
+                -5, run out), max_iterations 0, an excitation radius (0.8 of
+                the largest distance from the mean position, oblique
+                incidence); and node_selection with the automatic cell size,
+                a specified d_cell (2x), a target box widened by 3 and both
 Every solve also runs the single-origin post-processing (translation orders,
 merge_to_common_origin, scatteringmatrix on the default grid,
 fosmexpansion); those outputs are stored under obs_* keys.
@@ -128,6 +134,51 @@ def tmatrix_case(name, pos, radius, ri, eps, conveps, niter=10000):
     print(name, "T", res["T_in"], "->", res["T"], "orders", sorted(set(res["order"].tolist())), "tmat", res["tmat"].size)
 
 
+def excited_mask(pos, frac):
+    """excited_spheres of an excitation radius frac * (largest distance from
+    the mean position), as the harness forms it (excite_frac)."""
+    d = np.sqrt(((pos - pos.mean(axis=0)) ** 2).sum(axis=1))
+    return d <= frac * d.max()
+
+
+def boundary_case(name, pos, radius, ri):
+    """fixedorsoln's optional branches and node_selection's optional
+    arguments through the reference: the order-of-scattering series
+    (max_iterations < 0; converged and run out), no solve (0), an excitation
+    radius, and the node grid with a specified cell size and a wider target
+    box."""
+    ri = np.asarray(ri, np.complex128)
+    res = dict(pos=pos, radius=radius, ri=ri)
+    for tag, kw in (("oos", dict(niter=-40, eps=1e-8)), ("oos_short", dict(niter=-5, eps=1e-8)),
+                    ("noiter", dict(niter=0)), ("excited", dict(excite_frac=0.8, alpha=0.3, sinc=0.4))):
+        r = O.run_reference(pos, radius, ri, "solve", **kw)
+        ne = int(r["neqns"])
+        res.update({f"{tag}_amnp": r["amnp"].reshape(2, ne), f"{tag}_qeff": r["qeff"].reshape(-1, 3, 3),
+                    f"{tag}_iterations": int(r["iterations"]), f"{tag}_maxerr": float(r["maxerr"]),
+                    f"{tag}_istat": int(r["istat"])})
+        if "excited" in r:
+            res["excited_count"] = int(r["excited"])
+    auto = O.run_reference(pos, radius, ri, "setup", use_fft=True)
+    res.update(grid_auto_cell_dim=np.array(auto["cell_dim"], np.int32), grid_auto_d_cell=float(auto["d_cell"]),
+               grid_auto_fva=float(auto["cell_volume_fraction"]))
+    dsp = 2.0 * float(auto["d_cell"])
+    for tag, kw in (("dspec", dict(dspec=True, dcell_in=dsp)), ("box", dict(tbox_pad=3.0)),
+                    ("box_dspec", dict(tbox_pad=3.0, dspec=True, dcell_in=dsp))):
+        g = O.run_reference(pos, radius, ri, "setup", use_fft=True, **kw)
+        res.update({f"grid_{tag}_cell_dim": np.array(g["cell_dim"], np.int32), f"grid_{tag}_d_cell": float(g["d_cell"]),
+                    f"grid_{tag}_node_order": int(g["node_order"]),
+                    f"grid_{tag}_fva": float(g["cell_volume_fraction"])})
+    res["grid_dspec_in"] = dsp
+    res["grid_box_pad"] = 3.0
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **res)
+    print(name, {k: v for k, v in res.items() if np.ndim(v) == 0})
+
+
+def boundary_only():
+    c = G.rsa_in_sphere(20, 2.0, 0.10, seed=1, ri=1.5 + 0.01j)
+    boundary_case("boundary_cfg1", c.pos, c.radius, c.ri)
+
+
 def main():
     c = G.rsa_in_sphere(20, 2.0, 0.10, seed=1, ri=1.5 + 0.01j)
     case("cfg1_direct", c.pos, c.radius, c.ri, seed=11)
@@ -150,6 +201,7 @@ def main():
     radius = np.linspace(0.3, 1.6, c.n)
     tmatrix_case("tmat_poly6", c.pos, radius, c.ri, eps=1e-8, conveps=1e-7)
     lu_only()
+    boundary_only()
 
 
 def lu_only():
@@ -169,4 +221,4 @@ def tmatrix_only():
 
 
 if __name__ == "__main__":
-    {"tmatrix": tmatrix_only, "lu": lu_only}.get(sys.argv[1] if len(sys.argv) > 1 else "", main)()
+    {"tmatrix": tmatrix_only, "lu": lu_only, "boundary": boundary_only}.get(sys.argv[1] if len(sys.argv) > 1 else "", main)()
