@@ -551,9 +551,10 @@ int mstm_hip_node_selection(mstm_hip_ctx *c, double *fva, const double *target_m
                   c->offset.data(), c->rimed,   c->d_pos,   c->d_order,    c->d_offset,      c->d_an,
                   c->d_vcc,         c->vcc_LM};
     int err = 0;
+    const char *why = "node_selection failed";
     FftOp *f = fft_create(g, fva, target_min, target_max, d_specified, node_order, neighbor_node_model, cell_dim,
-                          d_cell, node_order_out, &err);
-    if (!f) return fail(c, err ? err : MSTM_HIP_EHIP, "node_selection failed");
+                          d_cell, node_order_out, &err, &why);
+    if (!f) return fail(c, err ? err : MSTM_HIP_EHIP, "%s", why);
     c->fft = f;
     return 0;
 }
@@ -638,6 +639,10 @@ static int matvec(mstm_hip_ctx *c, int nrhs, const double *ain, double *aout, co
 {
     if (use_fft) {
         if (!c->fft) return fail(c, MSTM_HIP_ESTATE, "FFT operator not set up (call node_selection + setup_fft)");
+        if (!fft_ready(c->fft, prec)) {  // first product at this precision: set up as mstm_hip_setup_fft does
+            int rc = mstm_hip_setup_fft(c, prec);
+            if (rc) return rc;
+        }
         const size_t V = 2 * (size_t)c->neqns;
         for (int r0 = 0; r0 < nrhs; r0 += 2) {  // the node-grid operator takes two columns per call
             const int nr = std::min(2, nrhs - r0);

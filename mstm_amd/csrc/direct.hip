@@ -83,8 +83,11 @@ template <typename T, int L> constexpr bool tile_staged() { return direct_sym(L,
 // two-column FP32 tiles up to order 4 keep the tile's two source blocks in
 // LDS, and software-pipeline the staging: step s+1's group is loaded into
 // registers while step s computes
-template <typename T, int L, int NR> constexpr bool tile_src_staged() { return sizeof(T) == 4 && L <= 4 && NR == 2; }
-template <typename T, int L, int NR> constexpr bool tile_pipelined() { return tile_src_staged<T, L, NR>(); }
+template <typename T, int L, int NR> constexpr bool tile_src_staged() { return L <= 4; }
+template <typename T, int L, int NR> constexpr bool tile_pipelined()
+{
+    return sizeof(T) == 4 && NR == 2 && tile_src_staged<T, L, NR>();
+}
 // occupancy target: FP32 tiles up to order 4 fit 2 waves/SIMD without
 // spilling (the pipelined form holds its prefetch in the full register file
 // at 1 wave/SIMD instead); larger states use the full register file
@@ -131,7 +134,7 @@ template <typename T, int L> constexpr int tile_lds_elems()
 #define SRC_PAD 2
 template <typename T, int L, int NR> constexpr int tile_src_elems()
 {
-    return tile_src_staged<T, L, NR>() ? 2 * MSTM_SLOTS * (4 * L * (L + 2) + SRC_PAD) * 2 : 1;
+    return tile_src_staged<T, L, NR>() ? 2 * MSTM_SLOTS * (2 * NR * L * (L + 2) + SRC_PAD) * 2 : 1;
 }
 // copy table group s of a tile into LDS, all threads of the workgroup
 template <typename T, int L, int NR>
@@ -184,7 +187,7 @@ __device__ __forceinline__ void tile_body_sym(int2 bt, int tile, const T *__rest
         }
         const cplx<T> *xs = xin + ((int64_t)(rhs * npad + src) * 2 + mode) * NB;
         // staged copy: block 0 = bt.y (forward sources), 1 = bt.x (reverse sources)
-        const cplx<T> *xl = srcl + (DIR * S + (src % S)) * (4 * NB + SRC_PAD) + ((rhs & 1) * 2 + mode) * NB;
+        const cplx<T> *xl = srcl + (DIR * S + (src % S)) * (2 * NR * NB + SRC_PAD) + (rhs * 2 + mode) * NB;
         __syncthreads();
         cplx<T> xv[PIPE ? NB : 1];
         if constexpr (PIPE) {
@@ -390,7 +393,7 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
         }
         const cplx<T> *xs = xin + ((int64_t)(rhs * npad + src) * 2 + mode) * NB;
         // staged copy: block 0 = bt.y (forward sources), 1 = bt.x (reverse sources)
-        const cplx<T> *xl = srcl + (DIR * S + (src % S)) * (4 * NB + SRC_PAD) + ((rhs & 1) * 2 + mode) * NB;
+        const cplx<T> *xl = srcl + (DIR * S + (src % S)) * (2 * NR * NB + SRC_PAD) + (rhs * 2 + mode) * NB;
         // all waves of the tile read the same table group in step s: keep them
         // in step so that the group is fetched from HBM once (and, staged,
         // copied to LDS once)
@@ -541,11 +544,11 @@ __global__ __launch_bounds__(64 * NR) __attribute__((amdgpu_waves_per_eu(tile_wa
         // the two source blocks of the tile, all threads; visible after the
         // first step's barrier
         constexpr int NB = L * (L + 2), BLK = MSTM_SLOTS * 2 * NB;  // one column r: 16 spheres x 2 modes
-        for (int e = threadIdx.x; e < 2 * 2 * BLK; e += 64 * NR) {
-            const int b = e / (2 * BLK), rem = e - b * 2 * BLK, r = rem / BLK, o = rem - r * BLK;
+        for (int e = threadIdx.x; e < 2 * NR * BLK; e += 64 * NR) {
+            const int b = e / (NR * BLK), rem = e - b * NR * BLK, r = rem / BLK, o = rem - r * BLK;
             const int blk = b == 0 ? bt.y : bt.x;
             const int i = o / (2 * NB), mo = o - i * 2 * NB;  // sphere in block, (mode, coefficient)
-            srcl[(b * MSTM_SLOTS + i) * (4 * NB + SRC_PAD) + r * 2 * NB + mo] =
+            srcl[(b * MSTM_SLOTS + i) * (2 * NR * NB + SRC_PAD) + r * 2 * NB + mo] =
                 xin[((int64_t)(r * npad + blk * MSTM_SLOTS + i) * 2) * NB + mo];
         }
     }

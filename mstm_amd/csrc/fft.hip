@@ -651,7 +651,7 @@ static bool is235(int n)
 
 FftOp *fft_create(const FftGeometry &g, double *fva, const double *target_min, const double *target_max,
                   int d_specified, int node_order, int neighbor_model, int *cell_dim, double *d_cell,
-                  int *node_order_out, int *err)
+                  int *node_order_out, int *err, const char **why)
 {
     FftOp *f = new FftOp();
     f->g = g;
@@ -693,7 +693,7 @@ FftOp *fft_create(const FftGeometry &g, double *fva, const double *target_min, c
     }
     if (!(d > 0.0)) {
         *err = 1;
-        f->err = "node_selection: cell size must be positive";
+        *why = "node_selection: cell size must be positive";
         delete f;
         return nullptr;
     }
@@ -701,7 +701,7 @@ FftOp *fft_create(const FftGeometry &g, double *fva, const double *target_min, c
         int nd = (int)ceil((tmax[m] - tmin[m]) / d);
         if (nd < 1) {  // correctn235 would not end on 0 (fft_translation-5.f90:1068-1076)
             *err = 1;
-            f->err = "node_selection: the target box has no extent along an axis";
+            *why = "node_selection: the target box has no extent along an axis";
             delete f;
             return nullptr;
         }
@@ -715,7 +715,7 @@ FftOp *fft_create(const FftGeometry &g, double *fva, const double *target_min, c
     f->LG = (node_order <= 0) ? -node_order + (int)ceil(d) : node_order;
     if (f->LG > LMAX_LIST || f->Lp > LMAX_LIST) {
         *err = 4;
-        f->err = "node order above the compiled limit";
+        *why = "node_selection: node order above the compiled limit (12)";
         delete f;
         return nullptr;
     }
@@ -796,6 +796,7 @@ void fft_bind(FftOp *f, const FftGeometry &g)
 }
 
 int fft_required_vcc(const FftOp *f) { return f ? std::max(f->LG, f->Lp) : 0; }
+bool fft_ready(const FftOp *f, int prec) { return f && f->ready[prec]; }
 const char *fft_last_error(const FftOp *f) { return f ? f->err.c_str() : "no FFT operator"; }
 double fft_stat(FftOp *f, int which)
 {
@@ -980,6 +981,12 @@ static int build_kernel(FftOp *f, hipStream_t st)
 int fft_setup(FftOp *f, int prec, hipStream_t st)
 {
     if (!f) return 3;
+    // the sphere<->node and near-field tables need the vector-coupling table
+    // up to the node order (the context binds it before setup)
+    if (!f->g.d_vcc || f->g.vcc_LM < fft_required_vcc(f)) {
+        f->err = "FFT set-up without a bound vector-coupling table of the node order";
+        return 3;
+    }
     auto t0 = std::chrono::steady_clock::now();
     const int n = f->nsph, Lp = f->Lp, LG = f->LG;
     const int NBP = Lp * (Lp + 2);
@@ -1164,9 +1171,9 @@ template <typename T> static int matvec_t(FftOp *f, int nrhs, const double *ain,
 int fft_matvec(FftOp *f, int nrhs, const double *ain, double *aout, const int *flags, int prec, hipStream_t st)
 {
     if (!f) return 3;
-    if (!f->ready[prec]) {
-        int rc = fft_setup(f, prec, st);
-        if (rc) return rc;
+    if (!f->ready[prec]) {  // the context sets the operator up (fft_setup after binding its tables)
+        f->err = "FFT operator of this precision not set up";
+        return 3;
     }
     return prec == 0 ? matvec_t<double>(f, nrhs, ain, aout, flags, st) : matvec_t<float>(f, nrhs, ain, aout, flags, st);
 }

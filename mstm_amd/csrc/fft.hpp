@@ -26,7 +26,7 @@ struct FftGeometry {
 // centre bounds), d_cell in (d_specified) / out
 FftOp *fft_create(const FftGeometry &g, double *fva, const double *target_min, const double *target_max,
                   int d_specified, int node_order, int neighbor_model, int *cell_dim, double *d_cell,
-                  int *node_order_out, int *err);
+                  int *node_order_out, int *err, const char **why);
 // the grid of node_selection: cell_boundary[3] (the target box minimum),
 // sphere_node[3 nsph] (1-based cell indices), number of stencil cells - 1
 // (number_neighbor_nodes) and near-field pairs
@@ -43,3 +43,4 @@ const char *fft_last_error(const FftOp *f);
 double fft_stat(FftOp *f, int which);
 void fft_reset_stats(FftOp *f);
 int fft_required_vcc(const FftOp *f);
+bool fft_ready(const FftOp *f, int prec);

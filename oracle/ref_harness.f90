@@ -42,6 +42,8 @@
       character(len=16) :: mode
       character(len=256) :: posfile,infile,outprefix,jobfile
       integer :: nsphere,niter,node_order_in,nbr_model,i,n,ios,istat,maxiter,ncol,reps,r,p
+! idir: incidence from below (1) or above (2), the dir argument of fixedorsoln
+      integer :: idir
       real(8) :: mie_eps,eps,cell_vf,alpha,sinc,vr,maxerr,t0,t1,rmed
       real(8) :: line(6),tmin(3),tmax(3),xg
       logical :: use_fft,store_tables,observe
@@ -67,7 +69,7 @@
       real(8) :: qtot(3,3)
       namelist /job/ mode,posfile,nsphere,mie_eps,use_fft,cell_vf,node_order_in,nbr_model, &
          eps,niter,alpha,sinc,infile,outprefix,reps,store_tables,rmed,observe,teps,conveps,tcap,method,ranorient, &
-         excite_frac,dspec,dcell_in,tbox_pad
+         excite_frac,dspec,dcell_in,tbox_pad,idir
 
       mode='setup'
       posfile='cluster.pos'
@@ -96,6 +98,7 @@
       dspec=.false.
       dcell_in=0.d0
       tbox_pad=0.d0
+      idir=1
       call get_command_argument(1,jobfile)
       open(10,file=trim(jobfile),status='old')
       read(10,nml=job)
@@ -123,7 +126,7 @@
       call sphere_layer_initialization()
       call miecoefcalc(mie_eps)
       call init(max_mie_order)
-      call incident_field_initialization(alpha,sinc,1)
+      call incident_field_initialization(alpha,sinc,idir)
       fft_translation_option=use_fft
       if(use_fft) then
          cell_volume_fraction=cell_vf
@@ -194,7 +197,7 @@
             ain=(0.d0,0.d0)
             call sphereinteraction(number_eqns,2,ain,aout,initial_run=.true., &
                con_tran=(/.false.,.true./))
-            call sphereplanewavecoef(alpha,sinc,1,pmnp0)
+            call sphereplanewavecoef(alpha,sinc,idir,pmnp0)
             do p=1,2
                istat=0
                call multmiecoeffmult(number_eqns,1,1,pmnp0(:,p),pmnpan)
@@ -221,7 +224,7 @@
                enddo
                write(20,'(a,i10)') 'excited ',count(exsph)
             endif
-            call fixedorsoln(alpha,sinc,1,eps,niter,amnp,qeff,3,maxerr,maxiter,0,istat,excited_spheres=exsph, &
+            call fixedorsoln(alpha,sinc,idir,eps,niter,amnp,qeff,3,maxerr,maxiter,0,istat,excited_spheres=exsph, &
                solution_method=method,initialize_solver=.true.)
          endif
          t1=mstm_mpi_wtime()

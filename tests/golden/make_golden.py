@@ -35,6 +35,13 @@ Cases
   lu_cfg1, lu_poly30  the clusters of cfg1_direct and poly30 through the
                 direct (LU) branch of fixedorsoln (solution_method 'd'):
                 solution and efficiencies
+  cfg2_cut250   the 250 lowest spheres of the 1000-sphere x = 3 cylinder
+                workload (seed 2, fv 0.25, m = 1.6+0.02i): FFT operator
+                (grid 8x8x5, node order 9), direct H, FFT solve
+  cfg4_cut300   the first 300 monomers of the 2000-monomer fractal workload
+                (seed 4, x = 0.5, m = 1.75+0.4i): FFT operator (node order
+                4), FFT solve at the workload's first orientation (from above)
+  frac_agg_fft  examples/frac_agg.pos at x = 0.5 through the FFT operator
   boundary_cfg1 the cluster of cfg1_direct through fixedorsoln's optional
                 branches: order of scattering (max_iterations -40, converged;
                 -5, run out), max_iterations 0, an excitation radius (0.8 of
@@ -67,7 +74,7 @@ def rhs(neqns, seed):
     return rng.normal(size=(2, neqns)) + 1j * rng.normal(size=(2, neqns))
 
 
-def case(name, pos, radius, ri, fft=False, solve=True, seed=0):
+def case(name, pos, radius, ri, fft=False, solve=True, seed=0, observe=True, **incidence):
     ri = np.asarray(ri, np.complex128)
     setup = O.run_reference(pos, radius, ri, "setup")
     neqns = int(setup["neqns"])
@@ -86,17 +93,22 @@ def case(name, pos, radius, ri, fft=False, solve=True, seed=0):
     else:
         res["hout"] = mv["hout"].reshape(2, neqns)
     if solve:
-        res.update(solve_case(pos, radius, ri, neqns, fft))
+        res.update(solve_case(pos, radius, ri, neqns, fft, observe, **incidence))
     np.savez_compressed(os.path.join(OUT, name + ".npz"), **res)
-    print(name, "neqns", neqns, "orders", sorted(set(res["order"].tolist())))
+    print(name, "neqns", neqns, "orders", sorted(set(res["order"].tolist())),
+          "grid", (res.get("cell_dim"), res.get("node_order")), "iterations", res.get("iterations"))
 
 
-def solve_case(pos, radius, ri, neqns, fft=False):
-    sv = O.run_reference(pos, radius, ri, "solve", use_fft=fft, observe=True)
+def solve_case(pos, radius, ri, neqns, fft=False, observe=True, **incidence):
+    """incidence: alpha, sinc, idir of fixedorsoln (default +z from below)."""
+    sv = O.run_reference(pos, radius, ri, "solve", use_fft=fft, observe=observe, **incidence)
+    out = dict(amnp=sv["amnp"].reshape(2, neqns), qeff=sv["qeff"].reshape(-1, 3, 3), qtot=sv["qtot"].reshape(3, 3),
+               iterations=int(sv["iterations"]), maxerr=sv["maxerr"], istat=int(sv["istat"]),
+               **{k: v for k, v in incidence.items()})
+    if not observe:
+        return out
     T = int(sv["t_matrix_order"])
-    return dict(amnp=sv["amnp"].reshape(2, neqns), qeff=sv["qeff"].reshape(-1, 3, 3), qtot=sv["qtot"].reshape(3, 3),
-                iterations=int(sv["iterations"]), maxerr=sv["maxerr"], istat=int(sv["istat"]),
-                obs_T=T, obs_torder=sv["torder"], obs_origin=np.array(sv["cluster_origin"]),
+    return dict(**out, obs_T=T, obs_torder=sv["torder"], obs_origin=np.array(sv["cluster_origin"]),
                 obs_amnp0=sv["amnp0"].reshape(2, 2, T * (T + 2)), obs_scatmat=sv["scatmat"].reshape(361, 16),
                 **{"obs_" + k: sv[k].reshape(2 * T + 1, 4, 4) for k in ("s00", "s02", "sp22", "sm22")})
 
@@ -174,6 +186,26 @@ def boundary_case(name, pos, radius, ri):
     print(name, {k: v for k, v in res.items() if np.ndim(v) == 0})
 
 
+def config_cuts():
+    """Pieces of the FFT workloads at their own geometry and optics: the 250
+    lowest spheres (a slab at the full volume fraction) of the 1000-sphere
+    x = 3 cylinder (seed 2, m = 1.6+0.02i), and the first 300 monomers (a
+    connected sub-aggregate) of the 2000-monomer fractal (seed 4, x = 0.5,
+    m = 1.75+0.4i) solved for its first orientation (incidence from above);
+    the reference's own aggregate file through the FFT operator."""
+    cl = G.rsa_in_cylinder(1000, 3.0, 0.25, seed=2, ri=1.6 + 0.02j)
+    idx = np.sort(np.argsort(cl.pos[:, 2])[:250])
+    case("cfg2_cut250", cl.pos[idx], cl.radius[idx], cl.ri[idx], fft=True, seed=17, observe=False)
+    agg = G.fractal_aggregate(2000, 0.5, 1.8, 1.3, seed=4, ri=1.75 + 0.4j)
+    rng = np.random.default_rng(4)
+    beta, alpha = float(np.arccos(1 - 2 * rng.random())), float(2 * np.pi * rng.random())
+    case("cfg4_cut300", agg.pos[:300], agg.radius[:300], agg.ri[:300], fft=True, seed=18, observe=False,
+         alpha=alpha, sinc=float(np.sin(beta)), idir=1 if np.cos(beta) >= 0 else 2)
+    fa = G.read_pos(os.path.join(REF_EXAMPLES, "frac_agg.pos"))
+    case("frac_agg_fft", fa.pos * 0.5, fa.radius * 0.5, np.full(fa.n, 1.75 + 0.4j), fft=True, seed=19,
+         observe=False)
+
+
 def boundary_only():
     c = G.rsa_in_sphere(20, 2.0, 0.10, seed=1, ri=1.5 + 0.01j)
     boundary_case("boundary_cfg1", c.pos, c.radius, c.ri)
@@ -202,6 +234,7 @@ def main():
     tmatrix_case("tmat_poly6", c.pos, radius, c.ri, eps=1e-8, conveps=1e-7)
     lu_only()
     boundary_only()
+    config_cuts()
 
 
 def lu_only():
@@ -221,4 +254,4 @@ def tmatrix_only():
 
 
 if __name__ == "__main__":
-    {"tmatrix": tmatrix_only, "lu": lu_only, "boundary": boundary_only}.get(sys.argv[1] if len(sys.argv) > 1 else "", main)()
+    {"tmatrix": tmatrix_only, "lu": lu_only, "boundary": boundary_only, "cuts": config_cuts}.get(sys.argv[1] if len(sys.argv) > 1 else "", main)()

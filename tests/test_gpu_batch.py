@@ -115,3 +115,98 @@ def test_graph_replays_match_eager(small_batch):
     np.testing.assert_array_equal(r1["iterations"], r2["iterations"])
     for a, c in zip(x1, x2):
         np.testing.assert_array_equal(a, c)
+
+
+# ---------------------------------------------------------------------------
+# Clusters of the 256-cluster workload's own distribution (N log-uniform in
+# [10, 300], x uniform in [0.5, 3], fv uniform in [0.05, 0.30],
+# m = 1.33+0.001i, bench.py's draws) with its 32 Fibonacci directions x 2
+# polarizations = 64 right-hand sides per cluster: the largest cluster (300
+# spheres), the densest order-5 one, both order-6 ones the draw holds, the
+# largest order-3 one, the smallest, two from the middle, plus the
+# distribution's corner (300 spheres, x = 3, order 6, fv 0.30).
+CFG3_PICKS = (156, 84, 4, 47, 117, 71, 120, 185)
+
+
+def _cfg3_clusters():
+    from mstm_amd import geometry as G
+
+    rng = np.random.default_rng(3)
+    ns = np.round(np.exp(rng.uniform(np.log(10), np.log(300), 256))).astype(int)
+    xs = rng.uniform(0.5, 3.0, 256)
+    fvs = rng.uniform(0.05, 0.30, 256)
+    small = int(np.argmin(ns))
+    cl = [G.rsa_in_sphere(int(ns[k]), float(xs[k]), float(fvs[k]), seed=3 * 100003 + k, ri=RI)
+          for k in CFG3_PICKS + (small,)]
+    cl.append(G.rsa_in_sphere(300, 3.0, 0.30, seed=31, ri=RI))
+    return cl
+
+
+@pytest.fixture(scope="module")
+def cfg3_batch():
+    from mstm_amd import geometry as G
+    from mstm_amd.batch import BatchSolver
+
+    cl = _cfg3_clusters()
+    dirs = G.fibonacci_directions(32)
+    b = BatchSolver(0)
+    b.set(cl, dirs)
+    b.setup(tune=True)
+    yield b, cl, dirs
+    b.close()
+
+
+def test_cfg3_orders_cover_the_distribution(cfg3_batch):
+    b, cl, dirs = cfg3_batch
+    assert b.R == 64
+    assert set(np.asarray(b.order).tolist()) == {3, 4, 5, 6}
+    assert max(c.n for c in cl) == 300 and min(c.n for c in cl) <= 12
+
+
+def test_cfg3_fp32_mfma_operator_against_oracle(cfg3_batch):
+    """All 64 columns go through the grouped MFMA GEMM; the oracle's pair sum
+    checks the first two and the last two (both ends of the 64-wide column
+    tile) of every cluster, H and H^H, at 2e-5."""
+    b, cl, dirs = cfg3_batch
+    rng = np.random.default_rng(7)
+    xs = [rng.normal(size=(b.R, ne)) + 1j * rng.normal(size=(b.R, ne)) for ne in b.neqns]
+    ys = b.product(xs, precision=1, op=0)
+    yh = b.product(xs, precision=1, op=1)
+    cols = [0, 1, b.R - 2, b.R - 1]
+    for k, c in enumerate(cl):
+        oc = _oracle(c)
+        xk = xs[k][cols]
+        assert rel(ys[k][cols], oc.external_to_external(xk, [0] * 4)) < 2e-5, k
+        refh = np.conj(oc.external_to_external(np.conj(xk), [1] * 4))
+        assert rel(yh[k][cols], refh) < 2e-5, k
+
+
+def test_cfg3_tight_solves_match_single_cluster_fp64(cfg3_batch):
+    """The batched solve (FP32 MFMA inner products inside FP64 refinement) to
+    |r|^2/|b|^2 < 1e-14 against one-cluster FP64 solves of the same cluster
+    and direction (every cluster; 4 of the 32 directions: both poles' nearest
+    and two oblique, from below and above) at 1e-5 relative, efficiencies 1e-6;
+    and the batched solve at the workload's 1e-6 reaches it in every column."""
+    from mstm_amd.solver import HipContext, SOLVE_PLAIN, qtotcalc
+
+    b, cl, dirs = cfg3_batch
+    r6 = b.solve(eps=1e-6, niter=3000, use_graph=True)
+    assert np.all(r6["err"] < 1e-6), float(r6["err"].max())
+    r = b.solve(eps=1e-14, niter=3000, use_graph=True)
+    assert np.all(r["err"] < 1e-14), float(r["err"].max())
+    x = b.vectors(0)
+    q = b.qeff()
+    ctx = HipContext(0)
+    try:
+        for k, c in enumerate(cl):
+            ctx.set_spheres(c.pos, c.radius, c.ri, 1.0, 1e-6)
+            for d in (0, 9, 22, 31):
+                beta, alpha = dirs[d]
+                sinc, sd = _sinc_dir(beta)
+                s = ctx.fixedorsoln(alpha=alpha, sinc=sinc, eps=1e-14, niter=5000, solver=SOLVE_PLAIN,
+                                    direction=sd)
+                assert rel(x[k][2 * d:2 * d + 2], s["amnp"]) < 1e-5, (k, d)
+                qt = qtotcalc(c.radius, s["qeff"])
+                assert np.abs(q[k, d] - qt).max() < 1e-6 * np.abs(qt).max(), (k, d)
+    finally:
+        ctx.close()

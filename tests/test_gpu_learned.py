@@ -173,3 +173,25 @@ def test_learned_solver_converges_and_saves_iterations(gpu_ctx, n, seed):
     assert st["lig_fallbacks"] == 0
     assert st["iters_fp32"] < runs["mixed"]["stats"]["iters_fp32"]
     gpu_ctx.set_learned(None)
+
+
+def test_learned_solver_tight_agreement_with_fp64(gpu_ctx):
+    """Driven to |r|^2/|b|^2 < 1e-14, the learned + mixed solve and the plain
+    FP64 solve of a held-out cluster agree to 1e-5 (coefficients) and 1e-6
+    (efficiencies): the network only moves the start, FP64 residuals judge."""
+    from mstm_amd import geometry as G
+    from mstm_amd.solver import SOLVE_LEARNED, SOLVE_PLAIN, qtotcalc
+
+    path = _weights()
+    c = G.rsa_in_sphere(1000, 2.0, 0.30, seed=900003, ri=1.31)
+    gpu_ctx.set_spheres(c.pos, c.radius, c.ri, 1.0, 1e-6)
+    gpu_ctx.load_learned(path)
+    gpu_ctx.clear(False)
+    rl = gpu_ctx.fixedorsoln(eps=1e-14, niter=10000, solver=SOLVE_LEARNED)
+    st = gpu_ctx.stats()
+    rp = gpu_ctx.fixedorsoln(eps=1e-14, niter=10000, solver=SOLVE_PLAIN)
+    gpu_ctx.set_learned(None)
+    assert rl["istat"] == 0 and rl["maxerr"] < 1e-14 and rp["maxerr"] < 1e-14
+    assert st["lig_err0"] < 1.0  # the network's start was used
+    assert rel(rl["amnp"], rp["amnp"]) < 1e-5
+    assert rel(qtotcalc(c.radius, rl["qeff"]), qtotcalc(c.radius, rp["qeff"])) < 1e-6
