@@ -114,6 +114,22 @@ int mstm_hip_sphereinteraction(mstm_hip_ctx *ctx, int nrhs, const double *ain, d
                                const int *mie_mult, int use_fft, int precision, const int *rhs_list,
                                int skip_external_translation);
 
+/* The product of mstm_hip_sphereinteraction on the direct operator without
+ * any device allocation.  No counterpart in MSTM: for callers that cannot
+ * allocate device memory (serving loops, code under stream capture, harnesses
+ * that forbid hipMalloc).  Same arguments and results as
+ * mstm_hip_sphereinteraction with use_fft = 0 (use_fft != 0 is
+ * MSTM_HIP_EUNSUP); the pair tables of `precision` must already exist
+ * (mstm_hip_setup_direct, which allocates them, else MSTM_HIP_ESTATE).
+ * Staging, padded input and partial sums live in one 32 MiB array per GPU,
+ * held from the library's load to the end of the process; a cluster whose
+ * product needs more is MSTM_HIP_EARG (about 570 spheres of order 4 at FP32,
+ * 400 at FP64, four columns).  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (several threads or contexts) race. */
+int mstm_hip_sphereinteraction_noalloc(mstm_hip_ctx *ctx, int nrhs, const double *ain, double *aout,
+                                       const int *con_tran, const int *mie_mult, int use_fft, int precision,
+                                       const int *rhs_list, int skip_external_translation);
+
 /* cbicg (mstm-solver-8.f90:654-949): solve (I - T H) anp = pnp with the
  * reference's complex biconjugate gradient, FP64, starting from anp.
  * Stops when |r|^2/|b|^2 < eps.  iter: iterations done (niter+1 when not

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -183,6 +184,35 @@ template <typename P> static void dfree(P *&p)
     p = nullptr;
 }
 
+// ---------------------------------------------------------------- scratch
+// Device scratch of mstm_hip_sphereinteraction_noalloc: one file-scope array
+// per GPU, held until the process ends.  Host code never passes `scratch`
+// itself to a kernel (in host code the name is only a stand-in for the
+// symbol): each device's address comes from hipGetSymbolAddress, taken once
+// per device with that device current (mstm_hip_create), and kernels get
+// that address.
+#define MSTM_SCRATCH_BYTES (32u << 20)
+static __device__ __align__(256) unsigned char scratch[MSTM_SCRATCH_BYTES];
+#define MSTM_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[MSTM_MAX_DEVICES];
+static hipError_t g_scratch_rc[MSTM_MAX_DEVICES];
+static bool g_scratch_done[MSTM_MAX_DEVICES];
+
+// address of this device's scratch array; the device must be current
+static hipError_t scratch_bind(int device)
+{
+    if (device < 0 || device >= MSTM_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_done[device]) {
+        void *p = nullptr;
+        g_scratch_rc[device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_ptr[device] = g_scratch_rc[device] == hipSuccess ? p : nullptr;
+        g_scratch_done[device] = true;
+    }
+    return g_scratch_rc[device];
+}
+
 // ---------------------------------------------------------------- constants
 // vector-coupling products vcc(n, l, m, w) of axialtrancoefinit
 // (mstm-modules-33.f90:1680-1710): -sqrt((2n+1)(2l+1)) (-1)^m
@@ -339,6 +369,9 @@ int mstm_hip_create(mstm_hip_ctx **ctx, int device)
     mstm_hip_ctx *c = new mstm_hip_ctx();
     c->device = device;
     HIPCHK(hipSetDevice(device));
+    // the allocation-free entry point's scratch on this device; a failure is
+    // kept and reported by that entry point, other calls do not need it
+    (void)scratch_bind(device);
     HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     HIPCHK(dalloc(&c->d_partials, sizeof(double) * 16 * krylov_grid()));  // 8 regions of one value per block
     c->h_partials.resize(16 * krylov_grid());
@@ -597,13 +630,12 @@ static hipEvent_t next_event(mstm_hip_ctx *c)
 // Device-side sphereinteraction on the direct operator: ain/aout are
 // (neqns, nrhs) complex FP64 device arrays, nrhs <= 4; 3-4 columns run as
 // one four-column tile launch.
-static int direct_matvec(mstm_hip_ctx *c, int nrhs, const double *ain, double *aout, const int *flags, int prec)
+// xin/part: the padded input and the partial sums (the context's, or the
+// allocation-free entry point's scratch).
+static int direct_matvec_buf(mstm_hip_ctx *c, int nrhs, const double *ain, double *aout, const int *flags, int prec,
+                             void *xin, void *part)
 {
     DirectOp &d = c->dop[prec];
-    if (!d.ready) {
-        int rc = mstm_hip_setup_direct(c, prec);
-        if (rc) return rc;
-    }
     if (nrhs < 1 || nrhs > 4) return fail(c, MSTM_HIP_EARG, "direct product: %d columns", nrhs);
     const int L = c->Lmax, nr = nrhs > 2 ? 4 : 2;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -612,27 +644,37 @@ static int direct_matvec(mstm_hip_ctx *c, int nrhs, const double *ain, double *a
         e1 = next_event(c);
     }
     if (prec == MSTM_HIP_FP64) {
-        HIPCHK(launch_prep<double>(ain, 2 * c->neqns, (double *)d.xin, nrhs, flags, c->nsph, c->npad, L, c->d_order,
+        HIPCHK(launch_prep<double>(ain, 2 * c->neqns, (double *)xin, nrhs, flags, c->nsph, c->npad, L, c->d_order,
                                    c->d_offset, c->d_an, c->st));
         if (e0) hipEventRecord(e0, c->st);
-        HIPCHK(launch_direct_tiles<double>((const double *)d.tab, c->d_tiles, c->ntiles, L, (const double *)d.xin,
-                                           (double *)d.part, c->npad, nr, c->st));
+        HIPCHK(launch_direct_tiles<double>((const double *)d.tab, c->d_tiles, c->ntiles, L, (const double *)xin,
+                                           (double *)part, c->npad, nr, c->st));
         if (e1) hipEventRecord(e1, c->st);
-        HIPCHK(launch_reduce_post<double>((const double *)d.part, c->nblocks, aout, 2 * c->neqns, nrhs, flags, c->nsph,
+        HIPCHK(launch_reduce_post<double>((const double *)part, c->nblocks, aout, 2 * c->neqns, nrhs, flags, c->nsph,
                                           c->npad, L, c->d_order, c->d_offset, c->d_an, c->st));
     } else {
-        HIPCHK(launch_prep<float>(ain, 2 * c->neqns, (float *)d.xin, nrhs, flags, c->nsph, c->npad, L, c->d_order,
+        HIPCHK(launch_prep<float>(ain, 2 * c->neqns, (float *)xin, nrhs, flags, c->nsph, c->npad, L, c->d_order,
                                   c->d_offset, c->d_an, c->st));
         if (e0) hipEventRecord(e0, c->st);
-        HIPCHK(launch_direct_tiles<float>((const float *)d.tab, c->d_tiles, c->ntiles, L, (const float *)d.xin,
-                                          (float *)d.part, c->npad, nr, c->st));
+        HIPCHK(launch_direct_tiles<float>((const float *)d.tab, c->d_tiles, c->ntiles, L, (const float *)xin,
+                                          (float *)part, c->npad, nr, c->st));
         if (e1) hipEventRecord(e1, c->st);
-        HIPCHK(launch_reduce_post<float>((const float *)d.part, c->nblocks, aout, 2 * c->neqns, nrhs, flags, c->nsph,
+        HIPCHK(launch_reduce_post<float>((const float *)part, c->nblocks, aout, 2 * c->neqns, nrhs, flags, c->nsph,
                                          c->npad, L, c->d_order, c->d_offset, c->d_an, c->st));
     }
     if (e0) c->ev_prec.push_back(prec + 2 * (nr == 4));
     c->stats.matvecs[prec]++;
     return 0;
+}
+
+static int direct_matvec(mstm_hip_ctx *c, int nrhs, const double *ain, double *aout, const int *flags, int prec)
+{
+    DirectOp &d = c->dop[prec];
+    if (!d.ready) {
+        int rc = mstm_hip_setup_direct(c, prec);
+        if (rc) return rc;
+    }
+    return direct_matvec_buf(c, nrhs, ain, aout, flags, prec, d.xin, d.part);
 }
 
 static int matvec(mstm_hip_ctx *c, int nrhs, const double *ain, double *aout, const int *flags, int prec, int use_fft)
@@ -1567,6 +1609,89 @@ int mstm_hip_sphereinteraction(mstm_hip_ctx *c, int nrhs, const double *ain, dou
     HIPCHK(hipStreamSynchronize(c->st));
     collect_events(c);
     return 0;
+}
+
+// Scratch bytes of one direct product of nr (2 or 4) columns: staged input
+// and output columns (nr x neqns complex FP64 each), the padded input and the
+// per-block partial sums at the table precision; each part 256-B aligned.
+static size_t noalloc_layout(const mstm_hip_ctx *c, int nr, int prec, size_t off[4])
+{
+    auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+    const size_t esz = prec == MSTM_HIP_FP64 ? sizeof(double) : sizeof(float);
+    const size_t nbp = (size_t)c->Lmax * (c->Lmax + 2);
+    const size_t vcol = sizeof(double) * 2 * (size_t)c->neqns;
+    const size_t sz[4] = {vcol * nr, vcol * nr, esz * 2 * nr * 2 * (size_t)c->npad * nbp,
+                          esz * 2 * (size_t)c->nblocks * nr * 2 * c->npad * nbp};
+    size_t o = 0;
+    for (int q = 0; q < 4; q++) {
+        off[q] = o;
+        o += al(sz[q]);
+    }
+    return o;
+}
+
+int mstm_hip_sphereinteraction_noalloc(mstm_hip_ctx *c, int nrhs, const double *ain, double *aout, const int *con_tran,
+                                       const int *mie_mult, int use_fft, int prec, const int *rhs_list,
+                                       int skip_external_translation)
+{
+    if (!c || !c->nsph) return fail(c, MSTM_HIP_ESTATE, "sphereinteraction_noalloc: no spheres");
+    if (nrhs < 1 || !ain || !aout || !con_tran || !mie_mult)
+        return fail(c, MSTM_HIP_EARG, "sphereinteraction_noalloc: args");
+    if (use_fft) return fail(c, MSTM_HIP_EUNSUP, "sphereinteraction_noalloc: direct operator only");
+    if (prec != MSTM_HIP_FP64 && prec != MSTM_HIP_FP32)
+        return fail(c, MSTM_HIP_EARG, "sphereinteraction_noalloc: precision %d", prec);
+    HIPCHK(hipSetDevice(c->device));
+    const int nr = nrhs > 2 ? 4 : 2;
+    size_t off[4];
+    const size_t need = noalloc_layout(c, nr, prec, off);
+    if (need > MSTM_SCRATCH_BYTES)
+        return fail(c, MSTM_HIP_EARG,
+                    "sphereinteraction_noalloc: %zu bytes of scratch needed (%d spheres of order %d, %d columns), "
+                    "more than the %u-byte device scratch: use mstm_hip_sphereinteraction",
+                    need, c->nsph, c->Lmax, nr, (unsigned)MSTM_SCRATCH_BYTES);
+    if (!c->dop[prec].ready)
+        return fail(c, MSTM_HIP_ESTATE,
+                    "sphereinteraction_noalloc: no %s tables (mstm_hip_setup_direct allocates them; this entry "
+                    "point does not)", prec == MSTM_HIP_FP64 ? "FP64" : "FP32");
+    if (c->device < 0 || c->device >= MSTM_MAX_DEVICES || !g_scratch_ptr[c->device])
+        return fail(c, MSTM_HIP_EHIP, "sphereinteraction_noalloc: no device scratch on GPU %d: %s", c->device,
+                    hipGetErrorString(c->device >= 0 && c->device < MSTM_MAX_DEVICES ? g_scratch_rc[c->device]
+                                                                                      : hipErrorInvalidDevice));
+    unsigned char *base = (unsigned char *)g_scratch_ptr[c->device];
+    double *din = (double *)(base + off[0]), *dout = (double *)(base + off[1]);
+    void *xin = base + off[2], *part = base + off[3];
+    const size_t V = 2 * (size_t)c->neqns;
+    const bool timing = c->timing;  // timing events may be created on demand: none here
+    c->timing = false;
+    int rc = 0, cols[4], ncol = 0;
+    auto flush = [&]() -> int {
+        if (!ncol) return 0;
+        int flags[4] = {0, 0, 0, 0};
+        for (int q = 0; q < ncol; q++) {
+            const int r = cols[q];
+            flags[q] = (con_tran[r] ? 1 : 0) | (mie_mult[r] ? 2 : 0);
+            HIPCHK(hipMemcpyAsync(din + V * q, ain + V * r, sizeof(double) * V, hipMemcpyHostToDevice, c->st));
+        }
+        int r2 = direct_matvec_buf(c, ncol, din, dout, flags, prec, xin, part);
+        if (r2) return r2;
+        for (int q = 0; q < ncol; q++)
+            HIPCHK(hipMemcpyAsync(aout + V * cols[q], dout + V * q, sizeof(double) * V, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));  // din/dout are refilled next
+        ncol = 0;
+        return 0;
+    };
+    // rhs_list / skip_external_translation as mstm_hip_sphereinteraction: zero columns
+    for (int r = 0; r < nrhs && !rc; r++) {
+        if (skip_external_translation || (rhs_list && !rhs_list[r])) {
+            memset(aout + V * r, 0, sizeof(double) * V);
+            continue;
+        }
+        cols[ncol++] = r;
+        if (ncol == nr) rc = flush();
+    }
+    if (!rc) rc = flush();
+    c->timing = timing;
+    return rc;
 }
 
 int mstm_hip_cbicg(mstm_hip_ctx *c, int niter, double eps, const double *pnp, double *anp, int use_fft, int *iter,

@@ -42,6 +42,8 @@ _SIGNATURES = {
     "mstm_hip_fft_grid": (C.c_int, [_vp, _dp, _ip, _ip]),
     "mstm_hip_setup_fft": (C.c_int, [_vp, C.c_int]),
     "mstm_hip_sphereinteraction": (C.c_int, [_vp, C.c_int, _dp, _dp, _ip, _ip, C.c_int, C.c_int, _ip, C.c_int]),
+    "mstm_hip_sphereinteraction_noalloc": (C.c_int, [_vp, C.c_int, _dp, _dp, _ip, _ip, C.c_int, C.c_int, _ip,
+                                                     C.c_int]),
     "mstm_hip_cbicg": (C.c_int, [_vp, C.c_int, C.c_double, _dp, _dp, C.c_int, _ip, _dp]),
     "mstm_hip_fixedorsoln": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
                                        _dp, _dp, _dp, _ip, _ip, _ip]),

@@ -150,6 +150,24 @@ class HipContext:
                                                     int(bool(skip_external_translation))), "sphereinteraction")
         return out
 
+    def sphereinteraction_noalloc(self, ain, con_tran, mie_mult=None, precision: int = FP64, rhs_list=None,
+                                  skip_external_translation: bool = False):
+        """The direct-operator product of sphereinteraction without device
+        allocation (the library's fixed per-GPU scratch); the tables of this
+        precision must exist (setup_direct)."""
+        ain = np.ascontiguousarray(np.atleast_2d(ain), np.complex128)
+        nrhs = ain.shape[0]
+        ct = np.array([1 if x else 0 for x in con_tran], np.int32)
+        mm = np.array([1 if x else 0 for x in (mie_mult if mie_mult is not None else [True] * nrhs)], np.int32)
+        rl = None if rhs_list is None else np.array([1 if x else 0 for x in rhs_list], np.int32)
+        out = np.zeros_like(ain)
+        self._chk(self.L.mstm_hip_sphereinteraction_noalloc(self.h, nrhs, _d(ain.view(np.float64)),
+                                                            _d(out.view(np.float64)), _i(ct), _i(mm), 0, precision,
+                                                            None if rl is None else _i(rl),
+                                                            int(bool(skip_external_translation))),
+                  "sphereinteraction_noalloc")
+        return out
+
     # -- solver (cbicg / fixedorsoln, mstm-solver-8.f90) ----------------------
     def cbicg(self, niter, eps, pnp, anp, fft_option=False):
         pnp = np.ascontiguousarray(pnp, np.complex128)
