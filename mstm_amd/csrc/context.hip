@@ -92,6 +92,7 @@ struct mstm_hip_ctx {
     double *d_qeff = nullptr;  // per-sphere efficiencies of fixedorsoln (9 nsph)
     int2 *d_tiles = nullptr;
     int ntiles = 0, nblocks = 0;
+    int tgroups = 1;  // workgroups per direct tile (its 16 steps split among them)
     double *d_vcc = nullptr;
     int vcc_LM = 0;
     DirectOp dop[2];
@@ -110,6 +111,8 @@ struct mstm_hip_ctx {
     size_t kv_n = 0;
     double *d_partials = nullptr;
     std::vector<double> h_partials;
+    BicgScal *d_bscal = nullptr;  // device-resident BiCG scalars of two systems
+    BicgScal *h_bscal = nullptr;  // their pinned host copy
     // event pool for tile-kernel timing
     std::vector<hipEvent_t> ev;
     std::vector<int> ev_prec;
@@ -375,6 +378,8 @@ int mstm_hip_create(mstm_hip_ctx **ctx, int device)
     HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     HIPCHK(dalloc(&c->d_partials, sizeof(double) * 16 * krylov_grid()));  // 8 regions of one value per block
     c->h_partials.resize(16 * krylov_grid());
+    HIPCHK(dalloc(&c->d_bscal, sizeof(BicgScal) * 2));
+    HIPCHK(hipHostMalloc((void **)&c->h_bscal, sizeof(BicgScal) * 2, hipHostMallocDefault));
     *ctx = c;
     return 0;
 }
@@ -391,6 +396,8 @@ int mstm_hip_destroy(mstm_hip_ctx *c)
     dfree(c->d_gp);
     dfree(c->d_vcc);
     dfree(c->d_partials);
+    dfree(c->d_bscal);
+    if (c->h_bscal) hipHostFree(c->h_bscal);
     for (auto &e : c->ev) hipEventDestroy(e);
     hipStreamDestroy(c->st);
     delete c;
@@ -426,6 +433,17 @@ static int finish_sphere_layout(mstm_hip_ctx *c)
     for (int bi = 0; bi < c->nblocks; bi++)
         for (int bj = bi; bj < c->nblocks; bj++) tiles.push_back(make_int2(bi, bj));
     c->ntiles = (int)tiles.size();
+    // few tiles (small clusters): split each tile's 16 steps over up to 16
+    // workgroups so that the launch fills the GPU; each keeps its own
+    // partial sums (partial buffer at most 256 MiB at FP64, four columns)
+    c->tgroups = 1;
+    {
+        const size_t nbp = (size_t)c->Lmax * (c->Lmax + 2);
+        auto part_bytes = [&](int G) { return sizeof(double) * 2 * (size_t)c->nblocks * G * 4 * 2 * c->npad * nbp; };
+        while (c->tgroups < MSTM_SLOTS && (int64_t)c->ntiles * c->tgroups < 2048 &&
+               part_bytes(2 * c->tgroups) <= ((size_t)256 << 20))
+            c->tgroups *= 2;
+    }
     dfree(c->d_tiles);
     HIPCHK(dalloc(&c->d_tiles, sizeof(int2) * tiles.size()));
     HIPCHK(hipMemcpy(c->d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice));
@@ -549,7 +567,7 @@ int mstm_hip_setup_direct(mstm_hip_ctx *c, int prec)
     size_t nbp = (size_t)L * (L + 2);
     HIPCHK(dalloc(&d.tab, d.tab_bytes));
     HIPCHK(dalloc(&d.xin, esz * 2 * 4 * 2 * (size_t)c->npad * nbp));
-    HIPCHK(dalloc(&d.part, esz * 2 * (size_t)c->nblocks * 4 * 2 * c->npad * nbp));
+    HIPCHK(dalloc(&d.part, esz * 2 * (size_t)c->nblocks * c->tgroups * 4 * 2 * c->npad * nbp));
     auto t0 = std::chrono::steady_clock::now();
     if (prec == MSTM_HIP_FP64)
         HIPCHK(launch_direct_tables<double>((double *)d.tab, c->d_tiles, c->ntiles, L, c->nsph, c->d_pos, c->rimed,
@@ -633,7 +651,7 @@ static hipEvent_t next_event(mstm_hip_ctx *c)
 // xin/part: the padded input and the partial sums (the context's, or the
 // allocation-free entry point's scratch).
 static int direct_matvec_buf(mstm_hip_ctx *c, int nrhs, const double *ain, double *aout, const int *flags, int prec,
-                             void *xin, void *part)
+                             void *xin, void *part, int G)
 {
     DirectOp &d = c->dop[prec];
     if (nrhs < 1 || nrhs > 4) return fail(c, MSTM_HIP_EARG, "direct product: %d columns", nrhs);
@@ -648,18 +666,18 @@ static int direct_matvec_buf(mstm_hip_ctx *c, int nrhs, const double *ain, doubl
                                    c->d_offset, c->d_an, c->st));
         if (e0) hipEventRecord(e0, c->st);
         HIPCHK(launch_direct_tiles<double>((const double *)d.tab, c->d_tiles, c->ntiles, L, (const double *)xin,
-                                           (double *)part, c->npad, nr, c->st));
+                                           (double *)part, c->npad, nr, G, c->st));
         if (e1) hipEventRecord(e1, c->st);
-        HIPCHK(launch_reduce_post<double>((const double *)part, c->nblocks, aout, 2 * c->neqns, nrhs, flags, c->nsph,
+        HIPCHK(launch_reduce_post<double>((const double *)part, c->nblocks * G, aout, 2 * c->neqns, nrhs, flags, c->nsph,
                                           c->npad, L, c->d_order, c->d_offset, c->d_an, c->st));
     } else {
         HIPCHK(launch_prep<float>(ain, 2 * c->neqns, (float *)xin, nrhs, flags, c->nsph, c->npad, L, c->d_order,
                                   c->d_offset, c->d_an, c->st));
         if (e0) hipEventRecord(e0, c->st);
         HIPCHK(launch_direct_tiles<float>((const float *)d.tab, c->d_tiles, c->ntiles, L, (const float *)xin,
-                                          (float *)part, c->npad, nr, c->st));
+                                          (float *)part, c->npad, nr, G, c->st));
         if (e1) hipEventRecord(e1, c->st);
-        HIPCHK(launch_reduce_post<float>((const float *)part, c->nblocks, aout, 2 * c->neqns, nrhs, flags, c->nsph,
+        HIPCHK(launch_reduce_post<float>((const float *)part, c->nblocks * G, aout, 2 * c->neqns, nrhs, flags, c->nsph,
                                          c->npad, L, c->d_order, c->d_offset, c->d_an, c->st));
     }
     if (e0) c->ev_prec.push_back(prec + 2 * (nr == 4));
@@ -674,7 +692,7 @@ static int direct_matvec(mstm_hip_ctx *c, int nrhs, const double *ain, double *a
         int rc = mstm_hip_setup_direct(c, prec);
         if (rc) return rc;
     }
-    return direct_matvec_buf(c, nrhs, ain, aout, flags, prec, d.xin, d.part);
+    return direct_matvec_buf(c, nrhs, ain, aout, flags, prec, d.xin, d.part, c->tgroups);
 }
 
 static int matvec(mstm_hip_ctx *c, int nrhs, const double *ain, double *aout, const int *flags, int prec, int use_fft)
@@ -750,12 +768,6 @@ static int sum_partials(mstm_hip_ctx *c, int nvals, double *out)
 static inline double *pregion(mstm_hip_ctx *c, int s) { return c->d_partials + (size_t)4 * krylov_grid() * s; }
 static inline int pregion_off(int s) { return 2 * krylov_grid() * s; }
 
-static inline void zdiv(double ar, double ai, double br, double bi, double &cr, double &ci)
-{
-    double d = br * br + bi * bi;
-    cr = (ar * br + ai * bi) / d;
-    ci = (ai * br - ar * bi) / d;
-}
 
 // cbicg (mstm-solver-8.f90:654-949) for ns = 1 or 2 independent systems
 // A x_s = b_s at once (the two polarizations of fixedorsoln).  b and x are
@@ -773,6 +785,8 @@ static int bicg2_dev(mstm_hip_ctx *c, int ns, int niter, const double *eps, cons
     double *P[2], *W[2], *AP[2], *AW[2], *Rr[2], *Q[2];
     const double *B[2];
     double *X[2];
+    BicgScal *S = c->d_bscal, *hS = c->h_bscal;
+    const int64_t region = 2 * (int64_t)krylov_grid();  // complex values per partial region
     for (int s = 0; s < ns; s++) {
         P[s] = kv + V * (KV_PW + 2 * s);
         W[s] = P[s] + V;
@@ -784,24 +798,27 @@ static int bicg2_dev(mstm_hip_ctx *c, int ns, int niter, const double *eps, cons
         X[s] = x + V * s;
         it[s] = 0;
         err[s] = 0.0;
+        hS[s] = BicgScal{};
+        hS[s].eps = eps[s];
     }
-    double enorm[2] = {0, 0}, csk[2][2];
-    bool act[2] = {false, false};
+    // the scalars live on the device; the host reads them once per iteration
+    auto sync_state = [&]() -> int {
+        HIPCHK(hipMemcpyAsync(hS, S, sizeof(BicgScal) * ns, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        return 0;
+    };
+    HIPCHK(hipMemcpyAsync(S, hS, sizeof(BicgScal) * ns, hipMemcpyHostToDevice, c->st));
     // |b|^2
     for (int s = 0; s < ns; s++) HIPCHK(launch_zdot(B[s], B[s], n, 1, pregion(c, s), nullptr, c->st));
-    int rc = fetch_partials(c, 2 * krylov_grid() * ns);
+    HIPCHK(launch_bicg_scal(0, c->d_partials, region, S, ns, 0, c->st));
+    int rc = sync_state();
     if (rc) return rc;
-    for (int s = 0; s < ns; s++) {
-        double v[2];
-        reduce_region(c, pregion_off(s), 1, v);
-        enorm[s] = v[0];
-        act[s] = v[0] != 0.0;
-    }
-    if (!act[0] && (ns < 2 || !act[1])) return 0;
+    bool act[2] = {hS[0].active != 0, ns == 2 && hS[1].active != 0};
+    if (!act[0] && !act[1]) return 0;
     // r = b - A x (:827-834), one product for the active systems' x
     {
         int f[2] = {2, 2};
-        if (ns == 2 && act[0] && act[1]) rc = matvec(c, 2, X[0], Rr[0], f, prec, use_fft);
+        if (act[0] && act[1]) rc = matvec(c, 2, X[0], Rr[0], f, prec, use_fft);
         else
             for (int s = 0; s < ns && !rc; s++)
                 if (act[s]) rc = matvec(c, 1, X[s], Rr[s], f, prec, use_fft);
@@ -809,64 +826,44 @@ static int bicg2_dev(mstm_hip_ctx *c, int ns, int niter, const double *eps, cons
     }
     for (int s = 0; s < ns; s++)
         if (act[s]) HIPCHK(launch_bicg_init(Rr[s], Q[s], W[s], P[s], B[s], X[s], n, pregion(c, s), c->st));
-    rc = fetch_partials(c, 2 * krylov_grid() * ns);
+    HIPCHK(launch_bicg_scal(1, c->d_partials, region, S, ns, 0, c->st));
+    rc = sync_state();
     if (rc) return rc;
-    for (int s = 0; s < ns; s++) {
-        if (!act[s]) continue;
-        reduce_region(c, pregion_off(s), 1, csk[s]);
-        if (csk[s][0] == 0.0 && csk[s][1] == 0.0) act[s] = false;  // it stays 0
-    }
+    act[0] = hS[0].active != 0;
+    act[1] = ns == 2 && hS[1].active != 0;
     int iter;
     for (iter = 1; iter <= niter; iter++) {
-        const bool both = ns == 2 && act[0] && act[1];
-        if (!act[0] && !(ns == 2 && act[1])) break;
+        if (!act[0] && !act[1]) break;
         int f4[4] = {2, 3, 2, 3};
-        if (both) rc = matvec(c, 4, P[0], AP[0], f4, prec, use_fft);
+        if (act[0] && act[1]) rc = matvec(c, 4, P[0], AP[0], f4, prec, use_fft);
         else rc = matvec(c, 2, act[0] ? P[0] : P[1], act[0] ? AP[0] : AP[1], f4, prec, use_fft);
         if (rc) return rc;
+        // the vector kernels of a system the device has stopped return at once
         for (int s = 0; s < ns; s++)
-            if (act[s]) HIPCHK(launch_bicg_a(AP[s], AW[s], P[s], W[s], n, pregion(c, s), c->st));
-        rc = fetch_partials(c, 2 * krylov_grid() * ns);
+            if (act[s]) HIPCHK(launch_bicg_a_dev(AP[s], AW[s], P[s], W[s], n, pregion(c, s), S + s, c->st));
+        HIPCHK(launch_bicg_scal(2, c->d_partials, region, S, ns, iter, c->st));
+        for (int s = 0; s < ns; s++)
+            if (act[s])
+                HIPCHK(launch_bicg_b_dev(X[s], Rr[s], Q[s], P[s], AP[s], AW[s], S + s, n, pregion(c, s), c->st));
+        HIPCHK(launch_bicg_scal(3, c->d_partials, region, S, ns, iter, c->st));
+        for (int s = 0; s < ns; s++)
+            if (act[s]) HIPCHK(launch_bicg_c_dev(P[s], W[s], Rr[s], Q[s], S + s, n, c->st));
+        rc = sync_state();
         if (rc) return rc;
-        bool step[2] = {false, false};
         for (int s = 0; s < ns; s++) {
             if (!act[s]) continue;
-            double den[2];
-            reduce_region(c, pregion_off(s), 1, den);
-            if (den[0] == 0.0 && den[1] == 0.0) {  // breakdown: the Fortran loop exits here
-                act[s] = false;
-                it[s] = iter;
-                continue;
+            // a step that passed the breakdown test counts, as the host loop did
+            if (!(hS[s].it == iter && hS[s].active == 0 && hS[s].cak.re == 0.0 && hS[s].cak.im == 0.0)) {
+                if (prec == MSTM_HIP_FP64) c->stats.iters_fp64++;
+                else c->stats.iters_fp32++;
             }
-            double cak[2];
-            zdiv(csk[s][0], csk[s][1], den[0], den[1], cak[0], cak[1]);
-            HIPCHK(launch_bicg_b(X[s], Rr[s], Q[s], P[s], AP[s], AW[s], cak[0], cak[1], n, pregion(c, s), c->st));
-            step[s] = true;
-        }
-        if (!step[0] && !(ns == 2 && step[1])) continue;
-        rc = fetch_partials(c, 2 * krylov_grid() * ns);
-        if (rc) return rc;
-        for (int s = 0; s < ns; s++) {
-            if (!step[s]) continue;
-            double d2[4];
-            reduce_region(c, pregion_off(s), 2, d2);
-            err[s] = d2[2] / enorm[s];
-            if (prec == MSTM_HIP_FP64) c->stats.iters_fp64++;
-            else c->stats.iters_fp32++;
-            if (err[s] < eps[s] || (csk[s][0] == 0.0 && csk[s][1] == 0.0)) {
-                act[s] = false;
-                it[s] = iter;
-                continue;
-            }
-            double cbk[2];
-            zdiv(d2[0], d2[1], csk[s][0], csk[s][1], cbk[0], cbk[1]);
-            csk[s][0] = d2[0];
-            csk[s][1] = d2[1];
-            HIPCHK(launch_bicg_c(P[s], W[s], Rr[s], Q[s], cbk[0], cbk[1], n, c->st));
+            act[s] = hS[s].active != 0;
         }
     }
-    for (int s = 0; s < ns; s++)
-        if (act[s]) it[s] = iter;  // ran out: niter + 1
+    for (int s = 0; s < ns; s++) {
+        it[s] = act[s] ? iter : hS[s].it;  // ran out: niter + 1
+        err[s] = hS[s].err;
+    }
     return 0;
 }
 
@@ -1614,14 +1611,14 @@ int mstm_hip_sphereinteraction(mstm_hip_ctx *c, int nrhs, const double *ain, dou
 // Scratch bytes of one direct product of nr (2 or 4) columns: staged input
 // and output columns (nr x neqns complex FP64 each), the padded input and the
 // per-block partial sums at the table precision; each part 256-B aligned.
-static size_t noalloc_layout(const mstm_hip_ctx *c, int nr, int prec, size_t off[4])
+static size_t noalloc_layout(const mstm_hip_ctx *c, int nr, int prec, int G, size_t off[4])
 {
     auto al = [](size_t x) { return (x + 255) / 256 * 256; };
     const size_t esz = prec == MSTM_HIP_FP64 ? sizeof(double) : sizeof(float);
     const size_t nbp = (size_t)c->Lmax * (c->Lmax + 2);
     const size_t vcol = sizeof(double) * 2 * (size_t)c->neqns;
     const size_t sz[4] = {vcol * nr, vcol * nr, esz * 2 * nr * 2 * (size_t)c->npad * nbp,
-                          esz * 2 * (size_t)c->nblocks * nr * 2 * c->npad * nbp};
+                          esz * 2 * (size_t)c->nblocks * G * nr * 2 * c->npad * nbp};
     size_t o = 0;
     for (int q = 0; q < 4; q++) {
         off[q] = o;
@@ -1643,7 +1640,11 @@ int mstm_hip_sphereinteraction_noalloc(mstm_hip_ctx *c, int nrhs, const double *
     HIPCHK(hipSetDevice(c->device));
     const int nr = nrhs > 2 ? 4 : 2;
     size_t off[4];
-    const size_t need = noalloc_layout(c, nr, prec, off);
+    // the workgroups per tile of the context's products, fewer if their
+    // partials would not fit (the sums are the same)
+    int G = c->tgroups;
+    size_t need = noalloc_layout(c, nr, prec, G, off);
+    while (need > MSTM_SCRATCH_BYTES && G > 1) need = noalloc_layout(c, nr, prec, G /= 2, off);
     if (need > MSTM_SCRATCH_BYTES)
         return fail(c, MSTM_HIP_EARG,
                     "sphereinteraction_noalloc: %zu bytes of scratch needed (%d spheres of order %d, %d columns), "
@@ -1672,7 +1673,7 @@ int mstm_hip_sphereinteraction_noalloc(mstm_hip_ctx *c, int nrhs, const double *
             flags[q] = (con_tran[r] ? 1 : 0) | (mie_mult[r] ? 2 : 0);
             HIPCHK(hipMemcpyAsync(din + V * q, ain + V * r, sizeof(double) * V, hipMemcpyHostToDevice, c->st));
         }
-        int r2 = direct_matvec_buf(c, ncol, din, dout, flags, prec, xin, part);
+        int r2 = direct_matvec_buf(c, ncol, din, dout, flags, prec, xin, part, G);
         if (r2) return r2;
         for (int q = 0; q < ncol; q++)
             HIPCHK(hipMemcpyAsync(aout + V * cols[q], dout + V * q, sizeof(double) * V, hipMemcpyDeviceToHost, c->st));

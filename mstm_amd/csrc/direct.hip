@@ -157,7 +157,8 @@ __device__ __forceinline__ void stage_group(const T *__restrict__ tab, int tile,
 template <typename T, int L, int DIR, int NR>
 __device__ __forceinline__ void tile_body_sym(int2 bt, int tile, const T *__restrict__ tab, cplx<T> *__restrict__ part,
                                               int npad, int rpair, T *__restrict__ grp,
-                                              const cplx<T> *__restrict__ srcl, const cplx<T> *__restrict__ xin)
+                                              const cplx<T> *__restrict__ srcl, const cplx<T> *__restrict__ xin,
+                                              int s0, int s1, int G, int g)
 {
     constexpr bool PIPE = tile_pipelined<T, L, NR>();
     constexpr int VW = 16 / sizeof(T);
@@ -173,10 +174,10 @@ __device__ __forceinline__ void tile_body_sym(int2 bt, int tile, const T *__rest
     for (int q = 0; q < NB; q++) W[q] = cmk((T)0, (T)0);
 
     GroupPrefetch<T, L, NR> pre;
-    if constexpr (PIPE) pre.load(tab, tile, 0);
+    if constexpr (PIPE) pre.load(tab, tile, s0);
 
 #pragma unroll 1
-    for (int s = 0; s < S; s++) {
+    for (int s = s0; s < s1; s++) {
         int slot, src;
         if (!DIR) {
             slot = kslot;
@@ -198,7 +199,7 @@ __device__ __forceinline__ void tile_body_sym(int2 bt, int tile, const T *__rest
 #pragma unroll
                     for (int k = -n; k <= n; k++) xv[nk(n, k)] = xl[amn(DIR ? -k : k, n, L)];
             }
-            if (s + 1 < S) pre.load(tab, tile, s + 1);
+            if (s + 1 < s1) pre.load(tab, tile, s + 1);
         } else {
             stage_group<T, L, NR>(tab, tile, s, grp);
         }
@@ -339,7 +340,7 @@ __device__ __forceinline__ void tile_body_sym(int2 bt, int tile, const T *__rest
         }
     }
     // partial for (target, partner block); m-flip back for the reverse wave
-    const int partner = DIR ? bt.x : bt.y;
+    const int partner = (DIR ? bt.x : bt.y) * G + g;  // one partial per (partner block, step group)
     cplx<T> *out = part + (((int64_t)partner * NR + rhs) * npad + tgt) * 2 * NB + mode * NB;
 #pragma unroll
     for (int n = 1; n <= L; n++)
@@ -361,7 +362,7 @@ __device__ __forceinline__ void tile_body_sym(int2 bt, int tile, const T *__rest
 template <typename T, int L, int DIR, int NR>
 __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, int tile, const cplx<T> *__restrict__ xin,
                                           cplx<T> *__restrict__ part, int npad, int rpair, T *__restrict__ grp,
-                                          const cplx<T> *__restrict__ srcl)
+                                          const cplx<T> *__restrict__ srcl, int s0, int s1, int G, int g)
 {
     constexpr bool STAGED = tile_staged<T, L>();
     constexpr bool PIPE = tile_pipelined<T, L, NR>();
@@ -379,10 +380,10 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
     for (int q = 0; q < NB; q++) W[q] = cmk((T)0, (T)0);
 
     GroupPrefetch<T, L, NR> pre;
-    if constexpr (PIPE) pre.load(tab, tile, 0);
+    if constexpr (PIPE) pre.load(tab, tile, s0);
 
 #pragma unroll 1
-    for (int s = 0; s < S; s++) {
+    for (int s = s0; s < s1; s++) {
         int slot, src;
         if (!DIR) {
             slot = kslot;
@@ -409,7 +410,7 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
 #pragma unroll
                     for (int k = -n; k <= n; k++) xv[nk(n, k)] = xl[amn(DIR ? -k : k, n, L)];
             }
-            if (s + 1 < S) pre.load(tab, tile, s + 1);
+            if (s + 1 < s1) pre.load(tab, tile, s + 1);
             __syncthreads();
         } else if constexpr (STAGED) {
             stage_group<T, L, NR>(tab, tile, s, grp);
@@ -509,7 +510,7 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
         }
     }
     // partial for (target, partner block); m-flip back for the reverse wave
-    const int partner = DIR ? bt.x : bt.y;
+    const int partner = (DIR ? bt.x : bt.y) * G + g;  // one partial per (partner block, step group)
     cplx<T> *out = part + (((int64_t)partner * NR + rhs) * npad + tgt) * 2 * NB + mode * NB;
 #pragma unroll
     for (int n = 1; n <= L; n++)
@@ -529,12 +530,15 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
 // (both polarizations' pairs at once): 4 waves; waves 0/1 serve right-hand
 // sides 0-1 and waves 2/3 sides 2-3 from the same table stream, so every
 // table byte read from HBM feeds four columns.
+// G workgroups share a tile (small clusters, G = 2..16): workgroup g takes
+// the steps [g S/G, (g+1) S/G) and writes its own partials.
 template <typename T, int L, int NR>
 __global__ __launch_bounds__(64 * NR) __attribute__((amdgpu_waves_per_eu(tile_waves<T, L, NR>()))) void direct_tile_kernel(
     const T *__restrict__ tab, const int2 *__restrict__ tiles, const cplx<T> *__restrict__ xin,
-    cplx<T> *__restrict__ part, int npad)
+    cplx<T> *__restrict__ part, int npad, int G)
 {
-    const int tile = blockIdx.x;
+    const int tile = blockIdx.x / G, g = blockIdx.x - tile * G;
+    const int s0 = g * (MSTM_SLOTS / G), s1 = s0 + MSTM_SLOTS / G;
     const int2 bt = tiles[tile];
     const int w = threadIdx.x >> 6;
     __shared__ __attribute__((aligned(16))) T grp[tile_lds_elems<T, L>()];
@@ -553,22 +557,23 @@ __global__ __launch_bounds__(64 * NR) __attribute__((amdgpu_waves_per_eu(tile_wa
         }
     }
     if constexpr (tile_staged<T, L>()) {
-        if (!(w & 1)) tile_body_sym<T, L, 0, NR>(bt, tile, tab, part, npad, w >> 1, grp, srcl, xin);
-        else if (bt.x != bt.y) tile_body_sym<T, L, 1, NR>(bt, tile, tab, part, npad, w >> 1, grp, srcl, xin);
+        if (!(w & 1)) tile_body_sym<T, L, 0, NR>(bt, tile, tab, part, npad, w >> 1, grp, srcl, xin, s0, s1, G, g);
+        else if (bt.x != bt.y)
+            tile_body_sym<T, L, 1, NR>(bt, tile, tab, part, npad, w >> 1, grp, srcl, xin, s0, s1, G, g);
     } else {
-        if (!(w & 1)) tile_body<T, L, 0, NR>(tab, bt, tile, xin, part, npad, w >> 1, grp, srcl);
-        else if (bt.x != bt.y) tile_body<T, L, 1, NR>(tab, bt, tile, xin, part, npad, w >> 1, grp, srcl);
+        if (!(w & 1)) tile_body<T, L, 0, NR>(tab, bt, tile, xin, part, npad, w >> 1, grp, srcl, s0, s1, G, g);
+        else if (bt.x != bt.y) tile_body<T, L, 1, NR>(tab, bt, tile, xin, part, npad, w >> 1, grp, srcl, s0, s1, G, g);
     }
     if (bt.x == bt.y && (w & 1))
     {
         // idle reverse wave of a diagonal tile: its share of the staging and the same barriers
         GroupPrefetch<T, L, NR> pre;
-        if constexpr (tile_pipelined<T, L, NR>()) pre.load(tab, tile, 0);
-        for (int s = 0; s < MSTM_SLOTS; s++) {
+        if constexpr (tile_pipelined<T, L, NR>()) pre.load(tab, tile, s0);
+        for (int s = s0; s < s1; s++) {
             __syncthreads();
             if constexpr (tile_pipelined<T, L, NR>()) {
                 pre.store(grp);
-                if (s + 1 < MSTM_SLOTS) pre.load(tab, tile, s + 1);
+                if (s + 1 < s1) pre.load(tab, tile, s + 1);
                 __syncthreads();
             } else if constexpr (tile_staged<T, L>()) {
                 stage_group<T, L, NR>(tab, tile, s, grp);
@@ -580,17 +585,20 @@ __global__ __launch_bounds__(64 * NR) __attribute__((amdgpu_waves_per_eu(tile_wa
 
 template <typename T>
 hipError_t launch_direct_tiles(const T *tab, const int2 *tiles, int ntiles, int L, const T *xin, T *partial, int npad,
-                               int nr, hipStream_t st)
+                               int nr, int G, hipStream_t st)
 {
     const cplx<T> *x = reinterpret_cast<const cplx<T> *>(xin);
     cplx<T> *p = reinterpret_cast<cplx<T> *>(partial);
-    dim3 g(ntiles);
     if (nr != 2 && nr != 4) return hipErrorInvalidValue;
+    if (G < 1 || G > MSTM_SLOTS || MSTM_SLOTS % G) return hipErrorInvalidValue;
+    dim3 g((unsigned)((int64_t)ntiles * G));
     switch (L * 8 + nr) {
 #define CASE(LL)                                                                                                       \
-    case LL * 8 + 2: hipLaunchKernelGGL((direct_tile_kernel<T, LL, 2>), g, dim3(128), 0, st, tab, tiles, x, p, npad);    \
+    case LL * 8 + 2:                                                                                                   \
+        hipLaunchKernelGGL((direct_tile_kernel<T, LL, 2>), g, dim3(128), 0, st, tab, tiles, x, p, npad, G);            \
         break;                                                                                                         \
-    case LL * 8 + 4: hipLaunchKernelGGL((direct_tile_kernel<T, LL, 4>), g, dim3(256), 0, st, tab, tiles, x, p, npad);    \
+    case LL * 8 + 4:                                                                                                   \
+        hipLaunchKernelGGL((direct_tile_kernel<T, LL, 4>), g, dim3(256), 0, st, tab, tiles, x, p, npad, G);            \
         break;
 #ifdef MSTM_TILE_L_ONLY
         CASE(MSTM_TILE_L_ONLY)
@@ -603,9 +611,9 @@ hipError_t launch_direct_tiles(const T *tab, const int2 *tiles, int ntiles, int 
     return hipGetLastError();
 }
 template hipError_t launch_direct_tiles<float>(const float *, const int2 *, int, int, const float *, float *, int, int,
-                                               hipStream_t);
+                                               int, hipStream_t);
 template hipError_t launch_direct_tiles<double>(const double *, const int2 *, int, int, const double *, double *, int,
-                                                int, hipStream_t);
+                                                int, int, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Input preparation (sphereinteraction :88-97): for a con_tran column

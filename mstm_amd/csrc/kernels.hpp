@@ -17,8 +17,10 @@ hipError_t launch_list_tables(T *tab, const double *rtran, int nslots, int LT, i
 
 // direct.hip
 template <typename T>
+// G (1, 2, 4, 8 or 16) workgroups per tile share its 16 steps; the partials
+// are then (partner block x G) per target
 hipError_t launch_direct_tiles(const T *tab, const int2 *tiles, int ntiles, int L, const T *xin, T *partial, int npad,
-                               int nr, hipStream_t st);
+                               int nr, int G, hipStream_t st);
 template <typename T>
 hipError_t launch_prep(const double *ain, int ldain, T *xin, int nrhs, const int *flags, int nsph, int npad, int Lp,
                        const int *order, const int *offset, const double *an, hipStream_t st);
@@ -41,6 +43,26 @@ hipError_t launch_qeff(const double *amnp, const double *pmnp, int ld, int nsph,
 
 // krylov.hip (all vectors complex FP64, lengths in complex elements)
 int krylov_grid();
+// Scalars of one BiCG system kept on the device (cbicg's csk, cak, cbk,
+// enorm and the convergence state), so that an iteration needs no host
+// reduction: the host reads the array once per iteration to steer.
+struct BicgScal {
+    cplx<double> csk, cak, cbk;
+    double enorm, err, eps;
+    int active, it;
+};
+// step 0: enorm = |b|^2 (active = enorm != 0); 1: csk = r.r (inactive if 0);
+// 2: den = <w, A p>, cak = csk / den (breakdown: inactive, it = iter);
+// 3: <q, r>, <r, r>: err, stop test, cbk, csk.  partials: region s of the
+// system s at partials + s * region (complex), one or two values per block.
+hipError_t launch_bicg_scal(int step, const double *partials, int64_t region, BicgScal *st, int ns, int iter,
+                            hipStream_t st_);
+hipError_t launch_bicg_a_dev(double *cap, double *caw, const double *cp, const double *cw, int n, double *partials,
+                             const BicgScal *sc, hipStream_t st);
+hipError_t launch_bicg_b_dev(double *anp, double *cr, double *cq, const double *cp, const double *cap,
+                             const double *caw, const BicgScal *sc, int n, double *partials, hipStream_t st);
+hipError_t launch_bicg_c_dev(double *cp, double *cw, const double *cr, const double *cq, const BicgScal *sc, int n,
+                             hipStream_t st);
 hipError_t launch_zdot(const double *a, const double *b, int n, int conj_a, double *partials, double *result,
                        hipStream_t st);
 hipError_t launch_bicg_a(double *cap, double *caw, const double *cp, const double *cw, int n, double *partials,

@@ -135,6 +135,165 @@ __global__ void zaxpby_kernel(cplx<double> a, const cplx<double> *__restrict__ x
 }
 
 int krylov_grid() { return KGRID; }
+
+// ---------------------------------------------------------------------------
+// Device-resident BiCG scalars (see BicgScal in kernels.hpp).  The vector
+// kernels of a system whose state is inactive return at once, so the host
+// may launch one iteration for both systems and learn only afterwards which
+// of them stopped.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(KB) void bicg_scal_kernel(int step, const cplx<double> *__restrict__ partials,
+                                                       int64_t region, BicgScal *__restrict__ sc, int ns, int iter)
+{
+    __shared__ cplx<double> red[2][KB];
+    const int nv = step == 3 ? 2 : 1;
+    for (int s = 0; s < ns; s++) {
+        BicgScal &S = sc[s];
+        const bool skip = step >= 1 && !S.active;  // uniform: read by all threads
+        if (!skip) {
+            const cplx<double> *p = partials + region * s;
+            for (int q = 0; q < nv; q++) {
+                cplx<double> v = cmk(0.0, 0.0);
+                for (int b = threadIdx.x; b < KGRID; b += KB) v = cadd(v, p[(int64_t)nv * b + q]);
+                red[q][threadIdx.x] = v;
+            }
+            __syncthreads();
+            for (int h = KB / 2; h > 0; h >>= 1) {
+                if (threadIdx.x < h)
+                    for (int q = 0; q < nv; q++) red[q][threadIdx.x] = cadd(red[q][threadIdx.x], red[q][threadIdx.x + h]);
+                __syncthreads();
+            }
+        }
+        if (threadIdx.x == 0) {
+            const cplx<double> v0 = red[0][0], v1 = red[1][0];
+            const cplx<double> zero = cmk(0.0, 0.0);
+            if (step == 0) {
+                S.enorm = v0.re;
+                S.active = v0.re != 0.0;
+                S.it = 0;
+                S.err = 0.0;
+                S.cak = S.cbk = zero;
+            } else if (skip) {
+                S.cak = S.cbk = zero;
+            } else if (step == 1) {
+                S.csk = v0;
+                if (v0.re == 0.0 && v0.im == 0.0) S.active = 0;  // it stays 0
+            } else if (step == 2) {
+                if (v0.re == 0.0 && v0.im == 0.0) {  // breakdown: the Fortran loop exits here
+                    S.active = 0;
+                    S.it = iter;
+                    S.cak = zero;
+                } else {
+                    S.cak = cdiv(S.csk, v0);
+                }
+            } else {
+                S.err = v1.re / S.enorm;
+                if (S.err < S.eps || (S.csk.re == 0.0 && S.csk.im == 0.0)) {
+                    S.active = 0;
+                    S.it = iter;
+                    S.cbk = zero;
+                } else {
+                    S.cbk = cdiv(v0, S.csk);
+                    S.csk = v0;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void bicg_a_dev_kernel(cplx<double> *__restrict__ cap, cplx<double> *__restrict__ caw,
+                                  const cplx<double> *__restrict__ cp, const cplx<double> *__restrict__ cw, int n,
+                                  cplx<double> *__restrict__ partials, const BicgScal *__restrict__ sc)
+{
+    if (!sc->active) return;
+    cplx<double> s = cmk(0.0, 0.0);
+    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) {
+        cplx<double> a = csub(cp[i], cap[i]);
+        cap[i] = a;
+        caw[i] = csub(cw[i], caw[i]);
+        cfma(s, cconj(cw[i]), a);
+    }
+    block_sum3(&s, 1);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void bicg_b_dev_kernel(cplx<double> *__restrict__ anp, cplx<double> *__restrict__ cr,
+                                  cplx<double> *__restrict__ cq, const cplx<double> *__restrict__ cp,
+                                  const cplx<double> *__restrict__ cap, const cplx<double> *__restrict__ caw,
+                                  const BicgScal *__restrict__ sc, int n, cplx<double> *__restrict__ partials)
+{
+    if (!sc->active) return;
+    const cplx<double> cak = sc->cak, cakc = cconj(cak);
+    cplx<double> s[2] = {cmk(0.0, 0.0), cmk(0.0, 0.0)};
+    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) {
+        cplx<double> a = anp[i];
+        cfma(a, cak, cp[i]);
+        anp[i] = a;
+        cplx<double> r = cr[i];
+        cfma(r, cscale(cak, -1.0), cap[i]);
+        cr[i] = r;
+        cplx<double> q = cq[i];
+        cfma(q, cscale(cakc, -1.0), caw[i]);
+        cq[i] = q;
+        cfma(s[0], cconj(q), r);
+        cfma(s[1], cconj(r), r);
+    }
+    block_sum3(s, 2);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = s[0];
+        partials[2 * blockIdx.x + 1] = s[1];
+    }
+}
+
+__global__ void bicg_c_dev_kernel(cplx<double> *__restrict__ cp, cplx<double> *__restrict__ cw,
+                                  const cplx<double> *__restrict__ cr, const cplx<double> *__restrict__ cq,
+                                  const BicgScal *__restrict__ sc, int n)
+{
+    if (!sc->active) return;
+    const cplx<double> cbk = sc->cbk, cbkc = cconj(cbk);
+    for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) {
+        cplx<double> p = cr[i];
+        cfma(p, cbk, cp[i]);
+        cp[i] = p;
+        cplx<double> w = cq[i];
+        cfma(w, cbkc, cw[i]);
+        cw[i] = w;
+    }
+}
+
+hipError_t launch_bicg_scal(int step, const double *partials, int64_t region, BicgScal *sc, int ns, int iter,
+                            hipStream_t st)
+{
+    if (ns < 1 || ns > 2 || step < 0 || step > 3) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(bicg_scal_kernel, dim3(1), dim3(KB), 0, st, step,
+                       reinterpret_cast<const cplx<double> *>(partials), region, sc, ns, iter);
+    return hipGetLastError();
+}
+hipError_t launch_bicg_a_dev(double *cap, double *caw, const double *cp, const double *cw, int n, double *partials,
+                             const BicgScal *sc, hipStream_t st)
+{
+    typedef cplx<double> Z;
+    hipLaunchKernelGGL(bicg_a_dev_kernel, dim3(KGRID), dim3(KB), 0, st, (Z *)cap, (Z *)caw, (const Z *)cp,
+                       (const Z *)cw, n, (Z *)partials, sc);
+    return hipGetLastError();
+}
+hipError_t launch_bicg_b_dev(double *anp, double *cr, double *cq, const double *cp, const double *cap,
+                             const double *caw, const BicgScal *sc, int n, double *partials, hipStream_t st)
+{
+    typedef cplx<double> Z;
+    hipLaunchKernelGGL(bicg_b_dev_kernel, dim3(KGRID), dim3(KB), 0, st, (Z *)anp, (Z *)cr, (Z *)cq, (const Z *)cp,
+                       (const Z *)cap, (const Z *)caw, sc, n, (Z *)partials);
+    return hipGetLastError();
+}
+hipError_t launch_bicg_c_dev(double *cp, double *cw, const double *cr, const double *cq, const BicgScal *sc, int n,
+                             hipStream_t st)
+{
+    typedef cplx<double> Z;
+    hipLaunchKernelGGL(bicg_c_dev_kernel, dim3(KGRID), dim3(KB), 0, st, (Z *)cp, (Z *)cw, (const Z *)cr,
+                       (const Z *)cq, sc, n);
+    return hipGetLastError();
+}
 hipError_t launch_bicg_a(double *cap, double *caw, const double *cp, const double *cw, int n, double *partials,
                          hipStream_t st)
 {
