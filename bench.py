@@ -368,6 +368,33 @@ def bench_lu(args):
     ctx.close()
 
 
+def small_cluster_solve(reps: int = 20):
+    """Solve latency of the 20-sphere cluster (x = 2, m = 1.5+0.01i, seeded
+    RSA in a sphere at fv 0.10, seed 1): plain FP64 BiCG with direct
+    translation to |r|^2/|b|^2 < 1e-6, both polarizations; median of reps
+    solves after two warm-up solves (tables built once, outside)."""
+    from mstm_amd import geometry as G
+    from mstm_amd.solver import FP64, HipContext
+
+    c = G.rsa_in_sphere(20, 2.0, 0.10, seed=1, ri=1.5 + 0.01j)
+    ctx = HipContext(0)
+    try:
+        order, _ = ctx.set_spheres(c.pos, c.radius, c.ri, 1.0, 1e-6)
+        ctx.setup_direct(FP64)
+        for _ in range(2):
+            r = ctx.fixedorsoln(eps=1e-6, niter=10000)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            r = ctx.fixedorsoln(eps=1e-6, niter=10000)
+            ts.append(time.perf_counter() - t0)
+        return dict(workload="20 spheres x=2.0 m=1.5+0.01i RSA sphere fv=0.10 seed=1, direct, plain FP64 BiCG, 2 pol",
+                    s_per_solve=round(float(np.median(ts)), 6), iterations=r["iterations"], maxerr=r["maxerr"],
+                    order=int(order.max()), number_eqns=ctx.number_eqns, reps=reps)
+    finally:
+        ctx.close()
+
+
 def observe(ctx):
     """Single-origin observables of the solution the last fixedorsoln left on the device."""
     ob = ctx.observables(None)
@@ -532,8 +559,14 @@ def main():
     # both are iterating; the roofline is quoted on that launch when present
     ncol = 4 if st_lig.get("tile4_launches_fp32", 0) > 0 else 2
     vec_bytes = ncol * 2 * ne * 8
-    alg32 = npairs * table_bytes_per_pair(L, 4) + vec_bytes
-    alg64 = npairs * table_bytes_per_pair(L, 8) + (plain["tile_columns"] if plain else 2) * 2 * ne * 16
+    # algorithmic bytes of one product: the minimal per-pair table (rotation
+    # rows sum_n (2n+1)^2 reals + both modes' axial blocks) at the stored
+    # precision for every unordered pair, plus the vectors; the tiles store
+    # and read the symmetric half of it (stored_bytes)
+    alg32 = npairs * full_table_bytes_per_pair(L, 4) + vec_bytes
+    st32 = npairs * table_bytes_per_pair(L, 4) + vec_bytes
+    cols64 = plain["tile_columns"] if plain else 2
+    alg64 = npairs * full_table_bytes_per_pair(L, 8) + cols64 * 2 * ne * 16
     if ncol == 4:
         t32 = st_lig["tile4_ms_fp32"] / st_lig["tile4_launches_fp32"]
     else:
@@ -544,27 +577,35 @@ def main():
     if os.path.exists(pmc):
         rec = json.load(open(pmc)).get("by_columns", {}).get(str(ncol))
         traffic = rec["hbm_bytes_per_launch"] if rec else None
-    fl32 = npairs * tile_flops_per_pair(L, ncol, 4)
-    surv32 = npairs * full_table_bytes_per_pair(L, 4) + vec_bytes
+    fl32 = npairs * full_tile_flops_per_pair(L, ncol)
+    ex32 = npairs * tile_flops_per_pair(L, ncol, 4)
     roof = dict(kernel=f"direct_tile_kernel<float,{L},{ncol}>", bound="hbm", achieved=round(ach32, 1),
                 peak=HBM_PEAK_GBPS, unit="GB/s", frac=round(ach32 / HBM_PEAK_GBPS, 4), traffic=traffic,
                 algorithmic_bytes=alg32, columns=ncol, avg_launch_ms=round(t32, 4),
+                stored_bytes=st32, stored_GBps=round(st32 / (t32 * 1e-3) / 1e9, 1),
                 flops_per_launch=fl32, tflops=round(fl32 / (t32 * 1e-3) / 1e12, 2),
+                executed_flops=ex32, executed_tflops=round(ex32 / (t32 * 1e-3) / 1e12, 2),
                 flop_per_byte=round(fl32 / alg32, 2),
-                full_table_bytes=surv32, full_table_GBps=round(surv32 / (t32 * 1e-3) / 1e9, 1),
-                note=("algorithmic bytes are the symmetric tables the kernel reads (blocks m >= 0, half-sum/"
-                      "half-difference Wigner rows, one axial block per mode): half of the full rotate-axial "
-                      "table (full_table_bytes, the count of the unsymmetrized algorithm); tables staged in LDS "
-                      "once per tile step; packed FP32 is disabled on this pool, so the FP32 vector peak is "
-                      "78.6 TF and the ridge 9.8 flop/B; SQ counters in profiles/direct_tile_cfg5_sq_counters.txt"))
+                note=("algorithmic bytes and flops per launch = the per-pair minimal table "
+                      f"({full_table_bytes_per_pair(L, 4)} B at order {L} in FP32) and full-rotation flops "
+                      "16 (S_rot + S_ax) per pair, direction and column, times the unordered pairs, plus the "
+                      "vectors; the kernel stores and reads the symmetric half-table (stored_bytes: blocks "
+                      "m >= 0, half-sum/half-difference Wigner rows, one axial block per mode), staged in LDS "
+                      "once per tile step, and executes executed_flops; traffic is the measured HBM bytes; "
+                      "packed FP32 is disabled, so the FP32 vector peak is 78.6 TF/s"))
     matvec = dict(fp32_tile_ms=round(t32, 4), fp32_columns=ncol, fp32_hbm_GBps=round(ach32, 1))
     if st_lig.get("tile_launches_fp32", 0) > 0:
         t2 = st_lig["tile_ms_fp32"] / st_lig["tile_launches_fp32"]
-        a2 = npairs * table_bytes_per_pair(L, 4) + 2 * 2 * ne * 8
+        a2 = npairs * full_table_bytes_per_pair(L, 4) + 2 * 2 * ne * 8
         matvec.update(fp32_2col_tile_ms=round(t2, 4), fp32_2col_hbm_GBps=round(a2 / (t2 * 1e-3) / 1e9, 1))
     if plain is not None:
         t64 = plain["tile_ms_avg"]
-        matvec.update(fp64_tile_ms=round(t64, 4), fp64_hbm_GBps=round(alg64 / (t64 * 1e-3) / 1e9, 1))
+        matvec.update(fp64_tile_ms=round(t64, 4), fp64_columns=cols64,
+                      fp64_hbm_GBps=round(alg64 / (t64 * 1e-3) / 1e9, 1),
+                      fp64_frac=round(alg64 / (t64 * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4))
+
+    small = small_cluster_solve()
+    print(f"[bench] 20-sphere solve: {small['s_per_solve'] * 1e3:.3f} ms", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -587,6 +628,7 @@ def main():
         "config": {"workload": f"{cl.n} spheres x=2.0 m=1.31 RSA sphere fv=0.30 seed=5, direct translation, "
                                "plane wave +z, 2 pol, |r|^2/|b|^2<1e-6 (learned start + mixed FP32/FP64)",
                    "nsphere": cl.n, "order": L, "number_eqns": ne, "pairs": npairs},
+        "time_to_solution_s": round(t_tables + lig_s, 4),
         "plain_fp64": plain,
         "speedup_vs_plain_fp64": round(plain["s_per_solve"] / lig_s, 3) if plain else None,
         "learned": dict(iterations_last_pol=res["iterations"], maxerr=res["maxerr"],
@@ -613,6 +655,7 @@ def main():
                             note="post-solve, on the GPU: translation orders (host), merge to the common origin, "
                                  "scattering matrix at 361 directions, s00 moments"),
         "matvec": matvec,
+        "small_cluster": small,
         "setup_s": dict(geometry=round(t_geom, 2), tables_fp64_fp32=round(t_tables, 3),
                         tables_fp64_ms=st0["tables_ms_fp64"], tables_fp32_ms=st0["tables_ms_fp32"],
                         table_bytes_fp64=st0["table_bytes_fp64"], table_bytes_fp32=st0["table_bytes_fp32"]),

@@ -103,47 +103,46 @@ MSTM_HD constexpr int slot_len(int LT, int LS, int VW) { return phi_off(LT, LS, 
 //   d^n(-m,-k) = (-1)^{m+k} d^n(m,k)      (Wigner d)
 //   A_{-m}(t,s) of mode p = A_m(t,s) of the other mode (axial coefficients;
 //   for m = 0 both modes coincide).
-// Only blocks m = 0..L are stored.  Block m holds
-//   ROT  per degree n = m1..L the half-sum and half-difference rows
+// Only blocks m = 0..L are stored:
+//   ROT  degree by degree, n = 1..L; degree n holds the rows of blocks
+//        m = 0..n: the half-sum and half-difference rows
 //          E(n,0) = d^n(m,0),  E(n,k) = (d^n(m,k) + (-1)^k d^n(m,-k)) / 2,
 //          O(n,k) = (d^n(m,k) - (-1)^k d^n(m,-k)) / 2        (k = 1..n)
-//        stored E(n,0..n) then O(n,1..n); for m = 0 only E (= d^n(0,k))
-//   AX   A_m(t,s), t fastest, for mode 0 and (m > 0) mode 1
-// each section padded to VW reals; then PHI (cos, sin).  With
-// u_k = x_k + (-1)^k x_{-k}, v_k = x_k - (-1)^k x_{-k} (k > 0), u_0 = x_0,
-// the rotation y_m = sum_k d(m,k) x_k of blocks m and -m becomes
+//        stored E(n,0..n) then O(n,1..n); for m = 0 only E (= d^n(0,k));
+//        the section is padded once, to VW reals
+//   AX   per block m = 0..L: A_m(t,s), t fastest, for mode 0 and (m > 0)
+//        mode 1, each padded to VW reals
+// then PHI (cos, sin).  With u_k = x_k + (-1)^k x_{-k},
+// v_k = x_k - (-1)^k x_{-k} (k > 0), u_0 = x_0, the rotation
+// y_m = sum_k d(m,k) x_k of blocks m and -m becomes
 //   Y+ = sum_{k>=0} E_k u_k,  Y- = sum_{k>0} O_k v_k,
 //   y_m = Y+ + Y-,  y_{-m} = (-1)^m (Y+ - Y-)
 // and the back rotation accumulates P_k = sum E_k (z_m + (-1)^m z_{-m}),
 // Q_k = sum O_k (z_m - (-1)^m z_{-m}), from which U_k = P_k + Q_k,
-// U_{-k} = (-1)^k (P_k - Q_k), U_0 = P_0.
+// U_{-k} = (-1)^k (P_k - Q_k), U_0 = P_0.  Both rotations of a degree read
+// the same contiguous rows, so a tile step runs degree by degree (forward),
+// block by block (axial), degree by degree (back).
 // ---------------------------------------------------------------------------
-MSTM_HD constexpr int srot_len(int m, int L)
+MSTM_HD constexpr int sdeg_len(int n) { return (n + 1) + n * (2 * n + 1); }
+MSTM_HD constexpr int sdeg_off(int n)
 {
-    int m1 = imx(1, m), s = 0;
-    for (int n = m1; n <= L; n++) s += (m == 0) ? n + 1 : 2 * n + 1;
+    int s = 0;
+    for (int q = 1; q < n; q++) s += sdeg_len(q);
     return s;
 }
-// offset of degree n inside block m's ROT section
-MSTM_HD constexpr int srot_deg(int m, int n)
-{
-    int m1 = imx(1, m), s = 0;
-    for (int q = m1; q < n; q++) s += (m == 0) ? q + 1 : 2 * q + 1;
-    return s;
-}
+// row of block m (0 <= m <= n) of degree n inside the ROT section
+MSTM_HD constexpr int srow_off(int m, int n) { return sdeg_off(n) + (m == 0 ? 0 : (n + 1) + (m - 1) * (2 * n + 1)); }
+MSTM_HD constexpr int srow_len(int m, int n) { return m == 0 ? n + 1 : 2 * n + 1; }
+MSTM_HD constexpr int srot_tot(int L) { return sdeg_off(L + 1); }
 MSTM_HD constexpr int sax_len(int m, int L) { return 2 * (L - imx(1, m) + 1) * (L - imx(1, m) + 1); }  // one mode
 MSTM_HD constexpr int sax_modes(int m) { return m == 0 ? 1 : 2; }
-MSTM_HD constexpr int ssec_len(int m, int L, int VW)
+MSTM_HD constexpr int sax_off(int m, int L, int VW)
 {
-    return pad_to(srot_len(m, L), VW) + sax_modes(m) * pad_to(sax_len(m, L), VW);
-}
-MSTM_HD constexpr int ssec_off(int m, int L, int VW)
-{
-    int o = 0;
-    for (int q = 0; q < m; q++) o += ssec_len(q, L, VW);
+    int o = pad_to(srot_tot(L), VW);
+    for (int q = 0; q < m; q++) o += sax_modes(q) * pad_to(sax_len(q, L), VW);
     return o;
 }
-MSTM_HD constexpr int sphi_off(int L, int VW) { return ssec_off(L + 1, L, VW); }
+MSTM_HD constexpr int sphi_off(int L, int VW) { return sax_off(L + 1, L, VW); }
 MSTM_HD constexpr int sslot_len(int L, int VW) { return sphi_off(L, VW) + VW; }
 // the direct tiles use the symmetric layout, staged in LDS, up to order 5
 // (FP32) / 4 (FP64); larger orders keep the full layout, whose unstaged form

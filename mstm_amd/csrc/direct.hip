@@ -77,23 +77,58 @@ __device__ __forceinline__ void load_sec(const T *__restrict__ p0, T *r)
     }
 }
 
+// Load LEN elements of a slot starting at element E0 (any alignment): the
+// 16-byte chunks that cover them, unused lanes of the end chunks dropped
+template <typename T, int E0, int LEN>
+__device__ __forceinline__ void load_span(const T *__restrict__ p0, T *r)
+{
+    constexpr int VW = 16 / sizeof(T);
+    typedef typename vec_of<T, VW>::type V;
+#pragma unroll
+    for (int c = E0 / VW; c <= (E0 + LEN - 1) / VW; c++) {
+        const V v = *reinterpret_cast<const V *>(p0 + (int64_t)c * MSTM_SLOTS * VW);
+#pragma unroll
+        for (int q = 0; q < VW; q++) {
+            const int e = c * VW + q - E0;
+            if (e >= 0 && e < LEN) r[e] = v[q];
+        }
+    }
+}
+
 // LDS staging of a step's table group (see the file header): exactly the
 // orders with the symmetric table layout
 template <typename T, int L> constexpr bool tile_staged() { return direct_sym(L, (int)sizeof(T)); }
-// two-column FP32 tiles up to order 4 keep the tile's two source blocks in
-// LDS, and software-pipeline the staging: step s+1's group is loaded into
-// registers while step s computes
-template <typename T, int L, int NR> constexpr bool tile_src_staged() { return L <= 4; }
+// Tiles up to order 4 keep the tile's two source blocks in LDS (instead of
+// every lane reloading its source sphere from L2 in every step).  FP32 tiles
+// also software-pipeline the staging: step s+1's group is loaded into
+// registers while step s computes (FP64 has no registers to spare for it).
+// NR here is the number of columns of one workgroup.
+template <typename T, int L, int NR> constexpr bool tile_src_staged()
+{
+#ifdef MSTM_TILE_SRC64
+    if (sizeof(T) == 8) return L <= 4 && MSTM_TILE_SRC64;
+#endif
+    return L <= 4;
+}
 template <typename T, int L, int NR> constexpr bool tile_pipelined()
 {
-    return sizeof(T) == 4 && NR == 2 && tile_src_staged<T, L, NR>();
+#ifdef MSTM_TILE_PIPE64
+    if (sizeof(T) == 8) return MSTM_TILE_PIPE64 && tile_src_staged<T, L, NR>();
+#endif
+    return sizeof(T) == 4 && tile_src_staged<T, L, NR>();
 }
-// occupancy target: FP32 tiles up to order 4 fit 2 waves/SIMD without
-// spilling (the pipelined form holds its prefetch in the full register file
-// at 1 wave/SIMD instead); larger states use the full register file
+// occupancy target handed to the compiler: tiles with staged sources are
+// scheduled as if alone on their SIMD (the FP32 prefetch then overlaps the
+// step, and FP64's 80 KB of LDS per workgroup allows one wave per SIMD
+// anyway; FP32 still fits two), other tiles up to order 4 for two waves per
+// SIMD, larger orders the full register file
 template <typename T, int L, int NR> constexpr int tile_waves()
 {
-    return tile_pipelined<T, L, NR>() ? 1 : ((sizeof(T) == 4 && L <= 4) ? 2 : 1);
+#ifdef MSTM_TILE_WAVES
+    return MSTM_TILE_WAVES;
+#else
+    return tile_src_staged<T, L, NR>() ? 1 : (L <= 4 ? 2 : 1);
+#endif
 }
 // register prefetch of a table group for the pipelined form
 template <typename T, int L, int NR> struct GroupPrefetch {
@@ -143,24 +178,40 @@ __device__ __forceinline__ void stage_group(const T *__restrict__ tab, int tile,
     constexpr int VW = 16 / sizeof(T);
     constexpr int NV = direct_slot_len(L, sizeof(T)) * MSTM_SLOTS / VW;
     typedef typename vec_of<T, VW>::type V;
+    constexpr int NT = 64 * NR, NQ = (NV + NT - 1) / NT;
     const V *g = reinterpret_cast<const V *>(tab + ((int64_t)tile * MSTM_SLOTS + s) *
                                                        (int64_t)direct_slot_len(L, sizeof(T)) * MSTM_SLOTS);
     V *d = reinterpret_cast<V *>(grp);
-    for (int q = threadIdx.x; q < NV; q += 64 * NR) d[q] = g[q];
+    // all of this thread's loads in flight at once, then the LDS stores
+    V r[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const int i = threadIdx.x + q * NT;
+        if (i < NV) r[q] = g[i];
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const int i = threadIdx.x + q * NT;
+        if (i < NV) d[i] = r[q];
+    }
 }
 // One wave's work on a tile with the symmetric table layout (staged in LDS);
 // DIR = 0 forward (I <- J), 1 reverse (J <- I); the wave serves right-hand
-// sides 2*rpair and 2*rpair+1 of NR.  Per step and m = 0..L: the rotation of
-// blocks m and -m from the half-sum/half-difference rows (see common.hpp),
-// the axial products of both (A_{-m} = A_m of the other mode), and the back
-// rotation into P/Q; the output phase is applied once per coefficient.
-template <typename T, int L, int DIR, int NR>
+// sides 2*rpair and 2*rpair+1 of NR.  A step runs in three passes over one
+// vector Y of NB values per lane: degree by degree, the phased source
+// coefficients are folded into u/v and rotated into the blocks m and -m
+// (Y(n,m)); block by block, the axial products replace Y(.,m) and Y(.,-m)
+// (A_{-m} = A_m of the other mode); degree by degree, the back rotation of
+// Y(t,.) gives P/Q and the output phase adds them into W(t,.).  Only W and
+// Y live across the passes.
+template <typename T, int L, int DIR, int NR, int NRL>
 __device__ __forceinline__ void tile_body_sym(int2 bt, int tile, const T *__restrict__ tab, cplx<T> *__restrict__ part,
-                                              int npad, int rpair, T *__restrict__ grp,
+                                              int npad, int rpair, int r0, T *__restrict__ grp,
                                               const cplx<T> *__restrict__ srcl, const cplx<T> *__restrict__ xin,
                                               int s0, int s1, int G, int g)
 {
-    constexpr bool PIPE = tile_pipelined<T, L, NR>();
+    constexpr bool PIPE = tile_pipelined<T, L, NRL>();
+    constexpr bool SRC = tile_src_staged<T, L, NRL>();
     constexpr int VW = 16 / sizeof(T);
     constexpr int NB = L * (L + 2);
     constexpr int S = MSTM_SLOTS;
@@ -173,7 +224,7 @@ __device__ __forceinline__ void tile_body_sym(int2 bt, int tile, const T *__rest
 #pragma unroll
     for (int q = 0; q < NB; q++) W[q] = cmk((T)0, (T)0);
 
-    GroupPrefetch<T, L, NR> pre;
+    GroupPrefetch<T, L, NRL> pre;
     if constexpr (PIPE) pre.load(tab, tile, s0);
 
 #pragma unroll 1
@@ -188,7 +239,7 @@ __device__ __forceinline__ void tile_body_sym(int2 bt, int tile, const T *__rest
         }
         const cplx<T> *xs = xin + ((int64_t)(rhs * npad + src) * 2 + mode) * NB;
         // staged copy: block 0 = bt.y (forward sources), 1 = bt.x (reverse sources)
-        const cplx<T> *xl = srcl + (DIR * S + (src % S)) * (2 * NR * NB + SRC_PAD) + (rhs * 2 + mode) * NB;
+        const cplx<T> *xl = srcl + (DIR * S + (src % S)) * (2 * NRL * NB + SRC_PAD) + ((rhs - r0) * 2 + mode) * NB;
         __syncthreads();
         cplx<T> xv[PIPE ? NB : 1];
         if constexpr (PIPE) {
@@ -201,143 +252,138 @@ __device__ __forceinline__ void tile_body_sym(int2 bt, int tile, const T *__rest
             }
             if (s + 1 < s1) pre.load(tab, tile, s + 1);
         } else {
-            stage_group<T, L, NR>(tab, tile, s, grp);
+            stage_group<T, L, NRL>(tab, tile, s, grp);
         }
         __syncthreads();
         if (diag && s == 0) continue;
         const T *p0 = grp + slot * VW;
-        // exp(i sigma k phi), sigma = +1 forward, -1 reverse (im of :6690)
-        T pv[2];
-        load_sec<T, sphi_off(L, VW), 2>(p0, pv);
-        cplx<T> e1 = cmk(pv[0], DIR ? -pv[1] : pv[1]);
-        cplx<T> ph[2 * L + 1];
-        ph[L] = cmk((T)1, (T)0);
+        // exp(i sigma k phi), k = 0..L; sigma = +1 forward, -1 reverse (im of :6690)
+        auto phases = [&](cplx<T> *ph) {
+            T pv[2];
+            load_sec<T, sphi_off(L, VW), 2>(p0, pv);
+            const cplx<T> e1 = cmk(pv[0], DIR ? -pv[1] : pv[1]);
+            ph[0] = cmk((T)1, (T)0);
 #pragma unroll
-        for (int k = 1; k <= L; k++) {
-            ph[L + k] = cmul(ph[L + k - 1], e1);
-            ph[L - k] = cconj(ph[L + k]);
-        }
-        // source coefficients (m-flip for the reverse direction, phase), then
-        // in place u_k at k >= 0 and v_k at -k
-        cplx<T> xp[NB];
-#pragma unroll
-        for (int n = 1; n <= L; n++)
-#pragma unroll
-            for (int k = -n; k <= n; k++) {
-                cplx<T> v;
-                if constexpr (PIPE) v = xv[nk(n, k)];
-                else if constexpr (tile_src_staged<T, L, NR>()) v = xl[amn(DIR ? -k : k, n, L)];
-                else v = xs[amn(DIR ? -k : k, n, L)];
-                if (DIR && (k & 1)) v = cmk(-v.re, -v.im);
-                xp[nk(n, k)] = cmul(ph[L + k], v);
-            }
-#pragma unroll
-        for (int n = 1; n <= L; n++)
+            for (int k = 1; k <= L; k++) ph[k] = cmul(ph[k - 1], e1);
+        };
+        cplx<T> ph[L + 1];
+        phases(ph);
+        // source coefficient (n,k) of this lane: m-flip for the reverse direction
+        auto xsrc = [&](int n, int k) -> cplx<T> {
+            cplx<T> v;
+            if constexpr (PIPE) v = xv[nk(n, k)];
+            else if constexpr (SRC) v = xl[amn(DIR ? -k : k, n, L)];
+            else v = xs[amn(DIR ? -k : k, n, L)];
+            if (DIR && (k & 1)) v = cmk(-v.re, -v.im);
+            return v;
+        };
+        cplx<T> Y[NB];
+#ifdef MSTM_TILE_FINE
+#define TB_FENCE asm volatile("" ::: "memory")
+#elif defined(MSTM_TILE_FINE64)
+#define TB_FENCE if constexpr (sizeof(T) == 8) asm volatile("" ::: "memory")
+#else
+#define TB_FENCE
+#endif
+        // ---- forward rotation, degree by degree
+        static_for<L>([&](auto nc) {
+            constexpr int n = decltype(nc)::value + 1;
+            cplx<T> u[n + 1], v[n + 1];
+            u[0] = xsrc(n, 0);
 #pragma unroll
             for (int k = 1; k <= n; k++) {
-                const cplx<T> a = xp[nk(n, k)], b = xp[nk(n, -k)];
-                if (k & 1) {
-                    xp[nk(n, k)] = csub(a, b);
-                    xp[nk(n, -k)] = cadd(a, b);
-                } else {
-                    xp[nk(n, k)] = cadd(a, b);
-                    xp[nk(n, -k)] = csub(a, b);
-                }
+                const cplx<T> a = cmul(ph[k], xsrc(n, k)), b = cmul(cconj(ph[k]), xsrc(n, -k));
+                u[k] = (k & 1) ? csub(a, b) : cadd(a, b);
+                v[k] = (k & 1) ? cadd(a, b) : csub(a, b);
             }
-        // back-rotation accumulators: P_k at k >= 0, Q_k at -k
-        cplx<T> PQ[NB];
-#pragma unroll
-        for (int q = 0; q < NB; q++) PQ[q] = cmk((T)0, (T)0);
-        static_for<L + 1>([&](auto mc) {
-            constexpr int m = decltype(mc)::value;
-            constexpr int m1 = imx(1, m);
-            constexpr int E0 = ssec_off(m, L, VW);
-            constexpr int RL = srot_len(m, L);
-            constexpr int AL = sax_len(m, L);
-            constexpr int AP = pad_to(AL, VW);
-            constexpr int ND = L - m1 + 1;
-            constexpr T sgm = (m & 1) ? (T)-1 : (T)1;
-            T dr[RL];
-            load_sec<T, E0, RL>(p0, dr);
-            // forward rotation of blocks m (y) and -m (yn)
-            cplx<T> y[ND], yn[ND];
-#pragma unroll
-            for (int n = m1; n <= L; n++) {
-                const int ro = srot_deg(m, n);
+            static_for<n + 1>([&](auto mc) {
+                constexpr int m = decltype(mc)::value;
+                T dr[srow_len(m, n)];
+                load_span<T, srow_off(m, n), srow_len(m, n)>(p0, dr);
                 cplx<T> yp = cmk((T)0, (T)0);
 #pragma unroll
-                for (int k = 0; k <= n; k++) crfma(yp, dr[ro + k], xp[nk(n, k)]);
+                for (int k = 0; k <= n; k++) crfma(yp, dr[k], u[k]);
                 if constexpr (m == 0) {
-                    y[n - m1] = yp;
+                    Y[nk(n, 0)] = yp;
                 } else {
                     cplx<T> ym = cmk((T)0, (T)0);
 #pragma unroll
-                    for (int k = 1; k <= n; k++) crfma(ym, dr[ro + n + k], xp[nk(n, -k)]);
-                    y[n - m1] = cadd(yp, ym);
-                    yn[n - m1] = cscale(csub(yp, ym), sgm);
+                    for (int k = 1; k <= n; k++) crfma(ym, dr[n + k], v[k]);
+                    Y[nk(n, m)] = cadd(yp, ym);
+                    Y[nk(n, -m)] = (m & 1) ? csub(ym, yp) : csub(yp, ym);
                 }
-            }
-            // axial products: A_m of this lane's mode for m, of the other mode for -m
-            // (transposed for the reverse direction)
-            cplx<T> z[ND], zn[ND];
-            {
+            });
+            TB_FENCE;
+        });
+        // the back pass reloads its rows from LDS rather than keeping the
+        // forward pass's copies live across the axial products
+        asm volatile("" ::: "memory");
+        // ---- axial products, block by block (transposed for the reverse direction)
+        static_for<L + 1>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            constexpr int m1 = imx(1, m);
+            constexpr int AL = sax_len(m, L);
+            constexpr int AP = pad_to(AL, VW);
+            constexpr int ND = L - m1 + 1;
+            static_for<(m == 0 ? 1 : 2)>([&](auto hc) {
+                constexpr int h = decltype(hc)::value;  // 0: block m (this mode), 1: block -m (other mode)
+                constexpr int mm = h ? -m : m;
                 T da[AL];
-                load_sec<T, E0 + pad_to(RL, VW), AL>(p0 + (int64_t)(m == 0 ? 0 : mode) * (AP / VW) * S * VW, da);
+                const int md = m == 0 ? 0 : (h ? 1 - mode : mode);
+                load_sec<T, sax_off(m, L, VW), AL>(p0 + (int64_t)md * (AP / VW) * S * VW, da);
+                cplx<T> z[ND];
 #pragma unroll
                 for (int t = 0; t < ND; t++) {
                     cplx<T> acc = cmk((T)0, (T)0);
 #pragma unroll
                     for (int q = 0; q < ND; q++) {
                         const int e = DIR ? 2 * (q + ND * t) : 2 * (t + ND * q);
-                        cfma(acc, cmk(da[e], da[e + 1]), y[q]);
+                        cfma(acc, cmk(da[e], da[e + 1]), Y[nk(q + m1, mm)]);
                     }
                     z[t] = acc;
                 }
-            }
-            if constexpr (m > 0) {
-                T db[AL];
-                load_sec<T, E0 + pad_to(RL, VW), AL>(p0 + (int64_t)(1 - mode) * (AP / VW) * S * VW, db);
 #pragma unroll
-                for (int t = 0; t < ND; t++) {
-                    cplx<T> acc = cmk((T)0, (T)0);
-#pragma unroll
-                    for (int q = 0; q < ND; q++) {
-                        const int e = DIR ? 2 * (q + ND * t) : 2 * (t + ND * q);
-                        cfma(acc, cmk(db[e], db[e + 1]), yn[q]);
-                    }
-                    zn[t] = acc;
-                }
-            }
-            // back rotation: P_k += E_k (z_m + (-1)^m z_-m), Q_k += O_k (z_m - (-1)^m z_-m)
-#pragma unroll
-            for (int t = m1; t <= L; t++) {
-                const int ro = srot_deg(m, t);
-                cplx<T> za = z[t - m1], zb = z[t - m1];
-                if constexpr (m > 0) {
-                    za = cadd(z[t - m1], cscale(zn[t - m1], sgm));
-                    zb = csub(z[t - m1], cscale(zn[t - m1], sgm));
-                }
-#pragma unroll
-                for (int k = 0; k <= t; k++) crfma(PQ[nk(t, k)], dr[ro + k], za);
-                if constexpr (m > 0) {
-#pragma unroll
-                    for (int k = 1; k <= t; k++) crfma(PQ[nk(t, -k)], dr[ro + t + k], zb);
-                }
-            }
+                for (int t = 0; t < ND; t++) Y[nk(t + m1, mm)] = z[t];
+            });
+            TB_FENCE;
         });
-        // U_0 = P_0, U_k = P_k + Q_k, U_-k = (-1)^k (P_k - Q_k); output phase once per coefficient
+        asm volatile("" ::: "memory");
+        // ---- back rotation and output phase, degree by degree
+        static_for<L>([&](auto tc) {
+            constexpr int t = decltype(tc)::value + 1;
+            cplx<T> P[t + 1], Q[t + 1];
 #pragma unroll
-        for (int t = 1; t <= L; t++) {
-            cfma(W[nk(t, 0)], ph[L], PQ[nk(t, 0)]);
+            for (int k = 0; k <= t; k++) P[k] = Q[k] = cmk((T)0, (T)0);
+            static_for<t + 1>([&](auto mc) {
+                constexpr int m = decltype(mc)::value;
+                T dr[srow_len(m, t)];
+                load_span<T, srow_off(m, t), srow_len(m, t)>(p0, dr);
+                if constexpr (m == 0) {
+                    const cplx<T> za = Y[nk(t, 0)];
+#pragma unroll
+                    for (int k = 0; k <= t; k++) crfma(P[k], dr[k], za);
+                } else {
+                    const cplx<T> z = Y[nk(t, m)], zn = Y[nk(t, -m)];
+                    const cplx<T> za = (m & 1) ? csub(z, zn) : cadd(z, zn);
+                    const cplx<T> zb = (m & 1) ? cadd(z, zn) : csub(z, zn);
+#pragma unroll
+                    for (int k = 0; k <= t; k++) crfma(P[k], dr[k], za);
+#pragma unroll
+                    for (int k = 1; k <= t; k++) crfma(Q[k], dr[t + k], zb);
+                }
+            });
+            // U_0 = P_0, U_k = P_k + Q_k, U_-k = (-1)^k (P_k - Q_k)
+            W[nk(t, 0)] = cadd(W[nk(t, 0)], P[0]);
 #pragma unroll
             for (int k = 1; k <= t; k++) {
-                const cplx<T> pk = PQ[nk(t, k)], qk = PQ[nk(t, -k)];
-                const cplx<T> up = cadd(pk, qk);
-                const cplx<T> um = (k & 1) ? csub(qk, pk) : csub(pk, qk);
-                cfma(W[nk(t, k)], ph[L - k], up);
-                cfma(W[nk(t, -k)], ph[L + k], um);
+                const cplx<T> up = cadd(P[k], Q[k]);
+                const cplx<T> um = (k & 1) ? csub(Q[k], P[k]) : csub(P[k], Q[k]);
+                cfma(W[nk(t, k)], cconj(ph[k]), up);
+                cfma(W[nk(t, -k)], ph[k], um);
             }
-        }
+            TB_FENCE;
+        });
+#undef TB_FENCE
     }
     // partial for (target, partner block); m-flip back for the reverse wave
     const int partner = (DIR ? bt.x : bt.y) * G + g;  // one partial per (partner block, step group)
@@ -359,13 +405,13 @@ __device__ __forceinline__ void tile_body_sym(int2 bt, int tile, const T *__rest
 // One wave's work on a tile with the full table layout (orders above the
 // symmetric range): tables read from L2/HBM directly, output phase folded
 // into every term; DIR and rpair as for tile_body_sym.
-template <typename T, int L, int DIR, int NR>
+template <typename T, int L, int DIR, int NR, int NRL>
 __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, int tile, const cplx<T> *__restrict__ xin,
-                                          cplx<T> *__restrict__ part, int npad, int rpair, T *__restrict__ grp,
+                                          cplx<T> *__restrict__ part, int npad, int rpair, int r0, T *__restrict__ grp,
                                           const cplx<T> *__restrict__ srcl, int s0, int s1, int G, int g)
 {
     constexpr bool STAGED = tile_staged<T, L>();
-    constexpr bool PIPE = tile_pipelined<T, L, NR>();
+    constexpr bool PIPE = tile_pipelined<T, L, NRL>();
     constexpr int VW = 16 / sizeof(T);
     constexpr int NB = L * (L + 2);
     constexpr int SL = slot_len(L, L, VW);
@@ -379,7 +425,7 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
 #pragma unroll
     for (int q = 0; q < NB; q++) W[q] = cmk((T)0, (T)0);
 
-    GroupPrefetch<T, L, NR> pre;
+    GroupPrefetch<T, L, NRL> pre;
     if constexpr (PIPE) pre.load(tab, tile, s0);
 
 #pragma unroll 1
@@ -394,7 +440,7 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
         }
         const cplx<T> *xs = xin + ((int64_t)(rhs * npad + src) * 2 + mode) * NB;
         // staged copy: block 0 = bt.y (forward sources), 1 = bt.x (reverse sources)
-        const cplx<T> *xl = srcl + (DIR * S + (src % S)) * (2 * NR * NB + SRC_PAD) + (rhs * 2 + mode) * NB;
+        const cplx<T> *xl = srcl + (DIR * S + (src % S)) * (2 * NRL * NB + SRC_PAD) + ((rhs - r0) * 2 + mode) * NB;
         // all waves of the tile read the same table group in step s: keep them
         // in step so that the group is fetched from HBM once (and, staged,
         // copied to LDS once)
@@ -413,7 +459,7 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
             if (s + 1 < s1) pre.load(tab, tile, s + 1);
             __syncthreads();
         } else if constexpr (STAGED) {
-            stage_group<T, L, NR>(tab, tile, s, grp);
+            stage_group<T, L, NRL>(tab, tile, s, grp);
             __syncthreads();
         }
         if (diag && s == 0) continue;
@@ -527,61 +573,82 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
 }
 
 // NR = 2 (one BiCG product pair): 2 waves, forward and reverse.  NR = 4
-// (both polarizations' pairs at once): 4 waves; waves 0/1 serve right-hand
-// sides 0-1 and waves 2/3 sides 2-3 from the same table stream, so every
-// table byte read from HBM feeds four columns.
+// (both polarizations' pairs at once): a workgroup serves NRL of the NR
+// columns, two waves per column pair.  NRL = 4: waves 0/1 serve columns 0-1
+// and waves 2/3 columns 2-3 from the same staged table group, so every table
+// byte read from HBM feeds four columns.  NRL = 2 (FP64): two workgroups per
+// tile, one per column pair, each staging its own group and sources; the
+// two run on the same XCD (workgroups are dealt to MI355X's 8 XCDs round
+// robin), so the second reads the group from that XCD's L2.
 // G workgroups share a tile (small clusters, G = 2..16): workgroup g takes
 // the steps [g S/G, (g+1) S/G) and writes its own partials.
-template <typename T, int L, int NR>
-__global__ __launch_bounds__(64 * NR) __attribute__((amdgpu_waves_per_eu(tile_waves<T, L, NR>()))) void direct_tile_kernel(
-    const T *__restrict__ tab, const int2 *__restrict__ tiles, const cplx<T> *__restrict__ xin,
+template <typename T, int L, int NR, int NRL>
+__global__ __launch_bounds__(64 * NRL) __attribute__((amdgpu_waves_per_eu(tile_waves<T, L, NRL>()))) void direct_tile_kernel(
+    const T *__restrict__ tab, const int2 *__restrict__ tiles, int nunits, const cplx<T> *__restrict__ xin,
     cplx<T> *__restrict__ part, int npad, int G)
 {
-    const int tile = blockIdx.x / G, g = blockIdx.x - tile * G;
+    constexpr int H = NR / NRL;  // workgroups per (tile, step group)
+    int u = blockIdx.x, h = 0;
+    if constexpr (H > 1) {
+        u = (int)(blockIdx.x / (8 * H)) * 8 + (int)(blockIdx.x % 8);
+        h = (int)(blockIdx.x / 8) % H;
+        if (u >= nunits) return;  // the grid is padded to whole groups of 8 H
+    }
+    const int tile = u / G, g = u - tile * G;
     const int s0 = g * (MSTM_SLOTS / G), s1 = s0 + MSTM_SLOTS / G;
+    const int r0 = h * NRL;  // first column of this workgroup
     const int2 bt = tiles[tile];
     const int w = threadIdx.x >> 6;
     __shared__ __attribute__((aligned(16))) T grp[tile_lds_elems<T, L>()];
-    __shared__ __attribute__((aligned(16))) T srcs[tile_src_elems<T, L, NR>()];
+    __shared__ __attribute__((aligned(16))) T srcs[tile_src_elems<T, L, NRL>()];
     cplx<T> *srcl = reinterpret_cast<cplx<T> *>(srcs);
-    if constexpr (tile_src_staged<T, L, NR>()) {
-        // the two source blocks of the tile, all threads; visible after the
-        // first step's barrier
+    if constexpr (tile_src_staged<T, L, NRL>()) {
+        // the two source blocks of the tile (this workgroup's columns), all
+        // threads; visible after the first step's barrier
         constexpr int NB = L * (L + 2), BLK = MSTM_SLOTS * 2 * NB;  // one column r: 16 spheres x 2 modes
-        for (int e = threadIdx.x; e < 2 * NR * BLK; e += 64 * NR) {
-            const int b = e / (NR * BLK), rem = e - b * NR * BLK, r = rem / BLK, o = rem - r * BLK;
+        for (int e = threadIdx.x; e < 2 * NRL * BLK; e += 64 * NRL) {
+            const int b = e / (NRL * BLK), rem = e - b * NRL * BLK, r = rem / BLK, o = rem - r * BLK;
             const int blk = b == 0 ? bt.y : bt.x;
             const int i = o / (2 * NB), mo = o - i * 2 * NB;  // sphere in block, (mode, coefficient)
-            srcl[(b * MSTM_SLOTS + i) * (2 * NR * NB + SRC_PAD) + r * 2 * NB + mo] =
-                xin[((int64_t)(r * npad + blk * MSTM_SLOTS + i) * 2) * NB + mo];
+            srcl[(b * MSTM_SLOTS + i) * (2 * NRL * NB + SRC_PAD) + r * 2 * NB + mo] =
+                xin[((int64_t)((r0 + r) * npad + blk * MSTM_SLOTS + i) * 2) * NB + mo];
         }
     }
+    const int rpair = (r0 >> 1) + (w >> 1);
     if constexpr (tile_staged<T, L>()) {
-        if (!(w & 1)) tile_body_sym<T, L, 0, NR>(bt, tile, tab, part, npad, w >> 1, grp, srcl, xin, s0, s1, G, g);
+        if (!(w & 1))
+            tile_body_sym<T, L, 0, NR, NRL>(bt, tile, tab, part, npad, rpair, r0, grp, srcl, xin, s0, s1, G, g);
         else if (bt.x != bt.y)
-            tile_body_sym<T, L, 1, NR>(bt, tile, tab, part, npad, w >> 1, grp, srcl, xin, s0, s1, G, g);
+            tile_body_sym<T, L, 1, NR, NRL>(bt, tile, tab, part, npad, rpair, r0, grp, srcl, xin, s0, s1, G, g);
     } else {
-        if (!(w & 1)) tile_body<T, L, 0, NR>(tab, bt, tile, xin, part, npad, w >> 1, grp, srcl, s0, s1, G, g);
-        else if (bt.x != bt.y) tile_body<T, L, 1, NR>(tab, bt, tile, xin, part, npad, w >> 1, grp, srcl, s0, s1, G, g);
+        if (!(w & 1))
+            tile_body<T, L, 0, NR, NRL>(tab, bt, tile, xin, part, npad, rpair, r0, grp, srcl, s0, s1, G, g);
+        else if (bt.x != bt.y)
+            tile_body<T, L, 1, NR, NRL>(tab, bt, tile, xin, part, npad, rpair, r0, grp, srcl, s0, s1, G, g);
     }
     if (bt.x == bt.y && (w & 1))
     {
         // idle reverse wave of a diagonal tile: its share of the staging and the same barriers
-        GroupPrefetch<T, L, NR> pre;
-        if constexpr (tile_pipelined<T, L, NR>()) pre.load(tab, tile, s0);
+        GroupPrefetch<T, L, NRL> pre;
+        if constexpr (tile_pipelined<T, L, NRL>()) pre.load(tab, tile, s0);
         for (int s = s0; s < s1; s++) {
             __syncthreads();
-            if constexpr (tile_pipelined<T, L, NR>()) {
+            if constexpr (tile_pipelined<T, L, NRL>()) {
                 pre.store(grp);
                 if (s + 1 < s1) pre.load(tab, tile, s + 1);
                 __syncthreads();
             } else if constexpr (tile_staged<T, L>()) {
-                stage_group<T, L, NR>(tab, tile, s, grp);
+                stage_group<T, L, NRL>(tab, tile, s, grp);
                 __syncthreads();
             }
         }
     }
 }
+
+// columns per workgroup of a four-column product: FP64 tiles with staged
+// sources split the four columns over two workgroups (the staged sources of
+// four FP64 columns would leave one workgroup per CU)
+template <typename T, int L> constexpr int tile4_nrl() { return tile_src_staged<T, L, 2>() && sizeof(T) == 8 ? 2 : 4; }
 
 template <typename T>
 hipError_t launch_direct_tiles(const T *tab, const int2 *tiles, int ntiles, int L, const T *xin, T *partial, int npad,
@@ -591,15 +658,20 @@ hipError_t launch_direct_tiles(const T *tab, const int2 *tiles, int ntiles, int 
     cplx<T> *p = reinterpret_cast<cplx<T> *>(partial);
     if (nr != 2 && nr != 4) return hipErrorInvalidValue;
     if (G < 1 || G > MSTM_SLOTS || MSTM_SLOTS % G) return hipErrorInvalidValue;
-    dim3 g((unsigned)((int64_t)ntiles * G));
+    const int nunits = ntiles * G;
     switch (L * 8 + nr) {
 #define CASE(LL)                                                                                                       \
     case LL * 8 + 2:                                                                                                   \
-        hipLaunchKernelGGL((direct_tile_kernel<T, LL, 2>), g, dim3(128), 0, st, tab, tiles, x, p, npad, G);            \
+        hipLaunchKernelGGL((direct_tile_kernel<T, LL, 2, 2>), dim3((unsigned)nunits), dim3(128), 0, st, tab, tiles,    \
+                           nunits, x, p, npad, G);                                                                     \
         break;                                                                                                         \
-    case LL * 8 + 4:                                                                                                   \
-        hipLaunchKernelGGL((direct_tile_kernel<T, LL, 4>), g, dim3(256), 0, st, tab, tiles, x, p, npad, G);            \
-        break;
+    case LL * 8 + 4: {                                                                                                 \
+        constexpr int NRL = tile4_nrl<T, LL>(), H = 4 / NRL;                                                           \
+        const unsigned nb = H == 1 ? (unsigned)nunits : (unsigned)((nunits + 7) / 8 * 8 * H);                          \
+        hipLaunchKernelGGL((direct_tile_kernel<T, LL, 4, NRL>), dim3(nb), dim3(64 * NRL), 0, st, tab, tiles, nunits,   \
+                           x, p, npad, G);                                                                             \
+        break;                                                                                                         \
+    }
 #ifdef MSTM_TILE_L_ONLY
         CASE(MSTM_TILE_L_ONLY)
 #else

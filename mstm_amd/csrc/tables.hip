@@ -66,7 +66,7 @@ __device__ void build_slot(T *__restrict__ grp, int k, int LT, int LS, int itype
         };
         // symmetric layout (LT = LS): half-sum / half-difference rows of block m >= 0
         auto put_sym = [&](int m, const double *row) {
-            int e0 = ssec_off(m, LT, VW) + srot_deg(m, n);
+            int e0 = srow_off(m, n);
             for (int kk = 0; kk <= n; kk++) {
                 double sg = (kk & 1) ? -1.0 : 1.0;
                 double e = (kk == 0) ? row[n] : 0.5 * (row[kk + n] + sg * row[-kk + n]);
@@ -149,8 +149,7 @@ __device__ void build_slot(T *__restrict__ grp, int k, int LT, int LS, int itype
                 }
             }
         }
-        int e0 = sym ? ssec_off(m, LT, VW) + pad_to(srot_len(m, LT), VW)
-                     : sec_off(b, LT, LS, VW) + pad_to(rot_len(m, LT, LS), VW);
+        int e0 = sym ? sax_off(m, LT, VW) : sec_off(b, LT, LS, VW) + pad_to(rot_len(m, LT, LS), VW);
         int ap = sym ? pad_to(sax_len(m, LT), VW) : pad_to(ax_len(m, LT, LS), VW);
         for (int p = 0; p < (sym ? sax_modes(m) : 2); p++)
             for (int l = m1; l <= LS; l++)
