@@ -19,6 +19,11 @@
 #include "fft.hpp"
 #include "learned.hpp"
 
+// largest number of workgroups sharing one direct tile's 16 steps
+#ifndef MSTM_TGROUPS_MAX
+#define MSTM_TGROUPS_MAX MSTM_SLOTS
+#endif
+
 namespace {
 
 struct DirectOp {
@@ -434,13 +439,16 @@ static int finish_sphere_layout(mstm_hip_ctx *c)
         for (int bj = bi; bj < c->nblocks; bj++) tiles.push_back(make_int2(bi, bj));
     c->ntiles = (int)tiles.size();
     // few tiles (small clusters): split each tile's 16 steps over up to 16
-    // workgroups so that the launch fills the GPU; each keeps its own
-    // partial sums (partial buffer at most 256 MiB at FP64, four columns)
+    // workgroups, doubling while the launch stays within 512 workgroups (two
+    // per CU of 256): more groups leave each with a few steps that cannot
+    // pay for staging its sources, and more partial sums to add; each group
+    // keeps its own partial sums (partial buffer at most 256 MiB at FP64,
+    // four columns)
     c->tgroups = 1;
     {
         const size_t nbp = (size_t)c->Lmax * (c->Lmax + 2);
         auto part_bytes = [&](int G) { return sizeof(double) * 2 * (size_t)c->nblocks * G * 4 * 2 * c->npad * nbp; };
-        while (c->tgroups < MSTM_SLOTS && (int64_t)c->ntiles * c->tgroups < 2048 &&
+        while (c->tgroups < MSTM_TGROUPS_MAX && (int64_t)c->ntiles * c->tgroups * 2 <= 512 &&
                part_bytes(2 * c->tgroups) <= ((size_t)256 << 20))
             c->tgroups *= 2;
     }
@@ -831,34 +839,39 @@ static int bicg2_dev(mstm_hip_ctx *c, int ns, int niter, const double *eps, cons
     if (rc) return rc;
     act[0] = hS[0].active != 0;
     act[1] = ns == 2 && hS[1].active != 0;
-    int iter;
+    // host reads of the scalars every K iterations: a stopped system's
+    // kernels return at once on the device, so the iterates and counts are
+    // the same for any K; products of small clusters cost less than the
+    // read, and run up to K - 1 surplus products after the last system stops
+    const int K = (!use_fft && c->nsph <= 500) ? 4 : 1;
+    int iter, synced = 0;
     for (iter = 1; iter <= niter; iter++) {
         if (!act[0] && !act[1]) break;
         int f4[4] = {2, 3, 2, 3};
         if (act[0] && act[1]) rc = matvec(c, 4, P[0], AP[0], f4, prec, use_fft);
         else rc = matvec(c, 2, act[0] ? P[0] : P[1], act[0] ? AP[0] : AP[1], f4, prec, use_fft);
         if (rc) return rc;
-        // the vector kernels of a system the device has stopped return at once
-        for (int s = 0; s < ns; s++)
-            if (act[s]) HIPCHK(launch_bicg_a_dev(AP[s], AW[s], P[s], W[s], n, pregion(c, s), S + s, c->st));
+        // both systems' vector steps in one launch each; the kernels of a
+        // system the device has stopped return at once
+        HIPCHK(launch_bicg_a_dev(AP[0], AW[0], P[0], W[0], 2 * (int64_t)V, n, pregion(c, 0), region, S, ns, c->st));
         HIPCHK(launch_bicg_scal(2, c->d_partials, region, S, ns, iter, c->st));
-        for (int s = 0; s < ns; s++)
-            if (act[s])
-                HIPCHK(launch_bicg_b_dev(X[s], Rr[s], Q[s], P[s], AP[s], AW[s], S + s, n, pregion(c, s), c->st));
+        HIPCHK(launch_bicg_b_dev(X[0], Rr[0], Q[0], P[0], AP[0], AW[0], (int64_t)V, 2 * (int64_t)V, S, n, pregion(c, 0),
+                                 region, ns, c->st));
         HIPCHK(launch_bicg_scal(3, c->d_partials, region, S, ns, iter, c->st));
-        for (int s = 0; s < ns; s++)
-            if (act[s]) HIPCHK(launch_bicg_c_dev(P[s], W[s], Rr[s], Q[s], S + s, n, c->st));
+        HIPCHK(launch_bicg_c_dev(P[0], W[0], Rr[0], Q[0], (int64_t)V, 2 * (int64_t)V, S, n, ns, c->st));
+        if (iter % K && iter < niter) continue;
         rc = sync_state();
         if (rc) return rc;
         for (int s = 0; s < ns; s++) {
             if (!act[s]) continue;
-            // a step that passed the breakdown test counts, as the host loop did
-            if (!(hS[s].it == iter && hS[s].active == 0 && hS[s].cak.re == 0.0 && hS[s].cak.im == 0.0)) {
-                if (prec == MSTM_HIP_FP64) c->stats.iters_fp64++;
-                else c->stats.iters_fp32++;
-            }
+            // steps since the last read; a step that failed the breakdown test
+            // does not count, as in the host loop
+            const int done = hS[s].active ? iter - synced : hS[s].it - synced - hS[s].brk;
+            if (prec == MSTM_HIP_FP64) c->stats.iters_fp64 += done;
+            else c->stats.iters_fp32 += done;
             act[s] = hS[s].active != 0;
         }
+        synced = iter;
     }
     for (int s = 0; s < ns; s++) {
         it[s] = act[s] ? iter : hS[s].it;  // ran out: niter + 1

@@ -785,36 +785,58 @@ template hipError_t launch_prep<double>(const double *, int, double *, int, cons
 // normal column  out = T h  (mie_mult) or h;  con_tran column out = conj(S h).
 // Written back in the reference (unpadded) layout.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ void reduce_post_kernel(const cplx<T> *__restrict__ part, int nparts, cplx<double> *__restrict__ aout,
-                                   int ldaout, int nrhs, int4 flags, int nr, int nsph, int npad, int Lp,
-                                   const int *__restrict__ order, const int *__restrict__ offset,
-                                   const cplx<double> *__restrict__ an)
+// KS lanes share an output (r, sphere, e): lane j sums the partials
+// b = j, j + KS, ... and the KS sums are then added in a fixed tree order
+// within the wave, so the result is deterministic; many partials per output
+// (small clusters, whose tiles are split in step groups) then keep many
+// loads in flight instead of one long dependent loop per thread.
+template <typename T, int KS>
+__global__ __launch_bounds__(256) void reduce_post_kernel(const cplx<T> *__restrict__ part, int nparts,
+                                                          cplx<double> *__restrict__ aout, int ldaout, int nrhs,
+                                                          int4 flags, int nr, int nsph, int npad, int Lp,
+                                                          const int *__restrict__ order, const int *__restrict__ offset,
+                                                          const cplx<double> *__restrict__ an)
 {
     const int NBP = Lp * (Lp + 2);
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (int64_t)nrhs * nsph * NBP) return;
-    int e = (int)(q % NBP);
-    int64_t t = q / NBP;
-    int sph = (int)(t % nsph);
-    int r = (int)(t / nsph);
-    int m, n;
-    decode_mn(e, Lp, m, n);
-    const int L = order[sph];
-    if (n > L) return;
-    const int NB = L * (L + 2);
-    const int f = r == 0 ? flags.x : (r == 1 ? flags.y : (r == 2 ? flags.z : flags.w));
-    int es = (f & 1) ? amn(-m, n, Lp) : e;
-    cplx<double> h0 = cmk(0.0, 0.0), h1 = cmk(0.0, 0.0);
-    const int64_t stride = (int64_t)nr * npad * 2 * NBP;
-    const cplx<T> *p = part + ((int64_t)r * npad + sph) * 2 * NBP;
-    for (int b = 0; b < nparts; b++) {
-        cplx<T> a0 = p[(int64_t)b * stride + es], a1 = p[(int64_t)b * stride + es + NBP];
-        h0.re += (double)a0.re;
-        h0.im += (double)a0.im;
-        h1.re += (double)a1.re;
-        h1.im += (double)a1.im;
+    const int j = threadIdx.x % KS;
+    const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / KS;
+    const int64_t nout = (int64_t)nrhs * nsph * NBP;
+    // out-of-range and padding items still take part in the shuffles
+    const bool live = q < nout;
+    int e = 0, sph = 0, r = 0, m = 0, n = 1, L = 0, f = 0, es = 0;
+    if (live) {
+        e = (int)(q % NBP);
+        const int64_t t = q / NBP;
+        sph = (int)(t % nsph);
+        r = (int)(t / nsph);
+        decode_mn(e, Lp, m, n);
+        L = order[sph];
+        f = r == 0 ? flags.x : (r == 1 ? flags.y : (r == 2 ? flags.z : flags.w));
+        es = (f & 1) ? amn(-m, n, Lp) : e;
     }
+    const bool use = live && n <= L;
+    cplx<double> h0 = cmk(0.0, 0.0), h1 = cmk(0.0, 0.0);
+    if (use) {
+        const int64_t stride = (int64_t)nr * npad * 2 * NBP;
+        const cplx<T> *p = part + ((int64_t)r * npad + sph) * 2 * NBP + es;
+#pragma unroll 4
+        for (int b = j; b < nparts; b += KS) {
+            const cplx<T> a0 = p[(int64_t)b * stride], a1 = p[(int64_t)b * stride + NBP];
+            h0.re += (double)a0.re;
+            h0.im += (double)a0.im;
+            h1.re += (double)a1.re;
+            h1.im += (double)a1.im;
+        }
+    }
+#pragma unroll
+    for (int w = 1; w < KS; w <<= 1) {
+        h0.re += __shfl_xor(h0.re, w, KS);
+        h0.im += __shfl_xor(h0.im, w, KS);
+        h1.re += __shfl_xor(h1.re, w, KS);
+        h1.im += __shfl_xor(h1.im, w, KS);
+    }
+    if (!use || j) return;
+    const int NB = L * (L + 2);
     cplx<double> *o = aout + (int64_t)r * ldaout + offset[sph];
     int eo = amn(m, n, L);
     if (f & 1) {
@@ -839,12 +861,18 @@ hipError_t launch_reduce_post(const T *partial, int nparts, double *aout, int ld
     if (nrhs < 1 || nrhs > 4) return hipErrorInvalidValue;
     const int nr = nrhs > 2 ? 4 : 2;
     int NBP = Lp * (Lp + 2);
-    int64_t n = (int64_t)nrhs * nsph * NBP;
-    int bs = 256;
-    hipLaunchKernelGGL(reduce_post_kernel<T>, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, st,
-                       reinterpret_cast<const cplx<T> *>(partial), nparts, reinterpret_cast<cplx<double> *>(aout),
-                       ldaout / 2, nrhs, flags4(flags, nrhs), nr, nsph, npad, Lp, order, offset,
-                       reinterpret_cast<const cplx<double> *>(an));
+    const int64_t n = (int64_t)nrhs * nsph * NBP;
+    const int bs = 256;
+#define RP(KS)                                                                                                         \
+    hipLaunchKernelGGL((reduce_post_kernel<T, KS>), dim3((unsigned)((n * KS + bs - 1) / bs)), dim3(bs), 0, st,         \
+                       reinterpret_cast<const cplx<T> *>(partial), nparts, reinterpret_cast<cplx<double> *>(aout),     \
+                       ldaout / 2, nrhs, flags4(flags, nrhs), nr, nsph, npad, Lp, order, offset,                       \
+                       reinterpret_cast<const cplx<double> *>(an))
+    // lanes per output: enough loads in flight without idle lanes
+    if (nparts >= 128 && n < (int64_t)1 << 20) RP(8);
+    else if (nparts >= 32) RP(4);
+    else RP(1);
+#undef RP
     return hipGetLastError();
 }
 template hipError_t launch_reduce_post<float>(const float *, int, double *, int, int, const int *, int, int, int,

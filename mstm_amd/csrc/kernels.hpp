@@ -45,24 +45,28 @@ hipError_t launch_qeff(const double *amnp, const double *pmnp, int ld, int nsph,
 int krylov_grid();
 // Scalars of one BiCG system kept on the device (cbicg's csk, cak, cbk,
 // enorm and the convergence state), so that an iteration needs no host
-// reduction: the host reads the array once per iteration to steer.
+// reduction: the host reads the array every few iterations to steer.
 struct BicgScal {
     cplx<double> csk, cak, cbk;
     double enorm, err, eps;
-    int active, it;
+    int active, it, brk;
 };
 // step 0: enorm = |b|^2 (active = enorm != 0); 1: csk = r.r (inactive if 0);
-// 2: den = <w, A p>, cak = csk / den (breakdown: inactive, it = iter);
+// 2: den = <w, A p>, cak = csk / den (breakdown: inactive, it = iter, brk = 1);
 // 3: <q, r>, <r, r>: err, stop test, cbk, csk.  partials: region s of the
 // system s at partials + s * region (complex), one or two values per block.
 hipError_t launch_bicg_scal(int step, const double *partials, int64_t region, BicgScal *st, int ns, int iter,
                             hipStream_t st_);
-hipError_t launch_bicg_a_dev(double *cap, double *caw, const double *cp, const double *cw, int n, double *partials,
-                             const BicgScal *sc, hipStream_t st);
+// the vector steps of ns systems in one launch: system s's vectors at the
+// given pointers plus s * sv (x, r, q) or s * sp (p, w, Ap, A^H w) doubles,
+// its partial sums at partials + s * region complex values
+hipError_t launch_bicg_a_dev(double *cap, double *caw, const double *cp, const double *cw, int64_t sp, int n,
+                             double *partials, int64_t region, const BicgScal *sc, int ns, hipStream_t st);
 hipError_t launch_bicg_b_dev(double *anp, double *cr, double *cq, const double *cp, const double *cap,
-                             const double *caw, const BicgScal *sc, int n, double *partials, hipStream_t st);
-hipError_t launch_bicg_c_dev(double *cp, double *cw, const double *cr, const double *cq, const BicgScal *sc, int n,
-                             hipStream_t st);
+                             const double *caw, int64_t sv, int64_t sp, const BicgScal *sc, int n, double *partials,
+                             int64_t region, int ns, hipStream_t st);
+hipError_t launch_bicg_c_dev(double *cp, double *cw, const double *cr, const double *cq, int64_t sv, int64_t sp,
+                             const BicgScal *sc, int n, int ns, hipStream_t st);
 hipError_t launch_zdot(const double *a, const double *b, int n, int conj_a, double *partials, double *result,
                        hipStream_t st);
 hipError_t launch_bicg_a(double *cap, double *caw, const double *cp, const double *cw, int n, double *partials,

@@ -171,6 +171,7 @@ __global__ __launch_bounds__(KB) void bicg_scal_kernel(int step, const cplx<doub
                 S.enorm = v0.re;
                 S.active = v0.re != 0.0;
                 S.it = 0;
+                S.brk = 0;
                 S.err = 0.0;
                 S.cak = S.cbk = zero;
             } else if (skip) {
@@ -182,6 +183,7 @@ __global__ __launch_bounds__(KB) void bicg_scal_kernel(int step, const cplx<doub
                 if (v0.re == 0.0 && v0.im == 0.0) {  // breakdown: the Fortran loop exits here
                     S.active = 0;
                     S.it = iter;
+                    S.brk = 1;
                     S.cak = zero;
                 } else {
                     S.cak = cdiv(S.csk, v0);
@@ -202,11 +204,22 @@ __global__ __launch_bounds__(KB) void bicg_scal_kernel(int step, const cplx<doub
     }
 }
 
+// The vector steps of ns systems in one launch: system s = blockIdx.y has
+// its vectors at the base pointers plus s times a stride (complex elements:
+// sv for x, r, q; sp for p, w, Ap, A^H w), its scalars at sc[s] and its
+// partial sums at partials + s * region; a stopped system returns at once.
 __global__ void bicg_a_dev_kernel(cplx<double> *__restrict__ cap, cplx<double> *__restrict__ caw,
-                                  const cplx<double> *__restrict__ cp, const cplx<double> *__restrict__ cw, int n,
-                                  cplx<double> *__restrict__ partials, const BicgScal *__restrict__ sc)
+                                  const cplx<double> *__restrict__ cp, const cplx<double> *__restrict__ cw, int64_t sp,
+                                  int n, cplx<double> *__restrict__ partials, int64_t region,
+                                  const BicgScal *__restrict__ sc)
 {
-    if (!sc->active) return;
+    const int64_t y = blockIdx.y;
+    if (!sc[y].active) return;
+    cap += y * sp;
+    caw += y * sp;
+    cp += y * sp;
+    cw += y * sp;
+    partials += y * region;
     cplx<double> s = cmk(0.0, 0.0);
     for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) {
         cplx<double> a = csub(cp[i], cap[i]);
@@ -221,10 +234,19 @@ __global__ void bicg_a_dev_kernel(cplx<double> *__restrict__ cap, cplx<double> *
 __global__ void bicg_b_dev_kernel(cplx<double> *__restrict__ anp, cplx<double> *__restrict__ cr,
                                   cplx<double> *__restrict__ cq, const cplx<double> *__restrict__ cp,
                                   const cplx<double> *__restrict__ cap, const cplx<double> *__restrict__ caw,
-                                  const BicgScal *__restrict__ sc, int n, cplx<double> *__restrict__ partials)
+                                  int64_t sv, int64_t sp, const BicgScal *__restrict__ sc, int n,
+                                  cplx<double> *__restrict__ partials, int64_t region)
 {
-    if (!sc->active) return;
-    const cplx<double> cak = sc->cak, cakc = cconj(cak);
+    const int64_t y = blockIdx.y;
+    if (!sc[y].active) return;
+    anp += y * sv;
+    cr += y * sv;
+    cq += y * sv;
+    cp += y * sp;
+    cap += y * sp;
+    caw += y * sp;
+    partials += y * region;
+    const cplx<double> cak = sc[y].cak, cakc = cconj(cak);
     cplx<double> s[2] = {cmk(0.0, 0.0), cmk(0.0, 0.0)};
     for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) {
         cplx<double> a = anp[i];
@@ -247,11 +269,16 @@ __global__ void bicg_b_dev_kernel(cplx<double> *__restrict__ anp, cplx<double> *
 }
 
 __global__ void bicg_c_dev_kernel(cplx<double> *__restrict__ cp, cplx<double> *__restrict__ cw,
-                                  const cplx<double> *__restrict__ cr, const cplx<double> *__restrict__ cq,
-                                  const BicgScal *__restrict__ sc, int n)
+                                  const cplx<double> *__restrict__ cr, const cplx<double> *__restrict__ cq, int64_t sv,
+                                  int64_t sp, const BicgScal *__restrict__ sc, int n)
 {
-    if (!sc->active) return;
-    const cplx<double> cbk = sc->cbk, cbkc = cconj(cbk);
+    const int64_t y = blockIdx.y;
+    if (!sc[y].active) return;
+    cp += y * sp;
+    cw += y * sp;
+    cr += y * sv;
+    cq += y * sv;
+    const cplx<double> cbk = sc[y].cbk, cbkc = cconj(cbk);
     for (int i = blockIdx.x * KB + threadIdx.x; i < n; i += gridDim.x * KB) {
         cplx<double> p = cr[i];
         cfma(p, cbk, cp[i]);
@@ -270,28 +297,29 @@ hipError_t launch_bicg_scal(int step, const double *partials, int64_t region, Bi
                        reinterpret_cast<const cplx<double> *>(partials), region, sc, ns, iter);
     return hipGetLastError();
 }
-hipError_t launch_bicg_a_dev(double *cap, double *caw, const double *cp, const double *cw, int n, double *partials,
-                             const BicgScal *sc, hipStream_t st)
+hipError_t launch_bicg_a_dev(double *cap, double *caw, const double *cp, const double *cw, int64_t sp, int n,
+                             double *partials, int64_t region, const BicgScal *sc, int ns, hipStream_t st)
 {
     typedef cplx<double> Z;
-    hipLaunchKernelGGL(bicg_a_dev_kernel, dim3(KGRID), dim3(KB), 0, st, (Z *)cap, (Z *)caw, (const Z *)cp,
-                       (const Z *)cw, n, (Z *)partials, sc);
+    hipLaunchKernelGGL(bicg_a_dev_kernel, dim3(KGRID, ns), dim3(KB), 0, st, (Z *)cap, (Z *)caw, (const Z *)cp,
+                       (const Z *)cw, sp / 2, n, (Z *)partials, region, sc);
     return hipGetLastError();
 }
 hipError_t launch_bicg_b_dev(double *anp, double *cr, double *cq, const double *cp, const double *cap,
-                             const double *caw, const BicgScal *sc, int n, double *partials, hipStream_t st)
+                             const double *caw, int64_t sv, int64_t sp, const BicgScal *sc, int n, double *partials,
+                             int64_t region, int ns, hipStream_t st)
 {
     typedef cplx<double> Z;
-    hipLaunchKernelGGL(bicg_b_dev_kernel, dim3(KGRID), dim3(KB), 0, st, (Z *)anp, (Z *)cr, (Z *)cq, (const Z *)cp,
-                       (const Z *)cap, (const Z *)caw, sc, n, (Z *)partials);
+    hipLaunchKernelGGL(bicg_b_dev_kernel, dim3(KGRID, ns), dim3(KB), 0, st, (Z *)anp, (Z *)cr, (Z *)cq, (const Z *)cp,
+                       (const Z *)cap, (const Z *)caw, sv / 2, sp / 2, sc, n, (Z *)partials, region);
     return hipGetLastError();
 }
-hipError_t launch_bicg_c_dev(double *cp, double *cw, const double *cr, const double *cq, const BicgScal *sc, int n,
-                             hipStream_t st)
+hipError_t launch_bicg_c_dev(double *cp, double *cw, const double *cr, const double *cq, int64_t sv, int64_t sp,
+                             const BicgScal *sc, int n, int ns, hipStream_t st)
 {
     typedef cplx<double> Z;
-    hipLaunchKernelGGL(bicg_c_dev_kernel, dim3(KGRID), dim3(KB), 0, st, (Z *)cp, (Z *)cw, (const Z *)cr,
-                       (const Z *)cq, sc, n);
+    hipLaunchKernelGGL(bicg_c_dev_kernel, dim3(KGRID, ns), dim3(KB), 0, st, (Z *)cp, (Z *)cw, (const Z *)cr,
+                       (const Z *)cq, sv / 2, sp / 2, sc, n);
     return hipGetLastError();
 }
 hipError_t launch_bicg_a(double *cap, double *caw, const double *cp, const double *cw, int n, double *partials,
