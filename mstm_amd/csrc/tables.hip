@@ -26,33 +26,33 @@ __device__ void build_slot(T *__restrict__ grp, int k, int LT, int LS, int itype
         sph = rt[1] / rho;
     }
     // ---- Wigner d rows, degree by degree --------------------------------
-    double sbe = (fabs(ct) >= 1.0) ? 0.0 : sqrt(fabs((1.0 + ct) * (1.0 - ct)));
-    double cbe2 = 0.5 * (1.0 + ct), sbe2 = 0.5 * (1.0 - ct);
-    double dk0[2 * MSTM_TAB_L + 3], dk01[2 * MSTM_TAB_L + 3];
+    double sinb = (fabs(ct) >= 1.0) ? 0.0 : sqrt(fabs((1.0 + ct) * (1.0 - ct)));
+    double cos2h = 0.5 * (1.0 + ct), sin2h = 0.5 * (1.0 - ct);
+    double row_cur[2 * MSTM_TAB_L + 3], row_prev[2 * MSTM_TAB_L + 3];
     double rowp[2 * MSTM_TAB_L + 1], rowc[2 * MSTM_TAB_L + 1];
-    const int o0 = MSTM_TAB_L + 1;
-    for (int q = 0; q < 2 * MSTM_TAB_L + 3; q++) dk0[q] = dk01[q] = 0.0;
-    dk0[o0] = 1.0;
-    int sgn = 1;
-    double sben = 1.0, bc = 1.0;  // bc = bcof(n,n) = sqrt((2n)!/(n!)^2)
+    const int mid = MSTM_TAB_L + 1;
+    for (int q = 0; q < 2 * MSTM_TAB_L + 3; q++) row_cur[q] = row_prev[q] = 0.0;
+    row_cur[mid] = 1.0;
+    int sgn_n = 1;
+    double sinpow = 1.0, binom = 1.0;  // binom = bcof(n,n) = sqrt((2n)!/(n!)^2)
     for (int n = 1; n <= nmax; n++) {
-        sgn = -sgn;
-        sben = sben * sbe / 2.0;
-        if (sben < 1e-30) sben = 0.0;
-        bc = bc * sqrt((double)(2 * n) * (double)(2 * n - 1)) / (double)n;
-        dk0[o0 + n] = sgn * sben * bc;
-        dk0[o0 - n] = sgn * dk0[o0 + n];
-        dk01[o0 + n] = 0.0;
-        dk01[o0 - n] = 0.0;
+        sgn_n = -sgn_n;
+        sinpow = sinpow * sinb / 2.0;
+        if (sinpow < 1e-30) sinpow = 0.0;
+        binom = binom * sqrt((double)(2 * n) * (double)(2 * n - 1)) / (double)n;
+        row_cur[mid + n] = sgn_n * sinpow * binom;
+        row_cur[mid - n] = sgn_n * row_cur[mid + n];
+        row_prev[mid + n] = 0.0;
+        row_prev[mid - n] = 0.0;
         for (int kk = -n + 1; kk <= n - 1; kk++) {
-            double dkt = dk01[o0 + kk];
-            dk01[o0 + kk] = dk0[o0 + kk];
-            dk0[o0 + kk] = (ct * (double)(2 * n - 1) * dk01[o0 + kk] -
-                            sqrt((double)(n - kk - 1)) * sqrt((double)(n + kk - 1)) * dkt) /
-                           (sqrt((double)(n + kk)) * sqrt((double)(n - kk)));
+            double older = row_prev[mid + kk];
+            row_prev[mid + kk] = row_cur[mid + kk];
+            row_cur[mid + kk] = (ct * (double)(2 * n - 1) * row_prev[mid + kk] -
+                                 sqrt((double)(n - kk - 1)) * sqrt((double)(n + kk - 1)) * older) /
+                                (sqrt((double)(n + kk)) * sqrt((double)(n - kk)));
         }
-        int kmx = imn(n, nmin);
-        for (int kk = -n; kk <= n; kk++) rowp[kk + n] = dk0[o0 + kk];
+        int mtop = imn(n, nmin);
+        for (int kk = -n; kk <= n; kk++) rowp[kk + n] = row_cur[mid + kk];
         // write row m = 0
         auto put_row = [&](int m, const double *row, double sgnm) {
             int b = (m == 0) ? 0 : (m < 0 ? 2 * (-m) - 1 : 2 * m);
@@ -76,26 +76,26 @@ __device__ void build_slot(T *__restrict__ grp, int k, int LT, int LS, int itype
         };
         if (sym) put_sym(0, rowp);
         else put_row(0, rowp, 0.0);
-        int im = 1;
-        for (int m = 1; m <= kmx; m++) {
-            im = -im;
-            double fmn = 1.0 / sqrt((double)(n - m + 1)) / sqrt((double)(n + m));
-            double dkm0 = 0.0, dkm1;
+        int sgn_m = 1;
+        for (int m = 1; m <= mtop; m++) {
+            sgn_m = -sgn_m;
+            double rnorm = 1.0 / sqrt((double)(n - m + 1)) / sqrt((double)(n + m));
+            double d_mid = 0.0, d_lo;
             for (int kk = -n; kk <= n; kk++) {
-                dkm1 = dkm0;
-                dkm0 = rowp[kk + n];
-                double dkn1 = (kk == n) ? 0.0 : rowp[kk + 1 + n];
-                rowc[kk + n] = (sqrt((double)(n + kk)) * sqrt((double)(n - kk + 1)) * cbe2 * dkm1 -
-                                sqrt((double)(n - kk)) * sqrt((double)(n + kk + 1)) * sbe2 * dkn1 -
-                                (double)kk * sbe * rowp[kk + n]) *
-                               fmn;
+                d_lo = d_mid;
+                d_mid = rowp[kk + n];
+                double d_hi = (kk == n) ? 0.0 : rowp[kk + 1 + n];
+                rowc[kk + n] = (sqrt((double)(n + kk)) * sqrt((double)(n - kk + 1)) * cos2h * d_lo -
+                                sqrt((double)(n - kk)) * sqrt((double)(n + kk + 1)) * sin2h * d_hi -
+                                (double)kk * sinb * rowp[kk + n]) *
+                               rnorm;
             }
             if (sym) {
                 put_sym(m, rowc);
             } else {
                 put_row(m, rowc, 0.0);
                 // d(-m, -k) = d(m, k) (-1)^(k+m)
-                put_row(-m, rowc, (double)im);
+                put_row(-m, rowc, (double)sgn_m);
             }
             for (int kk = -n; kk <= n; kk++) rowp[kk + n] = rowc[kk + n];
         }
