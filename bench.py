@@ -252,8 +252,9 @@ def bench_fft(args):
     rel = lambda a, b: float(np.abs(a - b).max() / np.abs(b).max())  # noqa: E731
     _line("s/solve (both polarizations) to residual 1e-6, FFT operator", round(t_mix, 4), "s/solve", args.steps,
           args.warmup, round(t_mix * 1e3, 2), "c64 FFT/translation + f64 refinement", name,
-          {"nsphere": cl.n, "order": int(order.max()), "node_grid": {k: (v.tolist() if hasattr(v, "tolist") else v)
-                                                                     for k, v in grid.items()},
+          {"nsphere": cl.n, "order": int(order.max()),
+           "node_grid": {k: (grid[k].tolist() if hasattr(grid[k], "tolist") else grid[k])
+                         for k in ("cell_dim", "d_cell", "node_order", "fva")},
            "fft_setup_s": round(t_setup, 3), "iterations_mixed": it_mix,
            "plain_fft_fp64": {"s_per_solve": round(t_plain, 4), "iterations": it_plain},
            "direct_fp64": {"s_per_solve": round(t_dir, 4), "iterations": it_dir},
