@@ -399,10 +399,12 @@ def small_cluster_solve(reps: int = 20):
 def observe(ctx):
     """Single-origin observables of the solution the last fixedorsoln left on the device."""
     ob = ctx.observables(None)
+    ctx.fosmexpansion()
     st = ctx.stats()
     return dict(T=ob["T"], g=ob["g"], s11=ob["scatmat"][:, 0].copy(), s12=ob["scatmat"][:, 4].copy(),
                 ms=dict(orders=round(st["obs_orders_ms"], 3), merge=round(st["obs_merge_ms"], 3),
-                        scatmat=round(st["obs_scatmat_ms"], 3), g=round(st["obs_moments_ms"], 3)))
+                        scatmat=round(st["obs_scatmat_ms"], 3), g=round(st["obs_moments_ms"], 3),
+                        fosmexpansion=round(st["obs_fosm_ms"], 3)))
 
 
 def main():
@@ -654,7 +656,7 @@ def main():
         "q_error_vs_tight_fp64": qerr,
         "observables": dict(t_matrix_order=ob_lig["T"], g=ob_lig["g"], ms=ob_lig["ms"],
                             note="post-solve, on the GPU: translation orders (host), merge to the common origin, "
-                                 "scattering matrix at 361 directions, s00 moments"),
+                                 "scattering matrix at 361 directions, s00 moments (g), the full fosmexpansion"),
         "matvec": matvec,
         "small_cluster": small,
         "setup_s": dict(geometry=round(t_geom, 2), tables_fp64_fp32=round(t_tables, 3),

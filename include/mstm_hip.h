@@ -217,6 +217,15 @@ int mstm_hip_scattering_matrix(mstm_hip_ctx *ctx, int T, const double *amnp0, in
  * matrix.  s00 (2 doubles) or g may be NULL. */
 int mstm_hip_asymmetry(mstm_hip_ctx *ctx, int T, const double *amnp0, double *s00, double *g);
 
+/* fosmexpansion (mstm-scatprops-26.f90:1820-1951): the generalized-spherical
+ * expansion coefficients s00, s02, sp22, sm22 of the scattering matrix, each
+ * 4 x 4 x (2T+1) doubles in the Fortran order of s(4,4,0:2T) (element
+ * (i,j,w) at (w*4 + j-1)*4 + i-1), with the reference's sign patch and
+ * 1/(2 m_medium) scale.  amnp0 NULL: the last merge, whose order must be T;
+ * else the caller's common-origin coefficients of order T. */
+int mstm_hip_fosmexpansion(mstm_hip_ctx *ctx, int T, const double *amnp0, double *s00, double *s02, double *sp22,
+                           double *sm22);
+
 /* direct_solver (mstm-solver-8.f90:571-642): anp = A^{-1} pnp with the dense
  * interaction matrix A = I - T H (general_interaction_matrix with mie_mult,
  * mstm-modules-33.f90:5854-5985, built from the FP64 operator) and its LU

@@ -144,7 +144,9 @@ struct mstm_hip_ctx {
         size_t ang_cap = 0;
         double *d_cg = nullptr, *d_mom = nullptr;
         int cg_T = 0;
-        double orders_ms = 0, merge_ms = 0, scatmat_ms = 0, moments_ms = 0;
+        double *d_bc = nullptr, *d_fpart = nullptr;  // fosmexpansion: bcof table, block sums
+        int bc_T = 0;
+        double orders_ms = 0, merge_ms = 0, scatmat_ms = 0, moments_ms = 0, fosm_ms = 0;
         // T-matrix solution: distribute table and the unit-excitation columns
         double *d_vccd = nullptr;
         int vccd_T = 0, vccd_LS = 0;
@@ -357,6 +359,9 @@ static void release_observables(mstm_hip_ctx *c)
     dfree(c->obs.d_sm);
     dfree(c->obs.d_cg);
     dfree(c->obs.d_mom);
+    dfree(c->obs.d_bc);
+    dfree(c->obs.d_fpart);
+    c->obs.bc_T = 0;
     dfree(c->obs.d_vccd);
     dfree(c->obs.d_dist);
     dfree(c->obs.d_dq);
@@ -2013,7 +2018,7 @@ const char *mstm_hip_stats_names(void)
            "lig_errb,fgmres_steps,tile4_ms_fp64,tile4_ms_fp32,tile4_launches_fp64,tile4_launches_fp32,inner_solver,"
            "obs_orders_ms,obs_merge_ms,obs_scatmat_ms,obs_moments_ms,obs_T,fft_contract_ms_fp32,"
            "fft_contract_launches_fp32,fft_contract_ms_fp64,fft_contract_launches_fp64,tmatrix_ms,tmatrix_solves,"
-           "lu_build_ms,lu_factor_ms,lu_solve_ms";
+           "lu_build_ms,lu_factor_ms,lu_solve_ms,obs_fosm_ms";
 }
 
 int mstm_hip_stats(const mstm_hip_ctx *c, double *out, int nout)
@@ -2063,7 +2068,8 @@ int mstm_hip_stats(const mstm_hip_ctx *c, double *out, int nout)
                   (double)c->obs.tm_solves,
                   c->lu.build_ms,
                   c->lu.factor_ms,
-                  c->lu.solve_ms};
+                  c->lu.solve_ms,
+                  c->obs.fosm_ms};
     int k = std::min(nout, (int)(sizeof v / sizeof v[0]));
     for (int q = 0; q < k; q++) out[q] = v[q];
     return k;
@@ -2369,6 +2375,71 @@ int mstm_hip_asymmetry(mstm_hip_ctx *c, int T, const double *amnp0, double *s00,
     }
     if (g) *g = s[1] / (3.0 * s[0]);
     c->obs.moments_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+int mstm_hip_fosmexpansion(mstm_hip_ctx *c, int T, const double *amnp0, double *s00, double *s02, double *sp22,
+                           double *sm22)
+{
+    if (!c) return fail(c, MSTM_HIP_EARG, "fosmexpansion: null context");
+    if (!s00 || !s02 || !sp22 || !sm22) return fail(c, MSTM_HIP_EARG, "fosmexpansion: null output");
+    HIPCHK(hipSetDevice(c->device));
+    auto t0 = std::chrono::steady_clock::now();
+    const double *src = nullptr;
+    int Tu = 0;
+    if (int rc = observe_source(c, T, amnp0, &src, &Tu)) return rc;
+    if (Tu != T)  // the outputs are sized by T
+        return fail(c, MSTM_HIP_EARG, "fosmexpansion: T = %d, the merged coefficients have order %d", T, Tu);
+    const int W = 2 * Tu + 1, ldb = 4 * Tu + 2, nb = fosm_blocks(Tu);
+    if (c->obs.bc_T != Tu) {
+        dfree(c->obs.d_bc);
+        dfree(c->obs.d_fpart);
+        c->obs.bc_T = 0;
+        // bcof(n,l) = sqrt((n+l)!/(n! l!)) by init's recurrence (mstm-modules-33.f90:46-50)
+        std::vector<double> bc((size_t)ldb * ldb, 0.0), fnr(2 * (size_t)ldb + 1);
+        for (int i = 0; i <= 2 * ldb; i++) fnr[i] = std::sqrt((double)i);
+        bc[0] = 1.0;
+        for (int n = 0; n < ldb - 1; n++) {
+            for (int l = n + 1; l < ldb; l++) {
+                bc[(size_t)n * ldb + l] = fnr[n + l] * bc[(size_t)n * ldb + l - 1] / fnr[l];
+                bc[(size_t)l * ldb + n] = bc[(size_t)n * ldb + l];
+            }
+            bc[(size_t)(n + 1) * ldb + n + 1] = fnr[n + n + 2] * fnr[n + n + 1] * bc[(size_t)n * ldb + n] / fnr[n + 1] / fnr[n + 1];
+        }
+        HIPCHK(dalloc(&c->obs.d_bc, sizeof(double) * (bc.size() + 8 * (size_t)W)));
+        HIPCHK(hipMemcpy(c->obs.d_bc, bc.data(), sizeof(double) * bc.size(), hipMemcpyHostToDevice));
+        HIPCHK(dalloc(&c->obs.d_fpart, sizeof(double) * (size_t)nb * 8 * W));
+        c->obs.bc_T = Tu;
+    }
+    double *dout = c->obs.d_bc + (size_t)ldb * ldb;
+    HIPCHK(launch_fosm(src, Tu, c->obs.d_bc, ldb, c->obs.d_fpart, dout, c->st));
+    std::vector<double> e8((size_t)8 * W);
+    HIPCHK(hipMemcpyAsync(e8.data(), dout, sizeof(double) * e8.size(), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    // the reference's sign patch and scale (mstm-scatprops-26.f90:1930-1948):
+    // (j,i) with i = 3, 4 and j <= i negated; sm22(3,3) = -sm22(2,2),
+    // sp22(3,3) = sp22(2,2); everything times 1/(2 m_medium)
+    const size_t nall = (size_t)16 * W;
+    std::fill(s00, s00 + nall, 0.0);
+    std::fill(s02, s02 + nall, 0.0);
+    std::fill(sp22, sp22 + nall, 0.0);
+    std::fill(sm22, sm22 + nall, 0.0);
+    const double f = 0.5 / c->rimed;
+    auto at = [&](double *s, int i, int j, int w) -> double & { return s[((size_t)w * 4 + (j - 1)) * 4 + (i - 1)]; };
+    for (int w = 0; w < W; w++) {
+        const double *v = e8.data() + w;
+        at(s00, 1, 1, w) = f * v[0];
+        at(s00, 4, 4, w) = -f * v[(size_t)W];
+        at(s02, 2, 1, w) = f * v[2 * (size_t)W];
+        at(s02, 1, 2, w) = f * v[3 * (size_t)W];
+        at(s02, 3, 4, w) = -f * v[4 * (size_t)W];
+        at(s02, 4, 3, w) = f * v[5 * (size_t)W];
+        at(sm22, 2, 2, w) = f * v[6 * (size_t)W];
+        at(sp22, 2, 2, w) = f * v[7 * (size_t)W];
+        at(sm22, 3, 3, w) = -f * v[6 * (size_t)W];
+        at(sp22, 3, 3, w) = f * v[7 * (size_t)W];
+    }
+    c->obs.fosm_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return 0;
 }
 

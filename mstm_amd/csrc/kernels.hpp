@@ -108,3 +108,7 @@ hipError_t launch_lu_solve(const double *LUm, int n, const int *piv, double *b, 
 hipError_t launch_lu_resid(const double *A, int n, const double *x, const double *p, double *r, hipStream_t st);
 hipError_t launch_s00_moments(const double *a0, int T, const double *cgm, const double *cg1, double *out,
                               hipStream_t st);
+// fosmexpansion's eight sequences over w = 0..2T (out[e*(2T+1) + w]); part:
+// fosm_blocks(T) * 8 * (2T+1) doubles; bc: bcof table, leading dimension ldb >= 4T+2
+int fosm_blocks(int T);
+hipError_t launch_fosm(const double *a0, int T, const double *bc, int ldb, double *part, double *out, hipStream_t st);

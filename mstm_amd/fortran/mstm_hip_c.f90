@@ -134,6 +134,14 @@
          complex(c_double_complex) :: amnp0(*)
          real(c_double) :: s00(2),g
          end function mstm_hip_asymmetry
+         integer(c_int) function mstm_hip_fosmexpansion(ctx,t,amnp0,s00,s02,sp22,sm22) &
+            bind(c,name='mstm_hip_fosmexpansion')
+         import :: c_ptr, c_int, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: t
+         complex(c_double_complex) :: amnp0(*)
+         real(c_double) :: s00(*),s02(*),sp22(*),sm22(*)
+         end function mstm_hip_fosmexpansion
          integer(c_int) function mstm_hip_direct_solver(ctx,pnp,anp,initialize,niter,solution_error) &
             bind(c,name='mstm_hip_direct_solver')
          import :: c_ptr, c_int, c_double, c_double_complex

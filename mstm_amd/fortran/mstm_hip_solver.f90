@@ -13,6 +13,7 @@
 !   hip_scatteringmatrix    <- the scatteringmatrix loop of
 !                              scattering_matrix_calculation (mstm-input-37.f90:3036-3062)
 !   hip_asymmetry           <- s00(1,1,0:1) of fosmexpansion (mstm-scatprops-26.f90:1820)
+!   hip_fosmexpansion       <- fosmexpansion       (mstm-scatprops-26.f90:1820-1951)
 !   hip_tmatrix_solution    <- tmatrix_solution    (mstm-solver-8.f90:22-25)
 !   hip_direct_solver       <- direct_solver       (mstm-solver-8.f90:571-572)
 !   hip_node_selection      <- node_selection      (fft_translation-5.f90:654)
@@ -415,6 +416,19 @@
          if(.not.hip_loaded) call hip_load_state()
          call hip_check(mstm_hip_asymmetry(hip_ctx,int(nodrt,c_int),amn0,s00,g),'asymmetry')
          end subroutine hip_asymmetry
+!
+! fosmexpansion (mstm-scatprops-26.f90:1820-1951) with its argument list:
+! s00, s02, sp22, sm22 of the common-origin expansion amnp, sign patch and
+! scale included; mpi_comm is accepted and not used (one GPU)
+!
+         subroutine hip_fosmexpansion(ntot,amnp,s00,s02,sp22,sm22,mpi_comm)
+         integer :: ntot
+         integer, optional :: mpi_comm
+         real(8) :: s00(4,4,0:ntot*2),s02(4,4,0:ntot*2),sp22(4,4,0:ntot*2),sm22(4,4,0:ntot*2)
+         complex(8) :: amnp(0:ntot+1,ntot,2,2)
+         if(.not.hip_loaded) call hip_load_state()
+         call hip_check(mstm_hip_fosmexpansion(hip_ctx,int(ntot,c_int),amnp,s00,s02,sp22,sm22),'fosmexpansion')
+         end subroutine hip_fosmexpansion
 
 !
 ! tmatrix_solution (mstm-solver-8.f90:22-332), iterative branch, on the GPU

@@ -53,6 +53,7 @@ _SIGNATURES = {
     "mstm_hip_merge_to_common_origin": (C.c_int, [_vp, _dp, _dp]),
     "mstm_hip_scattering_matrix": (C.c_int, [_vp, C.c_int, _dp, C.c_int, _dp, _dp, _dp]),
     "mstm_hip_asymmetry": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
+    "mstm_hip_fosmexpansion": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp, _dp]),
     "mstm_hip_set_inner_solver": (C.c_int, [_vp, C.c_int]),
     "mstm_hip_set_refinement_cycles": (C.c_int, [_vp, C.c_int]),
     "mstm_hip_direct_solver": (C.c_int, [_vp, _dp, _dp, C.c_int, C.c_int, _dp]),

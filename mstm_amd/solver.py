@@ -253,6 +253,18 @@ class HipContext:
                                             _d(g)), "asymmetry")
         return s, float(g[0])
 
+    def fosmexpansion(self, a0=None, T: int = 0):
+        """fosmexpansion (mstm-scatprops-26.f90:1820-1951): s00, s02, sp22,
+        sm22 as (2T+1, 4, 4) arrays [w, j-1, i-1] = s(i, j, w), with the
+        reference's sign patch and 1/(2 m_medium) scale."""
+        Tu = int(T) if a0 is not None else int(self.stats()["obs_T"])
+        W = 2 * Tu + 1
+        outs = [np.zeros((W, 4, 4)) for _ in range(4)]
+        a = None if a0 is None else np.ascontiguousarray(a0, np.complex128)
+        self._chk(self.L.mstm_hip_fosmexpansion(self.h, Tu, None if a is None else _d(a.view(np.float64)),
+                                                *[_d(o) for o in outs]), "fosmexpansion")
+        return tuple(outs)
+
     def observables(self, amnp=None, alpha: float = 0.0, eps: float = 1e-5, maxt: int = 100, copy: bool = False):
         """The single-origin post-processing of a solved cluster on the GPU:
         translation orders, merge, scattering matrix on the default grid

@@ -8,7 +8,7 @@
 !   host_integration <posfile> <n>       direct operator: fixedorsoln (full
 !        and partial excitation, order of scattering), sphereinteraction
 !        (rhs_list, skip_external_translation), cbicg, direct_solver before
-!        and after a geometry change, observables, tmatrix_solution
+!        and after a geometry change, observables (fosmexpansion included), tmatrix_solution
 !   host_integration <posfile> <n> fft   node-grid operator: node_selection
 !        (automatic and specified d_cell), sphereinteraction,
 !        fft_external_to_external_expansion, cbicg, clear_fft_matrix
@@ -114,8 +114,9 @@
       contains
          subroutine observables_check()
          integer :: j,nt
-         real(8) :: r0(3),rt,ct(361),ph(361),gg,s2(0:1),e1,e2,e3
-         real(8), allocatable :: sm_r(:,:),sm_g(:,:),s00(:,:,:),s02(:,:,:),sp22(:,:,:),sm22(:,:,:)
+         real(8) :: r0(3),rt,ct(361),ph(361),gg,s2(0:1),e1,e2,e3,e4
+         real(8), allocatable :: sm_r(:,:),sm_g(:,:),s00(:,:,:),s02(:,:,:),sp22(:,:,:),sm22(:,:,:), &
+            g00(:,:,:),g02(:,:,:),gp22(:,:,:),gm22(:,:,:)
          complex(8) :: rimed(2),sa(4)
          complex(8), allocatable :: a0_r(:,:),a0_g(:,:)
          r0=0.d0
@@ -153,8 +154,12 @@
          call fosmexpansion(nt,a0_r,s00,s02,sp22,sm22)
          call hip_asymmetry(a0_r,nt,s2,gg)
          e3=max(abs(s2(0)-s00(1,1,0))/abs(s00(1,1,0)),abs(s2(1)-s00(1,1,1))/abs(s00(1,1,0)))
-         write(*,'(a,i4,3es12.4,f10.6)') 'observables T rel_err(merge,scatmat,s00) g: ',nt,e1,e2,e3,gg
-         if(e1.gt.1.d-12.or.e2.gt.1.d-12.or.e3.gt.1.d-12) bad=1
+         allocate(g00(4,4,0:2*nt),g02(4,4,0:2*nt),gp22(4,4,0:2*nt),gm22(4,4,0:2*nt))
+         call hip_fosmexpansion(nt,a0_r,g00,g02,gp22,gm22)
+         e4=max(maxval(abs(g00-s00))/maxval(abs(s00)),maxval(abs(g02-s02))/maxval(abs(s02)), &
+            maxval(abs(gp22-sp22))/maxval(abs(sp22)),maxval(abs(gm22-sm22))/maxval(abs(sm22)))
+         write(*,'(a,i4,4es12.4,f10.6)') 'observables T rel_err(merge,scatmat,s00,fosm) g: ',nt,e1,e2,e3,e4,gg
+         if(e1.gt.1.d-12.or.e2.gt.1.d-12.or.e3.gt.1.d-12.or.e4.gt.1.d-12) bad=1
          end subroutine observables_check
 !
 ! tmatrix_solution against hip_tmatrix_solution on the same state, to

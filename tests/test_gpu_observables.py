@@ -3,9 +3,10 @@ common origin, scattering matrix, asymmetry parameter) through the C-ABI.
 
 Against the reference's own outputs (tests/golden/, obs_* keys) on the same
 solutions: orders exactly, merged coefficients, the 361 x 16 scattering
-matrix and s00(1,1,0:1) to 1e-12 relative (FP64 throughout; only the
-summation order differs).  At the 5000-sphere cluster (common-origin order
-about 60) against the C oracle on the GPU's own solution.
+matrix, s00(1,1,0:1) and the full fosmexpansion (s00, s02, sp22, sm22 for
+every w) to 1e-12 relative (FP64 throughout; only the summation order
+differs).  At the 5000-sphere cluster (common-origin order about 60) against
+the C oracle on the GPU's own solution.
 """
 import time
 
@@ -48,6 +49,10 @@ def test_observables_match_reference(gpu_ctx, case):
     assert rel(sm2, g["obs_scatmat"]) < TOL
     s00b, _ = gpu_ctx.asymmetry(a0=g["obs_amnp0"], T=T)
     assert rel(s00b, ref) < TOL
+    # fosmexpansion: the device-resident merge and the caller's coefficients
+    for fo in (gpu_ctx.fosmexpansion(), gpu_ctx.fosmexpansion(a0=g["obs_amnp0"], T=T)):
+        for key, out in zip(("s00", "s02", "sp22", "sm22"), fo):
+            assert rel(out, g["obs_" + key]) < TOL, key
 
 
 def test_merge_of_device_solution_equals_host_solution(gpu_ctx):
@@ -106,6 +111,24 @@ def test_observables_5000_sphere_cluster_match_oracle(gpu_ctx):
     ring = gpu_ctx.scattering_matrix(CT.ravel(), PH.ravel())[:, 0].reshape(CT.shape).mean(axis=1)
     g_quad = float(np.sum(w * x * ring) / np.sum(w * ring))
     assert abs(ob["g"] - g_quad) < 1e-10
+    # the full generalized-spherical expansion against the oracle's (T about 60)
+    for key, got, want in zip(("s00", "s02", "sp22", "sm22"), gpu_ctx.fosmexpansion(), O.fosm(T, a0)):
+        assert rel(got, want) < 1e-10, key
+
+
+def test_fosmexpansion_order_mismatch_is_an_error(gpu_ctx):
+    from mstm_amd.solver import MstmHipError
+
+    g = golden("cfg1_direct")
+    gpu_ctx.set_spheres(g["pos"], g["radius"], g["ri"], 1.0, 1e-6)
+    T, _ = gpu_ctx.translation_orders(g["obs_origin"], 1e-5, 100)
+    gpu_ctx.merge_to_common_origin(g["amnp"])
+    W = 2 * T + 3
+    outs = [np.zeros((W, 4, 4)) for _ in range(4)]
+    from mstm_amd.solver import _d
+
+    rc = gpu_ctx.L.mstm_hip_fosmexpansion(gpu_ctx.h, T + 1, None, *[_d(o) for o in outs])
+    assert rc != 0 and not any(np.any(o) for o in outs)
 
 
 def test_sphere_at_the_common_origin(gpu_ctx):
