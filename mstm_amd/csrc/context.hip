@@ -669,6 +669,11 @@ static int direct_matvec_buf(mstm_hip_ctx *c, int nrhs, const double *ain, doubl
     DirectOp &d = c->dop[prec];
     if (nrhs < 1 || nrhs > 4) return fail(c, MSTM_HIP_EARG, "direct product: %d columns", nrhs);
     const int L = c->Lmax, nr = nrhs > 2 ? 4 : 2;
+    // FP64 four-column products of order <= 4 run two workgroups per tile
+    // (step group): fewer step groups keep that launch within the same 512
+    // workgroups
+    if (prec == MSTM_HIP_FP64 && nr == 4 && L <= 4)
+        while (G > 1 && (int64_t)c->ntiles * G * 2 > 512) G /= 2;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {
         e0 = next_event(c);

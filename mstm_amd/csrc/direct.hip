@@ -162,14 +162,32 @@ template <typename T, int L> constexpr int tile_lds_elems()
 {
     return tile_staged<T, L>() ? direct_slot_len(L, sizeof(T)) * MSTM_SLOTS : 1;
 }
-// pipelined form: the tile's two source blocks (16 spheres x 2 columns x 2
-// modes x NB complex each) also sit in LDS for the whole tile; each sphere's
-// 4 NB values are followed by SRC_PAD unused ones, so that the 16 slots of a
-// wave, reading 16 different spheres, spread over the LDS banks
-#define SRC_PAD 2
+// staged sources: the tile's two source blocks (16 spheres x NR columns x 2
+// modes x NB complex each) sit in LDS for the whole tile.  A column's 2 NB
+// values are followed by src_cpad() unused complex values and a sphere's
+// columns by src_spad(), so that the lanes of a wave, which read 16 different
+// spheres and, per sphere, different columns and modes, spread over the LDS
+// banks.
+#ifndef MSTM_SRC_CPAD32
+#define MSTM_SRC_CPAD32 0
+#endif
+#ifndef MSTM_SRC_SPAD32
+#define MSTM_SRC_SPAD32 2
+#endif
+#ifndef MSTM_SRC_CPAD64
+#define MSTM_SRC_CPAD64 0
+#endif
+#ifndef MSTM_SRC_SPAD64
+#define MSTM_SRC_SPAD64 2
+#endif
+template <typename T> constexpr int src_cpad() { return sizeof(T) == 4 ? MSTM_SRC_CPAD32 : MSTM_SRC_CPAD64; }
+template <typename T> constexpr int src_spad() { return sizeof(T) == 4 ? MSTM_SRC_SPAD32 : MSTM_SRC_SPAD64; }
+// complex values per column and per sphere of the staged sources
+template <typename T, int L> constexpr int src_col() { return 2 * L * (L + 2) + src_cpad<T>(); }
+template <typename T, int L, int NR> constexpr int src_sph() { return NR * src_col<T, L>() + src_spad<T>(); }
 template <typename T, int L, int NR> constexpr int tile_src_elems()
 {
-    return tile_src_staged<T, L, NR>() ? 2 * MSTM_SLOTS * (2 * NR * L * (L + 2) + SRC_PAD) * 2 : 1;
+    return tile_src_staged<T, L, NR>() ? 2 * MSTM_SLOTS * src_sph<T, L, NR>() * 2 : 1;
 }
 // copy table group s of a tile into LDS, all threads of the workgroup
 template <typename T, int L, int NR>
@@ -239,7 +257,7 @@ __device__ __forceinline__ void tile_body_sym(int2 bt, int tile, const T *__rest
         }
         const cplx<T> *xs = xin + ((int64_t)(rhs * npad + src) * 2 + mode) * NB;
         // staged copy: block 0 = bt.y (forward sources), 1 = bt.x (reverse sources)
-        const cplx<T> *xl = srcl + (DIR * S + (src % S)) * (2 * NRL * NB + SRC_PAD) + ((rhs - r0) * 2 + mode) * NB;
+        const cplx<T> *xl = srcl + (DIR * S + (src % S)) * src_sph<T, L, NRL>() + (rhs - r0) * src_col<T, L>() + mode * NB;
         __syncthreads();
         cplx<T> xv[PIPE ? NB : 1];
         if constexpr (PIPE) {
@@ -440,7 +458,7 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
         }
         const cplx<T> *xs = xin + ((int64_t)(rhs * npad + src) * 2 + mode) * NB;
         // staged copy: block 0 = bt.y (forward sources), 1 = bt.x (reverse sources)
-        const cplx<T> *xl = srcl + (DIR * S + (src % S)) * (2 * NRL * NB + SRC_PAD) + ((rhs - r0) * 2 + mode) * NB;
+        const cplx<T> *xl = srcl + (DIR * S + (src % S)) * src_sph<T, L, NRL>() + (rhs - r0) * src_col<T, L>() + mode * NB;
         // all waves of the tile read the same table group in step s: keep them
         // in step so that the group is fetched from HBM once (and, staged,
         // copied to LDS once)
@@ -610,7 +628,7 @@ __global__ __launch_bounds__(64 * NRL) __attribute__((amdgpu_waves_per_eu(tile_w
             const int b = e / (NRL * BLK), rem = e - b * NRL * BLK, r = rem / BLK, o = rem - r * BLK;
             const int blk = b == 0 ? bt.y : bt.x;
             const int i = o / (2 * NB), mo = o - i * 2 * NB;  // sphere in block, (mode, coefficient)
-            srcl[(b * MSTM_SLOTS + i) * (2 * NRL * NB + SRC_PAD) + r * 2 * NB + mo] =
+            srcl[(b * MSTM_SLOTS + i) * src_sph<T, L, NRL>() + r * src_col<T, L>() + mo] =
                 xin[((int64_t)((r0 + r) * npad + blk * MSTM_SLOTS + i) * 2) * NB + mo];
         }
     }
