@@ -75,12 +75,30 @@ __global__ __launch_bounds__(LT_THREADS) void list_translate_kernel(const T *__r
         }
         return;
     }
+    // section offsets of the slot's table (block b = 0..NM-1 in the order
+    // 0,-1,1,...; NM: the phase), worked out once instead of by the
+    // layout functions' loops in every thread and term
+    __shared__ int boff[2 * LMAX_LIST + 2], baxo[2 * LMAX_LIST + 1], bapl[2 * LMAX_LIST + 1];
+    if (tid == 0) {
+        int o = 0;
+        for (int b = 0; b < NM; b++) {
+            const int m1 = imx(1, iab(mblock_m(b)));
+            const int rl = pad_to((nmax + 1) * (nmax + 1) - m1 * m1, VW);  // rot_len
+            const int al = pad_to(2 * (LT - m1 + 1) * (LS - m1 + 1), VW);  // ax_len
+            boff[b] = o;
+            baxo[b] = o + rl;
+            bapl[b] = al;
+            o += rl + 2 * al;
+        }
+        boff[NM] = o;
+    }
     extern __shared__ __attribute__((aligned(16))) unsigned char lt_smem[];
     cplx<T> *xp = reinterpret_cast<cplx<T> *>(lt_smem);  // [c][nk(n,k)], NBS each
     cplx<T> *ys = xp + 4 * NBS;                          // [c][m+nmin][n], n = 0..Ls
     cplx<T> *zs = ys + 4 * NM * (Ls + 1);                // [c][m+nmin][t], t = 0..Lt
     cplx<T> *ph = zs + 4 * NM * (Lt + 1);                // e^{i k phi}, k = -nmax..nmax
-    const cplx<T> e1 = cmk(E(phi_off(LT, LS, VW)), DIR ? -E(phi_off(LT, LS, VW) + 1) : E(phi_off(LT, LS, VW) + 1));
+    __syncthreads();
+    const cplx<T> e1 = cmk(E(boff[NM]), DIR ? -E(boff[NM] + 1) : E(boff[NM] + 1));
     for (int k = tid - nmax; k <= nmax; k += LT_THREADS) {
         cplx<T> r = cmk((T)1, (T)0), b = k < 0 ? cconj(e1) : e1;
         for (int u = 0; u < iab(k); u++) r = cmul(r, b);
@@ -110,7 +128,7 @@ __global__ __launch_bounds__(LT_THREADS) void list_translate_kernel(const T *__r
         const int c = e / (NM * Ls), rem = e - c * NM * Ls, mi = rem / Ls, n = rem - mi * Ls + 1;
         const int m = mi - nmin, m1 = imx(1, iab(m));
         if (n < m1) continue;
-        const int e0 = sec_off(bofm(m), LT, LS, VW) + n * n - m1 * m1;
+        const int e0 = boff[bofm(m)] + n * n - m1 * m1;
         cplx<T> acc = cmk((T)0, (T)0);
         for (int k = -n; k <= n; k++) crfma(acc, E(e0 + k + n), xp[c * NBS + nk(n, k)]);
         ys[(c * NM + mi) * (Ls + 1) + n] = acc;
@@ -121,7 +139,7 @@ __global__ __launch_bounds__(LT_THREADS) void list_translate_kernel(const T *__r
         const int c = e / (NM * Lt), rem = e - c * NM * Lt, mi = rem / Lt, t = rem - mi * Lt + 1;
         const int m = mi - nmin, m1 = imx(1, iab(m)), p = c & 1;
         if (t < m1) continue;
-        const int ea = sec_off(bofm(m), LT, LS, VW) + pad_to(rot_len(m, LT, LS), VW) + p * pad_to(ax_len(m, LT, LS), VW);
+        const int ea = baxo[bofm(m)] + p * bapl[bofm(m)];
         const int NR = LT - m1 + 1;
         cplx<T> acc = cmk((T)0, (T)0);
         for (int s = m1; s <= Ls; s++) {
@@ -143,7 +161,7 @@ __global__ __launch_bounds__(LT_THREADS) void list_translate_kernel(const T *__r
         const int mm = imn(t, nmin);
         for (int m = -mm; m <= mm; m++) {
             const int m1 = imx(1, iab(m));
-            const int e0 = sec_off(bofm(m), LT, LS, VW) + t * t - m1 * m1;
+            const int e0 = boff[bofm(m)] + t * t - m1 * m1;
             crfma(acc, E(e0 + kk + t), zs[(c * NM + m + nmin) * (Lt + 1) + t]);
         }
         cplx<T> v = cmul(ph[-kk + nmax], acc);
