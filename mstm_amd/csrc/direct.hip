@@ -103,18 +103,9 @@ template <typename T, int L> constexpr bool tile_staged() { return direct_sym(L,
 // also software-pipeline the staging: step s+1's group is loaded into
 // registers while step s computes (FP64 has no registers to spare for it).
 // NR here is the number of columns of one workgroup.
-template <typename T, int L, int NR> constexpr bool tile_src_staged()
-{
-#ifdef MSTM_TILE_SRC64
-    if (sizeof(T) == 8) return L <= 4 && MSTM_TILE_SRC64;
-#endif
-    return L <= 4;
-}
+template <typename T, int L, int NR> constexpr bool tile_src_staged() { return L <= 4; }
 template <typename T, int L, int NR> constexpr bool tile_pipelined()
 {
-#ifdef MSTM_TILE_PIPE64
-    if (sizeof(T) == 8) return MSTM_TILE_PIPE64 && tile_src_staged<T, L, NR>();
-#endif
     return sizeof(T) == 4 && tile_src_staged<T, L, NR>();
 }
 // occupancy target handed to the compiler: tiles with staged sources are
@@ -124,11 +115,7 @@ template <typename T, int L, int NR> constexpr bool tile_pipelined()
 // SIMD, larger orders the full register file
 template <typename T, int L, int NR> constexpr int tile_waves()
 {
-#ifdef MSTM_TILE_WAVES
-    return MSTM_TILE_WAVES;
-#else
     return tile_src_staged<T, L, NR>() ? 1 : (L <= 4 ? 2 : 1);
-#endif
 }
 // register prefetch of a table group for the pipelined form
 template <typename T, int L, int NR> struct GroupPrefetch {
@@ -163,28 +150,13 @@ template <typename T, int L> constexpr int tile_lds_elems()
     return tile_staged<T, L>() ? direct_slot_len(L, sizeof(T)) * MSTM_SLOTS : 1;
 }
 // staged sources: the tile's two source blocks (16 spheres x NR columns x 2
-// modes x NB complex each) sit in LDS for the whole tile.  A column's 2 NB
-// values are followed by src_cpad() unused complex values and a sphere's
-// columns by src_spad(), so that the lanes of a wave, which read 16 different
-// spheres and, per sphere, different columns and modes, spread over the LDS
-// banks.
-#ifndef MSTM_SRC_CPAD32
-#define MSTM_SRC_CPAD32 0
-#endif
-#ifndef MSTM_SRC_SPAD32
-#define MSTM_SRC_SPAD32 2
-#endif
-#ifndef MSTM_SRC_CPAD64
-#define MSTM_SRC_CPAD64 0
-#endif
-#ifndef MSTM_SRC_SPAD64
-#define MSTM_SRC_SPAD64 2
-#endif
-template <typename T> constexpr int src_cpad() { return sizeof(T) == 4 ? MSTM_SRC_CPAD32 : MSTM_SRC_CPAD64; }
-template <typename T> constexpr int src_spad() { return sizeof(T) == 4 ? MSTM_SRC_SPAD32 : MSTM_SRC_SPAD64; }
+// modes x NB complex each) sit in LDS for the whole tile.  A sphere's columns
+// are followed by two unused complex values, so that the 16 slots of a wave,
+// reading 16 different spheres, spread over the LDS banks (other pads of
+// columns and spheres measured no faster).
 // complex values per column and per sphere of the staged sources
-template <typename T, int L> constexpr int src_col() { return 2 * L * (L + 2) + src_cpad<T>(); }
-template <typename T, int L, int NR> constexpr int src_sph() { return NR * src_col<T, L>() + src_spad<T>(); }
+template <typename T, int L> constexpr int src_col() { return 2 * L * (L + 2); }
+template <typename T, int L, int NR> constexpr int src_sph() { return NR * src_col<T, L>() + 2; }
 template <typename T, int L, int NR> constexpr int tile_src_elems()
 {
     return tile_src_staged<T, L, NR>() ? 2 * MSTM_SLOTS * src_sph<T, L, NR>() * 2 : 1;
@@ -296,13 +268,6 @@ __device__ __forceinline__ void tile_body_sym(int2 bt, int tile, const T *__rest
             return v;
         };
         cplx<T> Y[NB];
-#ifdef MSTM_TILE_FINE
-#define TB_FENCE asm volatile("" ::: "memory")
-#elif defined(MSTM_TILE_FINE64)
-#define TB_FENCE if constexpr (sizeof(T) == 8) asm volatile("" ::: "memory")
-#else
-#define TB_FENCE
-#endif
         // ---- forward rotation, degree by degree
         static_for<L>([&](auto nc) {
             constexpr int n = decltype(nc)::value + 1;
@@ -331,7 +296,6 @@ __device__ __forceinline__ void tile_body_sym(int2 bt, int tile, const T *__rest
                     Y[nk(n, -m)] = (m & 1) ? csub(ym, yp) : csub(yp, ym);
                 }
             });
-            TB_FENCE;
         });
         // the back pass reloads its rows from LDS rather than keeping the
         // forward pass's copies live across the axial products
@@ -363,7 +327,6 @@ __device__ __forceinline__ void tile_body_sym(int2 bt, int tile, const T *__rest
 #pragma unroll
                 for (int t = 0; t < ND; t++) Y[nk(t + m1, mm)] = z[t];
             });
-            TB_FENCE;
         });
         asm volatile("" ::: "memory");
         // ---- back rotation and output phase, degree by degree
@@ -399,9 +362,7 @@ __device__ __forceinline__ void tile_body_sym(int2 bt, int tile, const T *__rest
                 cfma(W[nk(t, k)], cconj(ph[k]), up);
                 cfma(W[nk(t, -k)], ph[k], um);
             }
-            TB_FENCE;
         });
-#undef TB_FENCE
     }
     // partial for (target, partner block); m-flip back for the reverse wave
     const int partner = (DIR ? bt.x : bt.y) * G + g;  // one partial per (partner block, step group)

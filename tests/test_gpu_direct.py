@@ -243,3 +243,31 @@ def test_small_and_ragged_clusters_match_oracle(gpu_ctx, nsph, x):
     pw = oc.planewave()
     x = oc.direct_solve(np.stack([oc.mie_mult(pw[0]), oc.mie_mult(pw[1])]))
     assert rel(rl["amnp"], x) < 1e-10
+
+
+@pytest.mark.parametrize("nsph", [60, 100, 300])
+def test_step_groups_match_oracle(gpu_ctx, nsph):
+    """Clusters whose tiles are split in step groups: 60 spheres (10 tiles,
+    16 workgroups per tile), 100 (28 tiles, 8) and 300 (190 tiles, 2; the
+    split FP64 four-column launch 1): two- and four-column products in FP64
+    and FP32 against the oracle, each group keeping its own partial sums."""
+    from mstm_amd import geometry as G
+    from mstm_amd.solver import FP32, FP64
+    from oracle import oracle as O
+
+    c = G.rsa_in_sphere(nsph, 1.0, 0.2, seed=70 + nsph, ri=1.5 + 0.01j)
+    gpu_ctx.set_spheres(c.pos, c.radius, c.ri, 1.0, 1e-6)
+    oc = O.OracleCluster(c.pos, c.radius, c.ri)
+    ne = oc.neqns
+    rng = np.random.default_rng(nsph)
+    a = rng.normal(size=(2, ne)) + 1j * rng.normal(size=(2, ne))
+    ref = oc.sphereinteraction(a, [0, 1], [1, 1])
+    a4 = np.concatenate([a, a[::-1]])
+    for prec, tol in ((FP64, 1e-12), (FP32, 2e-5)):
+        out2 = gpu_ctx.sphereinteraction(a, [False, True], [True, True], precision=prec)
+        assert rel(out2, ref) < tol, (prec, 2)
+        out4 = gpu_ctx.sphereinteraction(a4, [False, True, True, False], [True] * 4, precision=prec)
+        for r, rr in enumerate([0, 1, 1, 0]):
+            assert rel(out4[r], ref[rr]) < tol, (prec, 4, r)
+    st = gpu_ctx.stats()
+    assert st["ntiles"] == {60: 10, 100: 28, 300: 190}[nsph]
