@@ -4,31 +4,37 @@
 // (mstm-modules-33.f90:6390-6525) and its per-pair coefficient_translation
 // (rot path, :6697-6749).  Work decomposition:
 //   * spheres are grouped in blocks of S = MSTM_SLOTS; a tile is an ordered
-//     block pair (bI <= bJ) and is one 128-lane workgroup;
-//   * wave 0 computes I <- J (forward, sop = .false.), wave 1 J <- I
-//     (reverse, sop = .true., which adds the m-flip of shiftcoefficient and
-//     uses the axial block transposed) from the same tables;
-//   * lane = slot k (16) x column (rhs, mode) (4).  In step s lane k of
-//     wave 0 targets I[k] and sources J[(k+s)%S]; lane k of wave 1 targets
-//     J[k] and sources I[(k-s)%S].  A lane's target never changes inside a
-//     tile, so it accumulates in registers and writes one partial sum per
-//     (target, partner block): no atomics, deterministic order;
+//     block pair (bI <= bJ); per column pair it has two waves;
+//   * the first wave computes I <- J (forward, sop = .false.), the second
+//     J <- I (reverse, sop = .true., which adds the m-flip of
+//     shiftcoefficient and uses the axial block transposed) from the same
+//     tables;
+//   * lane = slot k (16) x column (rhs, mode) (4).  In step s lane k of the
+//     forward wave targets I[k] and sources J[(k+s)%S]; lane k of the reverse
+//     wave targets J[k] and sources I[(k-s)%S].  A lane's target never
+//     changes inside a tile, so it accumulates in registers and writes one
+//     partial sum per (target, partner block): no atomics, deterministic
+//     order;
 //   * pair tables are laid out in HBM in exactly this consumption order and
 //     interleaved over the 16 slots, so each table load is 16 x 16 B;
 //   * up to order 5 (FP32) / 4 (FP64) the tables use the symmetric layout
 //     of common.hpp: only blocks m >= 0, half-sum/half-difference Wigner rows
 //     and one axial block per mode serve the blocks m and -m, which halves
-//     the table bytes and the rotation work;
+//     the table bytes and the rotation work; a step then runs in three
+//     passes over one vector (forward rotation by degree, axial products by
+//     block, back rotation and output phase by degree), so that only the
+//     accumulators and that vector live across the passes;
 //   * in that range all waves of the tile copy a step's table group (16
 //     slots) into LDS once per step and read it there: every table byte
 //     then leaves the L2 once per tile instead of once per lane that needs
 //     it (4 lanes x 2 or 4 waves);
-//   * the pipelined form (two-column FP32, order <= 4) also copies the
-//     tile's two source blocks into LDS once per tile, instead of every lane
-//     reloading its source sphere from L2 in every step;
-//   * the back rotation sums over the m blocks of a step and applies the
-//     output phase exp(-i sigma k phi) once per coefficient (staged forms);
-//     larger orders fold the phase into every term.
+//   * up to order 4 the tile's two source blocks are copied into LDS once
+//     per tile too, instead of every lane reloading its source sphere from
+//     L2 in every step; FP32 tiles also load the next step's table group
+//     into registers while the current step computes;
+//   * FP64 four-column products run as two workgroups per tile, one per
+//     column pair, on the same XCD; small clusters split a tile's 16 steps
+//     over G workgroups (step groups), each with its own partial sums.
 // The reduce kernel sums the partials and applies the Mie/con_tran post-op
 // of sphereinteraction (mstm-scatprops-26.f90:158-168).
 #include <utility>
@@ -847,8 +853,12 @@ hipError_t launch_reduce_post(const T *partial, int nparts, double *aout, int ld
                        reinterpret_cast<const cplx<T> *>(partial), nparts, reinterpret_cast<cplx<double> *>(aout),     \
                        ldaout / 2, nrhs, flags4(flags, nrhs), nr, nsph, npad, Lp, order, offset,                       \
                        reinterpret_cast<const cplx<double> *>(an))
-    // lanes per output: enough loads in flight without idle lanes
-    if (nparts >= 128 && n < (int64_t)1 << 20) RP(8);
+    // lanes per output: with many outputs one lane each keeps the loads of
+    // a wave on consecutive elements of one partial; with few (small
+    // clusters, many step-group partials) several lanes per output keep
+    // enough loads in flight
+    if (n >= 65536) RP(1);
+    else if (nparts >= 128) RP(8);
     else if (nparts >= 32) RP(4);
     else RP(1);
 #undef RP

@@ -271,3 +271,47 @@ def test_step_groups_match_oracle(gpu_ctx, nsph):
             assert rel(out4[r], ref[rr]) < tol, (prec, 4, r)
     st = gpu_ctx.stats()
     assert st["ntiles"] == {60: 10, 100: 28, 300: 190}[nsph]
+
+
+def test_full_size_product_matches_oracle_on_a_subset(gpu_ctx):
+    """The 5000-sphere workload (12.5 M pairs through the full tile grid):
+    with the input nonzero on 40 source spheres only, the product on those
+    40 and on 40 other spheres must equal the oracle's product on the
+    80-sphere cluster they form (sources as given, the others zero) to the
+    FP64 bar; the FP32 product to 2e-5; the product is linear, and the
+    con_tran column is the adjoint (<y, T H x> = <(T H)^H y, x>)."""
+    from mstm_amd import geometry as G
+    from mstm_amd.solver import FP32, FP64
+    from oracle import oracle as O
+
+    c = G.rsa_in_sphere(5000, 2.0, 0.30, seed=5, ri=1.31 + 0.0j)
+    gpu_ctx.set_spheres(c.pos, c.radius, c.ri, 1.0, 1e-6)
+    assert np.all(gpu_ctx.sphere_order == gpu_ctx.sphere_order[0])
+    nb = 2 * int(gpu_ctx.sphere_order[0]) * (int(gpu_ctx.sphere_order[0]) + 2)
+    ne = gpu_ctx.number_eqns
+    rng = np.random.default_rng(11)
+    pick = rng.choice(5000, 80, replace=False)
+    src, tgt = pick[:40], pick[40:]
+    x = np.zeros((2, ne), np.complex128)
+    for i in src:
+        x[:, i * nb:(i + 1) * nb] = rng.normal(size=(2, nb)) + 1j * rng.normal(size=(2, nb))
+    out64 = gpu_ctx.sphereinteraction(x, [False, False], [False, False], precision=FP64)
+    out32 = gpu_ctx.sphereinteraction(x, [False, False], [False, False], precision=FP32)
+    oc = O.OracleCluster(c.pos[pick], c.radius[pick], c.ri[pick])
+    xs = np.concatenate([x[:, i * nb:(i + 1) * nb] for i in pick], axis=1)
+    ref = oc.sphereinteraction(xs, [0, 0], [0, 0])
+    got64 = np.concatenate([out64[:, i * nb:(i + 1) * nb] for i in pick], axis=1)
+    got32 = np.concatenate([out32[:, i * nb:(i + 1) * nb] for i in pick], axis=1)
+    assert rel(got64, ref) < 1e-12
+    assert rel(got32, ref) < 2e-5
+    # linearity and the adjoint on full random vectors
+    a = rng.normal(size=(2, ne)) + 1j * rng.normal(size=(2, ne))
+    b = rng.normal(size=(2, ne)) + 1j * rng.normal(size=(2, ne))
+    ha = gpu_ctx.sphereinteraction(a, [False, False], [True, True], precision=FP64)
+    hb = gpu_ctx.sphereinteraction(b, [False, False], [True, True], precision=FP64)
+    hab = gpu_ctx.sphereinteraction(a + 2.0 * b, [False, False], [True, True], precision=FP64)
+    assert rel(hab, ha + 2.0 * hb) < 1e-12
+    hhb = gpu_ctx.sphereinteraction(b, [True, True], [True, True], precision=FP64)  # (T H)^H b
+    lhs = np.vdot(b[0], ha[0])
+    rhs = np.vdot(hhb[0], a[0])
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
