@@ -400,13 +400,14 @@ static hipError_t fft_axis(const cplx<T> *in, cplx<T> *out, int batch, int3 din,
 // Per-frequency contraction g(k,l) = sum_n K(p,l,n,k) a(k,n) for both RHS
 // (the pointwise products of fft_node_to_node_translation, :864-872).
 // The node-kernel spectrum is FP32 (the reference stores complex(4)); the
-// data precision is T.  One thread per (frequency, mode, block of 8 l).
+// data precision is T.  One thread per (frequency, mode, block of CONTRACT_LC l).
 // ===========================================================================
+#define CONTRACT_LC 4  // output degrees l per thread: each source value read serves LC of them
 template <typename T>
 __global__ void contract_kernel(const cplx<float> *__restrict__ K, const cplx<T> *__restrict__ a,
                                 cplx<T> *__restrict__ g, int M, int nblk)
 {
-    constexpr int LC = 8;
+    constexpr int LC = CONTRACT_LC;
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     int p = blockIdx.y;
     int l0 = blockIdx.z * LC;
@@ -1143,7 +1144,7 @@ template <typename T> static int matvec_t(FftOp *f, int nrhs, const double *ain,
     FCHK(fft_axis<T>(A, B, batch, d1, d2, 1, 2 * N.y, N.y, 2 * N.y, -1, st));
     FCHK(fft_axis<T>(B, A, batch, d2, d3, 0, 2 * N.x, N.x, 2 * N.x, -1, st));
     // per-frequency contraction with the node kernel spectrum
-    dim3 cg_((f->M + 127) / 128, 2, (nblk + 7) / 8);
+    dim3 cg_((f->M + 127) / 128, 2, (nblk + CONTRACT_LC - 1) / CONTRACT_LC);
     if (f->cev_used + 2 > 512) collect_contract(f);
     while (f->cev.size() < f->cev_used + 2) {
         hipEvent_t e;
