@@ -17,6 +17,8 @@
 #include <string>
 #include <vector>
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "fft.hpp"
 #include "kernels.hpp"
@@ -58,7 +60,6 @@ __global__ __launch_bounds__(LT_THREADS) void list_translate_kernel(const T *__r
     const int nmin = imn(LT, LS), nmax = imx(LT, LS), NM = 2 * nmin + 1;
     const int SL = slot_len(LT, LS, VW);
     const T *p0 = tab + (int64_t)(q / MSTM_SLOTS) * SL * MSTM_SLOTS + (q % MSTM_SLOTS) * VW;
-    auto E = [&](int e) -> T { return p0[(int64_t)(e / VW) * MSTM_SLOTS * VW + e % VW]; };
     const cplx<T> *xs = src + src_off[q];
     cplx<T> *out = dst + (int64_t)q * 4 * NBT;
     auto srcv = [&](int c, int n, int k) -> cplx<T> {
@@ -97,7 +98,18 @@ __global__ __launch_bounds__(LT_THREADS) void list_translate_kernel(const T *__r
     cplx<T> *ys = xp + 4 * NBS;                          // [c][m+nmin][n], n = 0..Ls
     cplx<T> *zs = ys + 4 * NM * (Ls + 1);                // [c][m+nmin][t], t = 0..Lt
     cplx<T> *ph = zs + 4 * NM * (Lt + 1);                // e^{i k phi}, k = -nmax..nmax
+    // the slot's table, copied from its interleaved chunks once by all
+    // threads (the stages below read each value many times)
+    const size_t stage_bytes = sizeof(cplx<T>) * ((size_t)4 * nmax * (nmax + 2) + (size_t)8 * NM * (nmax + 1) + (2 * nmax + 1));
+    T *tl = reinterpret_cast<T *>(lt_smem + pad_to((int)stage_bytes, 16));  // as list_translate_lds lays it out
+    {
+        typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type V;
+        const V *gsrc = reinterpret_cast<const V *>(p0);
+        V *d = reinterpret_cast<V *>(tl);
+        for (int c = tid; c < SL / VW; c += LT_THREADS) d[c] = gsrc[(int64_t)c * MSTM_SLOTS];
+    }
     __syncthreads();
+    auto E = [&](int e) -> T { return tl[e]; };
     const cplx<T> e1 = cmk(E(boff[NM]), DIR ? -E(boff[NM] + 1) : E(boff[NM] + 1));
     for (int k = tid - nmax; k <= nmax; k += LT_THREADS) {
         cplx<T> r = cmk((T)1, (T)0), b = k < 0 ? cconj(e1) : e1;
@@ -173,7 +185,9 @@ __global__ __launch_bounds__(LT_THREADS) void list_translate_kernel(const T *__r
 template <typename T> static size_t list_translate_lds(int LT, int LS)
 {
     const int nmin = imn(LT, LS), nmax = imx(LT, LS), NM = 2 * nmin + 1, L1 = imx(LT, LS) + 1;
-    return sizeof(cplx<T>) * (4 * (size_t)nmax * (nmax + 2) + 2 * 4 * (size_t)NM * L1 + (2 * nmax + 1));
+    // stages, then the slot's table (slot_len reals, at a 16-byte boundary)
+    const size_t stages = sizeof(cplx<T>) * (4 * (size_t)nmax * (nmax + 2) + 2 * 4 * (size_t)NM * L1 + (2 * nmax + 1));
+    return pad_to((int)stages, 16) + sizeof(T) * (size_t)slot_len(LT, LS, 16 / (int)sizeof(T));
 }
 
 // anode[r][p][n][cell] = sum over spheres of the cell of asph[i][r][p][n]
