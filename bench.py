@@ -320,8 +320,8 @@ def bench_tmatrix(args):
           "200 spheres x=1.0 m=1.5+0.01i RSA sphere fv=0.20 seed=41, T-matrix to convergence eps 1e-6",
           {"solver": best, "order": res[best]["T"], "order_bound": res[best]["T_in"], "solves": nsolve,
            **{k: {kk: v for kk, v in res[k].items() if kk != "tmat"} for k in res},
-           "mixed_vs_plain_tmat_rel": d, "tile4_ms_fp64": st["tile4_ms_fp64"],
-           "tile4_launches_fp64": int(st["tile4_launches_fp64"]), "cpu_baseline": cpu})
+           "mixed_vs_plain_tmat_rel": d, "fp64_products": int(st["matvecs_fp64"]),
+           "fp32_products": int(st["matvecs_fp32"]), "cpu_baseline": cpu})
     ctx.close()
 
 

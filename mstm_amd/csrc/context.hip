@@ -118,6 +118,16 @@ struct mstm_hip_ctx {
     std::vector<double> h_partials;
     BicgScal *d_bscal = nullptr;  // device-resident BiCG scalars of two systems
     BicgScal *h_bscal = nullptr;  // their pinned host copy
+    int *d_gate = nullptr;        // 1 while any of those systems is active
+    const int *gate = nullptr;    // set while a lazily read BiCG loop issues products
+    // HIP graphs of K lazily read BiCG iterations, one per pattern of active
+    // systems (both, the first, the second), with the buffers they were
+    // captured on and the products one replay makes
+    struct BicgGraph {
+        hipGraphExec_t exec = nullptr;
+        std::vector<intptr_t> key;
+        int matvecs = 0;
+    } bgraph[3];
     // event pool for tile-kernel timing
     std::vector<hipEvent_t> ev;
     std::vector<int> ev_prec;
@@ -154,6 +164,8 @@ struct mstm_hip_ctx {
         size_t dist_bytes = 0, dq_bytes = 0;
         double tmatrix_ms = 0;
         int tm_solves = 0;
+        double *h_pin = nullptr;  // pinned host copies of each column pair's efficiencies and merge
+        size_t pin_bytes = 0;
     } obs;
     // dense interaction matrix A = I - T H and its LU factors (direct solution)
     struct {
@@ -365,6 +377,9 @@ static void release_observables(mstm_hip_ctx *c)
     dfree(c->obs.d_vccd);
     dfree(c->obs.d_dist);
     dfree(c->obs.d_dq);
+    if (c->obs.h_pin) hipHostFree(c->obs.h_pin);
+    c->obs.h_pin = nullptr;
+    c->obs.pin_bytes = 0;
     c->obs.vccm_T = c->obs.vccm_LS = c->obs.cg_T = c->obs.vccd_T = c->obs.vccd_LS = 0;
     c->obs.part_bytes = c->obs.a0_bytes = c->obs.in_bytes = c->obs.ang_cap = 0;
     c->obs.dist_bytes = c->obs.dq_bytes = 0;
@@ -389,6 +404,7 @@ int mstm_hip_create(mstm_hip_ctx **ctx, int device)
     HIPCHK(dalloc(&c->d_partials, sizeof(double) * 16 * krylov_grid()));  // 8 regions of one value per block
     c->h_partials.resize(16 * krylov_grid());
     HIPCHK(dalloc(&c->d_bscal, sizeof(BicgScal) * 2));
+    HIPCHK(dalloc(&c->d_gate, sizeof(int)));
     HIPCHK(hipHostMalloc((void **)&c->h_bscal, sizeof(BicgScal) * 2, hipHostMallocDefault));
     *ctx = c;
     return 0;
@@ -407,6 +423,9 @@ int mstm_hip_destroy(mstm_hip_ctx *c)
     dfree(c->d_vcc);
     dfree(c->d_partials);
     dfree(c->d_bscal);
+    dfree(c->d_gate);
+    for (auto &g : c->bgraph)
+        if (g.exec) hipGraphExecDestroy(g.exec);
     if (c->h_bscal) hipHostFree(c->h_bscal);
     for (auto &e : c->ev) hipEventDestroy(e);
     hipStreamDestroy(c->st);
@@ -679,24 +698,25 @@ static int direct_matvec_buf(mstm_hip_ctx *c, int nrhs, const double *ain, doubl
         e0 = next_event(c);
         e1 = next_event(c);
     }
+    const int *gate = c->gate;
     if (prec == MSTM_HIP_FP64) {
         HIPCHK(launch_prep<double>(ain, 2 * c->neqns, (double *)xin, nrhs, flags, c->nsph, c->npad, L, c->d_order,
-                                   c->d_offset, c->d_an, c->st));
+                                   c->d_offset, c->d_an, c->st, gate));
         if (e0) hipEventRecord(e0, c->st);
         HIPCHK(launch_direct_tiles<double>((const double *)d.tab, c->d_tiles, c->ntiles, L, (const double *)xin,
-                                           (double *)part, c->npad, nr, G, c->st));
+                                           (double *)part, c->npad, nr, G, c->st, gate));
         if (e1) hipEventRecord(e1, c->st);
         HIPCHK(launch_reduce_post<double>((const double *)part, c->nblocks * G, aout, 2 * c->neqns, nrhs, flags, c->nsph,
-                                          c->npad, L, c->d_order, c->d_offset, c->d_an, c->st));
+                                          c->npad, L, c->d_order, c->d_offset, c->d_an, c->st, gate));
     } else {
         HIPCHK(launch_prep<float>(ain, 2 * c->neqns, (float *)xin, nrhs, flags, c->nsph, c->npad, L, c->d_order,
-                                  c->d_offset, c->d_an, c->st));
+                                  c->d_offset, c->d_an, c->st, gate));
         if (e0) hipEventRecord(e0, c->st);
         HIPCHK(launch_direct_tiles<float>((const float *)d.tab, c->d_tiles, c->ntiles, L, (const float *)xin,
-                                          (float *)part, c->npad, nr, G, c->st));
+                                          (float *)part, c->npad, nr, G, c->st, gate));
         if (e1) hipEventRecord(e1, c->st);
         HIPCHK(launch_reduce_post<float>((const float *)part, c->nblocks * G, aout, 2 * c->neqns, nrhs, flags, c->nsph,
-                                         c->npad, L, c->d_order, c->d_offset, c->d_an, c->st));
+                                         c->npad, L, c->d_order, c->d_offset, c->d_an, c->st, gate));
     }
     if (e0) c->ev_prec.push_back(prec + 2 * (nr == 4));
     c->stats.matvecs[prec]++;
@@ -787,6 +807,60 @@ static inline double *pregion(mstm_hip_ctx *c, int s) { return c->d_partials + (
 static inline int pregion_off(int s) { return 2 * krylov_grid() * s; }
 
 
+// K iterations of a lazily read BiCG loop (small clusters, direct
+// operator) as one replay of a HIP graph captured from step(): the launches
+// of an iteration are identical every time (iteration numbers, scalars and
+// the stop test live on the device; flags and pointers are fixed), so a
+// graph captured once per pattern of active systems and set of buffers
+// replaces 8 K launches by one.  Tile timing events are left out of the
+// graph; the product count is kept.
+template <typename F>
+static int bicg_graph_steps(mstm_hip_ctx *c, int ns, int prec, const bool act[2], const double *x0, int K, F &&step)
+{
+    const DirectOp &d = c->dop[prec];
+    const std::vector<intptr_t> key = {ns, prec, act[0], act[1], K, (intptr_t)x0, (intptr_t)c->d_kv, (intptr_t)c->kv_n,
+                                       c->neqns, (intptr_t)d.tab, (intptr_t)d.xin, (intptr_t)d.part, c->tgroups,
+                                       c->ntiles, c->nblocks, c->Lmax, c->nsph, c->npad, (intptr_t)c->d_tiles,
+                                       (intptr_t)c->d_order, (intptr_t)c->d_offset, (intptr_t)c->d_an,
+                                       (intptr_t)c->d_partials, (intptr_t)c->d_bscal, (intptr_t)c->gate};
+    mstm_hip_ctx::BicgGraph &g = c->bgraph[act[0] && act[1] ? 0 : (act[0] ? 1 : 2)];
+    if (!g.exec || g.key != key) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+        g.key.clear();
+        const bool timing = c->timing;
+        c->timing = false;
+        const long mv0 = c->stats.matvecs[prec];
+        hipGraph_t gr = nullptr;
+        int rc = 0;
+        hipError_t e = hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            for (int q = 0; q < K && !rc; q++) rc = step();
+            const hipError_t e2 = hipStreamEndCapture(c->st, &gr);
+            if (!rc && e2 != hipSuccess)
+                rc = fail(c, MSTM_HIP_EHIP, "BiCG graph capture: %s", hipGetErrorString(e2));
+        } else {
+            rc = fail(c, MSTM_HIP_EHIP, "BiCG graph capture: %s", hipGetErrorString(e));
+        }
+        c->timing = timing;
+        g.matvecs = (int)(c->stats.matvecs[prec] - mv0);
+        c->stats.matvecs[prec] = mv0;
+        if (!rc) {
+            e = hipGraphInstantiate(&g.exec, gr, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                g.exec = nullptr;
+                rc = fail(c, MSTM_HIP_EHIP, "BiCG graph instantiate: %s", hipGetErrorString(e));
+            }
+        }
+        if (gr) (void)hipGraphDestroy(gr);
+        if (rc) return rc;
+        g.key = key;
+    }
+    HIPCHK(hipGraphLaunch(g.exec, c->st));
+    c->stats.matvecs[prec] += g.matvecs;
+    return 0;
+}
+
 // cbicg (mstm-solver-8.f90:654-949) for ns = 1 or 2 independent systems
 // A x_s = b_s at once (the two polarizations of fixedorsoln).  b and x are
 // blocks of ns contiguous vectors; every system keeps the reference's own
@@ -819,21 +893,42 @@ static int bicg2_dev(mstm_hip_ctx *c, int ns, int niter, const double *eps, cons
         hS[s] = BicgScal{};
         hS[s].eps = eps[s];
     }
-    // the scalars live on the device; the host reads them once per iteration
+    // the scalars live on the device; the host reads them every K iterations
     auto sync_state = [&]() -> int {
         HIPCHK(hipMemcpyAsync(hS, S, sizeof(BicgScal) * ns, hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
         return 0;
     };
+    // host reads of the scalars every K iterations: a stopped system's
+    // kernels return at once on the device, so the iterates and counts are
+    // the same for any K.  Products of small clusters cost less than the
+    // read: there the loop does not wait for the start-up state either, and
+    // its products carry the device gate (1 while a system is active), so
+    // the up to K - 1 products enqueued after the last system stops return at
+    // once.
+    const bool lazy = !use_fft && c->nsph <= 500;
+    const int K = lazy ? 4 : 1;
+    int *gate = lazy ? c->d_gate : nullptr;
+    struct GateScope {  // products outside this loop are never gated
+        mstm_hip_ctx *c;
+        ~GateScope() { c->gate = nullptr; }
+    } gate_scope{c};
+    c->gate = gate;
     HIPCHK(hipMemcpyAsync(S, hS, sizeof(BicgScal) * ns, hipMemcpyHostToDevice, c->st));
     // |b|^2
     for (int s = 0; s < ns; s++) HIPCHK(launch_zdot(B[s], B[s], n, 1, pregion(c, s), nullptr, c->st));
-    HIPCHK(launch_bicg_scal(0, c->d_partials, region, S, ns, 0, c->st));
-    int rc = sync_state();
-    if (rc) return rc;
-    bool act[2] = {hS[0].active != 0, ns == 2 && hS[1].active != 0};
-    if (!act[0] && !act[1]) return 0;
-    // r = b - A x (:827-834), one product for the active systems' x
+    HIPCHK(launch_bicg_scal(0, c->d_partials, region, S, ns, c->st, gate));
+    int rc = 0;
+    bool act[2] = {true, ns == 2};
+    if (!lazy) {
+        rc = sync_state();
+        if (rc) return rc;
+        act[0] = hS[0].active != 0;
+        act[1] = ns == 2 && hS[1].active != 0;
+        if (!act[0] && !act[1]) return 0;
+    }
+    // r = b - A x (:827-834), one product for the active systems' x (lazy:
+    // both, a system with b = 0 having been stopped on the device)
     {
         int f[2] = {2, 2};
         if (act[0] && act[1]) rc = matvec(c, 2, X[0], Rr[0], f, prec, use_fft);
@@ -844,44 +939,57 @@ static int bicg2_dev(mstm_hip_ctx *c, int ns, int niter, const double *eps, cons
     }
     for (int s = 0; s < ns; s++)
         if (act[s]) HIPCHK(launch_bicg_init(Rr[s], Q[s], W[s], P[s], B[s], X[s], n, pregion(c, s), c->st));
-    HIPCHK(launch_bicg_scal(1, c->d_partials, region, S, ns, 0, c->st));
-    rc = sync_state();
-    if (rc) return rc;
-    act[0] = hS[0].active != 0;
-    act[1] = ns == 2 && hS[1].active != 0;
-    // host reads of the scalars every K iterations: a stopped system's
-    // kernels return at once on the device, so the iterates and counts are
-    // the same for any K; products of small clusters cost less than the
-    // read, and run up to K - 1 surplus products after the last system stops
-    const int K = (!use_fft && c->nsph <= 500) ? 4 : 1;
-    int iter, synced = 0;
-    for (iter = 1; iter <= niter; iter++) {
-        if (!act[0] && !act[1]) break;
-        int f4[4] = {2, 3, 2, 3};
-        if (act[0] && act[1]) rc = matvec(c, 4, P[0], AP[0], f4, prec, use_fft);
-        else rc = matvec(c, 2, act[0] ? P[0] : P[1], act[0] ? AP[0] : AP[1], f4, prec, use_fft);
+    HIPCHK(launch_bicg_scal(1, c->d_partials, region, S, ns, c->st, gate));
+    if (!lazy) {
+        rc = sync_state();
         if (rc) return rc;
-        // both systems' vector steps in one launch each; the kernels of a
-        // system the device has stopped return at once
+        act[0] = hS[0].active != 0;
+        act[1] = ns == 2 && hS[1].active != 0;
+    }
+    // one iteration (:836-948) of the active systems: the product of p and
+    // w, then both systems' vector steps in one launch each (the kernels of
+    // a system the device has stopped return at once)
+    auto step = [&]() -> int {
+        int f4[4] = {2, 3, 2, 3};
+        int rc_;
+        if (act[0] && act[1]) rc_ = matvec(c, 4, P[0], AP[0], f4, prec, use_fft);
+        else rc_ = matvec(c, 2, act[0] ? P[0] : P[1], act[0] ? AP[0] : AP[1], f4, prec, use_fft);
+        if (rc_) return rc_;
         HIPCHK(launch_bicg_a_dev(AP[0], AW[0], P[0], W[0], 2 * (int64_t)V, n, pregion(c, 0), region, S, ns, c->st));
-        HIPCHK(launch_bicg_scal(2, c->d_partials, region, S, ns, iter, c->st));
+        HIPCHK(launch_bicg_scal(2, c->d_partials, region, S, ns, c->st, gate));
         HIPCHK(launch_bicg_b_dev(X[0], Rr[0], Q[0], P[0], AP[0], AW[0], (int64_t)V, 2 * (int64_t)V, S, n, pregion(c, 0),
                                  region, ns, c->st));
-        HIPCHK(launch_bicg_scal(3, c->d_partials, region, S, ns, iter, c->st));
+        HIPCHK(launch_bicg_scal(3, c->d_partials, region, S, ns, c->st, gate));
         HIPCHK(launch_bicg_c_dev(P[0], W[0], Rr[0], Q[0], (int64_t)V, 2 * (int64_t)V, S, n, ns, c->st));
-        if (iter % K && iter < niter) continue;
+        return 0;
+    };
+    int iter = 1, synced = 0;
+    while (iter <= niter) {
+        if (!act[0] && !act[1]) break;
+        const int nst = std::min(K, niter - iter + 1);
+        if (lazy && nst == K) rc = bicg_graph_steps(c, ns, prec, act, X[0], K, step);
+        else
+            for (int q = 0; q < nst && !rc; q++) rc = step();
+        if (rc) return rc;
+        iter += nst;
+        const int last = iter - 1;
         rc = sync_state();
         if (rc) return rc;
         for (int s = 0; s < ns; s++) {
             if (!act[s]) continue;
             // steps since the last read; a step that failed the breakdown test
             // does not count, as in the host loop
-            const int done = hS[s].active ? iter - synced : hS[s].it - synced - hS[s].brk;
+            const int done = hS[s].active ? last - synced : hS[s].it - synced - hS[s].brk;
             if (prec == MSTM_HIP_FP64) c->stats.iters_fp64 += done;
             else c->stats.iters_fp32 += done;
             act[s] = hS[s].active != 0;
         }
-        synced = iter;
+        synced = last;
+    }
+    if (lazy && niter < 1) {  // no iteration ran: the start-up state decides
+        rc = sync_state();
+        if (rc) return rc;
+        for (int s = 0; s < ns; s++) act[s] = hS[s].active != 0;
     }
     for (int s = 0; s < ns; s++) {
         it[s] = act[s] ? iter : hS[s].it;  // ran out: niter + 1
@@ -2488,7 +2596,18 @@ int mstm_hip_tmatrix_solution(mstm_hip_ctx *c, double eps, int niter, double con
     for (int i = 0; i < nsph; i++) vr += c->radius[i] * c->radius[i] * c->radius[i];
     vr = std::pow(vr, (double).333333f);
     const double vr2 = vr * vr;
-    std::vector<double> hq((size_t)9 * nsph), ha0((size_t)8 * bT), sq((size_t)3 * nsph, 0.0);
+    // the per-pair copies land in pinned memory, so that they are
+    // asynchronous and the pair waits for the device only once
+    const size_t pin_need = sizeof(double) * ((size_t)9 * nsph + (size_t)8 * bT);
+    if (c->obs.pin_bytes < pin_need) {
+        if (c->obs.h_pin) hipHostFree(c->obs.h_pin);
+        c->obs.h_pin = nullptr;
+        c->obs.pin_bytes = 0;
+        HIPCHK(hipHostMalloc((void **)&c->obs.h_pin, pin_need, hipHostMallocDefault));
+        c->obs.pin_bytes = pin_need;
+    }
+    double *hq = c->obs.h_pin, *ha0 = c->obs.h_pin + (size_t)9 * nsph;
+    std::vector<double> sq((size_t)3 * nsph, 0.0);
     double qt[3] = {0.0, 0.0, 0.0}, qeffold = 0.0;
     int64_t nt = 0;
     int lreach = T;
@@ -2536,16 +2655,16 @@ int mstm_hip_tmatrix_solution(mstm_hip_ctx *c, double eps, int niter, double con
             // per-sphere efficiencies of each column (qefficiencyfactors, npol = 1)
             HIPCHK(launch_qeff(x, dp, 2 * n, nsph, c->d_radius, c->d_order, c->d_offset, c->d_mieoff, c->d_an,
                                c->rimed, c->obs.d_dq, c->st));
-            HIPCHK(hipMemcpyAsync(hq.data(), c->obs.d_dq, sizeof(double) * 9 * nsph, hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipMemcpyAsync(hq, c->obs.d_dq, sizeof(double) * 9 * nsph, hipMemcpyDeviceToHost, c->st));
             // both columns merged to the origin at order l (target orders min(l, torder_i))
             HIPCHK(launch_merge(x, n, c->d_order, c->d_offset, c->obs.d_torder, c->d_pos, nsph, l, c->Lmax,
                                 c->obs.r0, c->rimed, c->obs.d_vccm, c->obs.d_part, G, c->obs.d_a0, c->st));
-            HIPCHK(hipMemcpyAsync(ha0.data(), c->obs.d_a0, sizeof(double) * 8 * b0, hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipMemcpyAsync(ha0, c->obs.d_a0, sizeof(double) * 8 * b0, hipMemcpyDeviceToHost, c->st));
             HIPCHK(hipStreamSynchronize(c->st));
             for (int q = 0; q < 2; q++) {
                 for (int i = 0; i < nsph; i++)
                     for (int e = 0; e < 3; e++) sq[3 * i + e] += hq[9 * i + 3 * (q + 1) + e];
-                const double *a0 = ha0.data() + 4 * (size_t)b0 * q;  // [mode][b0] complex
+                const double *a0 = ha0 + 4 * (size_t)b0 * q;  // [mode][b0] complex
                 const double d1 = -2.0 / vr2 * a0[2 * ((size_t)q * b0 + amn(k0, l, l))];
                 double d3 = 0.0;
                 for (int e = 0; e < 2 * b0; e++) d3 += a0[2 * e] * a0[2 * e] + a0[2 * e + 1] * a0[2 * e + 1];

@@ -570,8 +570,9 @@ __device__ __forceinline__ void tile_body(const T *__restrict__ tab, int2 bt, in
 template <typename T, int L, int NR, int NRL>
 __global__ __launch_bounds__(64 * NRL) __attribute__((amdgpu_waves_per_eu(tile_waves<T, L, NRL>()))) void direct_tile_kernel(
     const T *__restrict__ tab, const int2 *__restrict__ tiles, int nunits, const cplx<T> *__restrict__ xin,
-    cplx<T> *__restrict__ part, int npad, int G)
+    cplx<T> *__restrict__ part, int npad, int G, const int *__restrict__ gate)
 {
+    if (gate && !*gate) return;  // the issuing solver has stopped every system
     constexpr int H = NR / NRL;  // workgroups per (tile, step group)
     int u = blockIdx.x, h = 0;
     if constexpr (H > 1) {
@@ -637,7 +638,7 @@ template <typename T, int L> constexpr int tile4_nrl() { return tile_src_staged<
 
 template <typename T>
 hipError_t launch_direct_tiles(const T *tab, const int2 *tiles, int ntiles, int L, const T *xin, T *partial, int npad,
-                               int nr, int G, hipStream_t st)
+                               int nr, int G, hipStream_t st, const int *gate)
 {
     const cplx<T> *x = reinterpret_cast<const cplx<T> *>(xin);
     cplx<T> *p = reinterpret_cast<cplx<T> *>(partial);
@@ -648,13 +649,13 @@ hipError_t launch_direct_tiles(const T *tab, const int2 *tiles, int ntiles, int 
 #define CASE(LL)                                                                                                       \
     case LL * 8 + 2:                                                                                                   \
         hipLaunchKernelGGL((direct_tile_kernel<T, LL, 2, 2>), dim3((unsigned)nunits), dim3(128), 0, st, tab, tiles,    \
-                           nunits, x, p, npad, G);                                                                     \
+                           nunits, x, p, npad, G, gate);                                                               \
         break;                                                                                                         \
     case LL * 8 + 4: {                                                                                                 \
         constexpr int NRL = tile4_nrl<T, LL>(), H = 4 / NRL;                                                           \
         const unsigned nb = H == 1 ? (unsigned)nunits : (unsigned)((nunits + 7) / 8 * 8 * H);                          \
         hipLaunchKernelGGL((direct_tile_kernel<T, LL, 4, NRL>), dim3(nb), dim3(64 * NRL), 0, st, tab, tiles, nunits,   \
-                           x, p, npad, G);                                                                             \
+                           x, p, npad, G, gate);                                                                       \
         break;                                                                                                         \
     }
 #ifdef MSTM_TILE_L_ONLY
@@ -668,9 +669,9 @@ hipError_t launch_direct_tiles(const T *tab, const int2 *tiles, int ntiles, int 
     return hipGetLastError();
 }
 template hipError_t launch_direct_tiles<float>(const float *, const int2 *, int, int, const float *, float *, int, int,
-                                               int, hipStream_t);
+                                               int, hipStream_t, const int *);
 template hipError_t launch_direct_tiles<double>(const double *, const int2 *, int, int, const double *, double *, int,
-                                                int, int, hipStream_t);
+                                                int, int, hipStream_t, const int *);
 
 // ---------------------------------------------------------------------------
 // Input preparation (sphereinteraction :88-97): for a con_tran column
@@ -698,8 +699,10 @@ __device__ __forceinline__ void decode_mn(int e, int L, int &m, int &n)
 template <typename T>
 __global__ void prep_kernel(const cplx<double> *__restrict__ ain, int ldain, cplx<T> *__restrict__ xin, int nrhs,
                             int4 flags, int nr, int nsph, int npad, int Lp, const int *__restrict__ order,
-                            const int *__restrict__ offset, const cplx<double> *__restrict__ an)
+                            const int *__restrict__ offset, const cplx<double> *__restrict__ an,
+                            const int *__restrict__ gate)
 {
+    if (gate && !*gate) return;
     const int NBP = Lp * (Lp + 2);
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (int64_t)nr * npad * NBP) return;
@@ -747,7 +750,7 @@ static int4 flags4(const int *flags, int nrhs)
 
 template <typename T>
 hipError_t launch_prep(const double *ain, int ldain, T *xin, int nrhs, const int *flags, int nsph, int npad, int Lp,
-                       const int *order, const int *offset, const double *an, hipStream_t st)
+                       const int *order, const int *offset, const double *an, hipStream_t st, const int *gate)
 {
     if (nrhs < 1 || nrhs > 4) return hipErrorInvalidValue;
     const int nr = nrhs > 2 ? 4 : 2;
@@ -757,13 +760,13 @@ hipError_t launch_prep(const double *ain, int ldain, T *xin, int nrhs, const int
     hipLaunchKernelGGL(prep_kernel<T>, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, st,
                        reinterpret_cast<const cplx<double> *>(ain), ldain / 2, reinterpret_cast<cplx<T> *>(xin), nrhs,
                        flags4(flags, nrhs), nr, nsph, npad, Lp, order, offset,
-                       reinterpret_cast<const cplx<double> *>(an));
+                       reinterpret_cast<const cplx<double> *>(an), gate);
     return hipGetLastError();
 }
 template hipError_t launch_prep<float>(const double *, int, float *, int, const int *, int, int, int, const int *,
-                                       const int *, const double *, hipStream_t);
+                                       const int *, const double *, hipStream_t, const int *);
 template hipError_t launch_prep<double>(const double *, int, double *, int, const int *, int, int, int, const int *,
-                                        const int *, const double *, hipStream_t);
+                                        const int *, const double *, hipStream_t, const int *);
 
 // ---------------------------------------------------------------------------
 // Sum of the per-block partials + post-op (sphereinteraction :158-168):
@@ -780,8 +783,10 @@ __global__ __launch_bounds__(256) void reduce_post_kernel(const cplx<T> *__restr
                                                           cplx<double> *__restrict__ aout, int ldaout, int nrhs,
                                                           int4 flags, int nr, int nsph, int npad, int Lp,
                                                           const int *__restrict__ order, const int *__restrict__ offset,
-                                                          const cplx<double> *__restrict__ an)
+                                                          const cplx<double> *__restrict__ an,
+                                                          const int *__restrict__ gate)
 {
+    if (gate && !*gate) return;  // uniform over the grid
     const int NBP = Lp * (Lp + 2);
     const int j = threadIdx.x % KS;
     const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / KS;
@@ -841,7 +846,7 @@ __global__ __launch_bounds__(256) void reduce_post_kernel(const cplx<T> *__restr
 template <typename T>
 hipError_t launch_reduce_post(const T *partial, int nparts, double *aout, int ldaout, int nrhs, const int *flags,
                               int nsph, int npad, int Lp, const int *order, const int *offset, const double *an,
-                              hipStream_t st)
+                              hipStream_t st, const int *gate)
 {
     if (nrhs < 1 || nrhs > 4) return hipErrorInvalidValue;
     const int nr = nrhs > 2 ? 4 : 2;
@@ -852,7 +857,7 @@ hipError_t launch_reduce_post(const T *partial, int nparts, double *aout, int ld
     hipLaunchKernelGGL((reduce_post_kernel<T, KS>), dim3((unsigned)((n * KS + bs - 1) / bs)), dim3(bs), 0, st,         \
                        reinterpret_cast<const cplx<T> *>(partial), nparts, reinterpret_cast<cplx<double> *>(aout),     \
                        ldaout / 2, nrhs, flags4(flags, nrhs), nr, nsph, npad, Lp, order, offset,                       \
-                       reinterpret_cast<const cplx<double> *>(an))
+                       reinterpret_cast<const cplx<double> *>(an), gate)
     // lanes per output: with many outputs one lane each keeps the loads of
     // a wave on consecutive elements of one partial; with few (small
     // clusters, many step-group partials) several lanes per output keep
@@ -865,6 +870,6 @@ hipError_t launch_reduce_post(const T *partial, int nparts, double *aout, int ld
     return hipGetLastError();
 }
 template hipError_t launch_reduce_post<float>(const float *, int, double *, int, int, const int *, int, int, int,
-                                              const int *, const int *, const double *, hipStream_t);
+                                              const int *, const int *, const double *, hipStream_t, const int *);
 template hipError_t launch_reduce_post<double>(const double *, int, double *, int, int, const int *, int, int, int,
-                                               const int *, const int *, const double *, hipStream_t);
+                                               const int *, const int *, const double *, hipStream_t, const int *);

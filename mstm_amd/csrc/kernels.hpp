@@ -16,18 +16,23 @@ hipError_t launch_list_tables(T *tab, const double *rtran, int nslots, int LT, i
                               const double *vcc, int LM, hipStream_t st);
 
 // direct.hip
+// gate (optional, device int): the kernels of a product return at once
+// when *gate == 0 (every system of the Krylov loop that issued the product
+// has stopped), so that products enqueued ahead of the host's next read of
+// the solver state cost little.
 template <typename T>
 // G (1, 2, 4, 8 or 16) workgroups per tile share its 16 steps; the partials
 // are then (partner block x G) per target
 hipError_t launch_direct_tiles(const T *tab, const int2 *tiles, int ntiles, int L, const T *xin, T *partial, int npad,
-                               int nr, int G, hipStream_t st);
+                               int nr, int G, hipStream_t st, const int *gate = nullptr);
 template <typename T>
 hipError_t launch_prep(const double *ain, int ldain, T *xin, int nrhs, const int *flags, int nsph, int npad, int Lp,
-                       const int *order, const int *offset, const double *an, hipStream_t st);
+                       const int *order, const int *offset, const double *an, hipStream_t st,
+                       const int *gate = nullptr);
 template <typename T>
 hipError_t launch_reduce_post(const T *partial, int nparts, double *aout, int ldaout, int nrhs, const int *flags,
                               int nsph, int npad, int Lp, const int *order, const int *offset, const double *an,
-                              hipStream_t st);
+                              hipStream_t st, const int *gate = nullptr);
 
 // mie.hip
 hipError_t launch_mie(const double *x, const double *mre, const double *mim, double mmed, int nsph, double qeps,
@@ -50,13 +55,18 @@ struct BicgScal {
     cplx<double> csk, cak, cbk;
     double enorm, err, eps;
     int active, it, brk;
+    int cur;  // iterations begun while active (the loop's iteration number)
 };
 // step 0: enorm = |b|^2 (active = enorm != 0); 1: csk = r.r (inactive if 0);
-// 2: den = <w, A p>, cak = csk / den (breakdown: inactive, it = iter, brk = 1);
-// 3: <q, r>, <r, r>: err, stop test, cbk, csk.  partials: region s of the
-// system s at partials + s * region (complex), one or two values per block.
-hipError_t launch_bicg_scal(int step, const double *partials, int64_t region, BicgScal *st, int ns, int iter,
-                            hipStream_t st_);
+// 2: a new iteration (cur + 1), den = <w, A p>, cak = csk / den (breakdown:
+// inactive, it = cur, brk = 1); 3: <q, r>, <r, r>: err, stop test (it = cur),
+// cbk, csk.  The iteration number is counted on the device, so the launches
+// of an iteration are the same every time (they can be replayed from a
+// graph).  partials: region s of the system s at partials + s * region
+// (complex), one or two values per block.  gate (optional): set to 1 while
+// any system is active, else 0.
+hipError_t launch_bicg_scal(int step, const double *partials, int64_t region, BicgScal *st, int ns, hipStream_t st_,
+                            int *gate = nullptr);
 // the vector steps of ns systems in one launch: system s's vectors at the
 // given pointers plus s * sv (x, r, q) or s * sp (p, w, Ap, A^H w) doubles,
 // its partial sums at partials + s * region complex values

@@ -143,7 +143,8 @@ int krylov_grid() { return KGRID; }
 // of them stopped.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(KB) void bicg_scal_kernel(int step, const cplx<double> *__restrict__ partials,
-                                                       int64_t region, BicgScal *__restrict__ sc, int ns, int iter)
+                                                       int64_t region, BicgScal *__restrict__ sc, int ns,
+                                                       int *__restrict__ gate)
 {
     __shared__ cplx<double> red[2][KB];
     const int nv = step == 3 ? 2 : 1;
@@ -172,6 +173,7 @@ __global__ __launch_bounds__(KB) void bicg_scal_kernel(int step, const cplx<doub
                 S.active = v0.re != 0.0;
                 S.it = 0;
                 S.brk = 0;
+                S.cur = 0;
                 S.err = 0.0;
                 S.cak = S.cbk = zero;
             } else if (skip) {
@@ -180,9 +182,10 @@ __global__ __launch_bounds__(KB) void bicg_scal_kernel(int step, const cplx<doub
                 S.csk = v0;
                 if (v0.re == 0.0 && v0.im == 0.0) S.active = 0;  // it stays 0
             } else if (step == 2) {
+                S.cur++;
                 if (v0.re == 0.0 && v0.im == 0.0) {  // breakdown: the Fortran loop exits here
                     S.active = 0;
-                    S.it = iter;
+                    S.it = S.cur;
                     S.brk = 1;
                     S.cak = zero;
                 } else {
@@ -192,7 +195,7 @@ __global__ __launch_bounds__(KB) void bicg_scal_kernel(int step, const cplx<doub
                 S.err = v1.re / S.enorm;
                 if (S.err < S.eps || (S.csk.re == 0.0 && S.csk.im == 0.0)) {
                     S.active = 0;
-                    S.it = iter;
+                    S.it = S.cur;
                     S.cbk = zero;
                 } else {
                     S.cbk = cdiv(v0, S.csk);
@@ -201,6 +204,11 @@ __global__ __launch_bounds__(KB) void bicg_scal_kernel(int step, const cplx<doub
             }
         }
         __syncthreads();
+    }
+    if (gate && threadIdx.x == 0) {
+        int any = 0;
+        for (int s = 0; s < ns; s++) any |= sc[s].active != 0;
+        *gate = any;
     }
 }
 
@@ -289,12 +297,12 @@ __global__ void bicg_c_dev_kernel(cplx<double> *__restrict__ cp, cplx<double> *_
     }
 }
 
-hipError_t launch_bicg_scal(int step, const double *partials, int64_t region, BicgScal *sc, int ns, int iter,
-                            hipStream_t st)
+hipError_t launch_bicg_scal(int step, const double *partials, int64_t region, BicgScal *sc, int ns, hipStream_t st,
+                            int *gate)
 {
     if (ns < 1 || ns > 2 || step < 0 || step > 3) return hipErrorInvalidValue;
     hipLaunchKernelGGL(bicg_scal_kernel, dim3(1), dim3(KB), 0, st, step,
-                       reinterpret_cast<const cplx<double> *>(partials), region, sc, ns, iter);
+                       reinterpret_cast<const cplx<double> *>(partials), region, sc, ns, gate);
     return hipGetLastError();
 }
 hipError_t launch_bicg_a_dev(double *cap, double *caw, const double *cp, const double *cw, int64_t sp, int n,

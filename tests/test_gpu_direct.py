@@ -128,6 +128,41 @@ def test_nonconvergence_flag(gpu_ctx):
     assert r["istat"] == 1 and r["iterations"] == 3
 
 
+def test_graph_replays_match_oracle(gpu_ctx):
+    """Small clusters run BiCG iterations four at a time from a captured HIP
+    graph: repeated solves, with eager products and reads of the state in
+    between, each give the oracle's iterates; a limit that is not a multiple
+    of four (graph replays, then single eager iterations) gives the oracle's
+    iterate at that limit; a one-system solve uses its own graph."""
+    from oracle import oracle as O
+
+    g = golden("cfg1_direct")
+    load_cluster(gpu_ctx, g)
+    oc = O.OracleCluster(g["pos"], g["radius"], g["ri"])
+    ro = oc.fixedorsoln(eps=1e-6, niter=10000)
+    rng = np.random.default_rng(7)
+    a = rng.normal(size=(2, oc.neqns)) + 1j * rng.normal(size=(2, oc.neqns))
+    ref_prod = oc.sphereinteraction(a, [0, 1], [1, 1])
+    for _ in range(3):
+        r = gpu_ctx.fixedorsoln(eps=1e-6, niter=10000)
+        assert r["iterations"] == ro["iterations"] and r["istat"] == 0
+        assert rel(r["amnp"], ro["amnp"]) < FP64_TOL
+        out = gpu_ctx.sphereinteraction(a, [False, True], [True, True])
+        assert rel(out, ref_prod) < 1e-12
+        gpu_ctx.stats()
+    for niter in (5, 6):
+        r = gpu_ctx.fixedorsoln(eps=1e-6, niter=niter)
+        rn = oc.fixedorsoln(eps=1e-6, niter=niter)
+        assert r["istat"] == rn["istat"] == 1 and r["iterations"] == rn["iterations"] == niter + 1
+        assert rel(r["amnp"], rn["amnp"]) < FP64_TOL
+    f = oc.planewave()
+    b = oc.mie_mult(f[1])
+    xo, ito, _ = oc.cbicg(b, b, niter=200, eps=1e-8)
+    for _ in range(2):
+        xg, itg, _ = gpu_ctx.cbicg(200, 1e-8, b, b)
+        assert itg == ito and rel(xg, xo) < FP64_TOL
+
+
 def test_fixedorsoln_oblique_and_from_above_matches_oracle(gpu_ctx):
     """Incidence at beta = 40 deg from below (dir 1) and beta = 130 deg from
     above (dir 2, cos beta < 0; layerplanewavecoef :3934): the plain FP64
