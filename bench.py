@@ -246,6 +246,9 @@ def bench_fft(args):
                     achieved=round(ach, 1), peak=HBM_PEAK_GBPS, unit="GB/s", frac=round(ach / HBM_PEAK_GBPS, 4),
                     traffic=None, algorithmic_bytes=alg_c, avg_launch_ms=round(t_c, 4), launches=int(nc),
                     flops_per_launch=2 * 2 * nblk * nblk * M * 8)
+    conc = None
+    if len(dirs) > 1:
+        conc = orientations_concurrent(cl, dirs, args.steps, q_mix)
     t_plain, q_plain, it_plain = solve_all(SOLVE_PLAIN, True)
     ctx.setup_direct(FP64)
     t_dir, q_dir, it_dir = solve_all(SOLVE_PLAIN, False)
@@ -262,8 +265,52 @@ def bench_fft(args):
            "q_error": {"mixed_fft_vs_plain_fft": rel(q_mix[:, 0, :2], q_plain[:, 0, :2]),
                        "mixed_fft_vs_direct": rel(q_mix[:, 0, :2], q_dir[:, 0, :2]),
                        "plain_fft_vs_direct": rel(q_plain[:, 0, :2], q_dir[:, 0, :2])},
-           "roofline": roof, "cpu_baseline": None})
+           "orientations_concurrent": conc, "roofline": roof, "cpu_baseline": None})
     ctx.close()
+
+
+def orientations_concurrent(cl, dirs, steps, q_serial):
+    """The orientations at once: one context (own stream, FFT operator and
+    Krylov vectors) and one host thread per orientation, each running the
+    mixed FFT solve of its direction; the library calls release the GIL.
+    Returns the time per orientation and the largest Q difference from the
+    serial solves (the solves are the same, so it is 0)."""
+    import threading
+
+    from mstm_amd.solver import FP32, FP64, SOLVE_MIXED, HipContext, qtotcalc
+
+    ctxs = []
+    for _ in dirs:
+        c = HipContext(0)
+        c.set_spheres(cl.pos, cl.radius, cl.ri, 1.0, 1e-6)
+        c.node_selection()
+        c.setup_fft(FP64)
+        c.setup_fft(FP32)
+        ctxs.append(c)
+    out = [None] * len(dirs)
+
+    def run(k):
+        beta, alpha = dirs[k]
+        sd = 1 if np.cos(beta) >= 0 else 2
+        out[k] = ctxs[k].fixedorsoln(alpha=alpha, sinc=float(np.sin(beta)), eps=1e-6, niter=10000,
+                                     solver=SOLVE_MIXED, fft_option=True, direction=sd)
+
+    def once():
+        th = [threading.Thread(target=run, args=(k,)) for k in range(len(dirs))]
+        t0 = time.perf_counter()
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        return (time.perf_counter() - t0) / len(dirs)
+
+    once()  # warm-up
+    ts = [once() for _ in range(max(1, steps))]
+    q = np.array([qtotcalc(cl.radius, r["qeff"]) for r in out])
+    d = float(np.abs(q[:, 0, :2] - q_serial[:, 0, :2]).max())
+    for c in ctxs:
+        c.close()
+    return {"contexts": len(dirs), "s_per_orientation": round(float(np.mean(ts)), 4), "q_abs_diff_vs_serial": d}
 
 
 def bench_tmatrix(args):
@@ -280,6 +327,8 @@ def bench_tmatrix(args):
     cl = G.rsa_in_sphere(200, 1.0, 0.20, seed=41, ri=1.5 + 0.01j)
     ctx = HipContext(0)
     ctx.set_spheres(cl.pos, cl.radius, cl.ri, 1.0, 1e-6)
+    if args.no_graphs:
+        ctx.set_graphs(False)
     res = {}
     for name, solver in (("plain_fp64", SOLVE_PLAIN), ("mixed", SOLVE_MIXED)):
         for _ in range(args.warmup):
@@ -416,6 +465,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tight", action="store_true", help="skip the tight FP64 reference solve")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="tmatrix: issue the small-cluster BiCG launches one by one instead of replaying HIP graphs "
+                         "(for kernel tracing; same results)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many MiB (rocprof FETCH/WRITE calibration)")
     ap.add_argument("--workload", default="direct5000",

@@ -168,6 +168,12 @@ int mstm_hip_set_inner_solver(mstm_hip_ctx *ctx, int kind);
  * solver from b; 0 hands it over at once, after the first FP64 residual. */
 int mstm_hip_set_refinement_cycles(mstm_hip_ctx *ctx, int cycles);
 
+/* BiCG iterations of small clusters (up to 500 spheres, direct operator) are
+ * replayed four at a time from a HIP graph captured once per set of buffers
+ * (no counterpart in MSTM).  on = 0 issues the same launches one by one, with
+ * identical results: for tools that do not follow graph launches.  Default 1. */
+int mstm_hip_set_graphs(mstm_hip_ctx *ctx, int on);
+
 /* ---------------------------------------------------------------------------
  * Post-solve observables of a single-origin expansion (a cluster without
  * plane boundaries, single_origin_expansion = .true., the reference default;

@@ -118,6 +118,7 @@ struct mstm_hip_ctx {
     std::vector<double> h_partials;
     BicgScal *d_bscal = nullptr;  // device-resident BiCG scalars of two systems
     BicgScal *h_bscal = nullptr;  // their pinned host copy
+    bool graphs = true;           // replay lazily read BiCG iterations from HIP graphs
     int *d_gate = nullptr;        // 1 while any of those systems is active
     const int *gate = nullptr;    // set while a lazily read BiCG loop issues products
     // HIP graphs of K lazily read BiCG iterations, one per pattern of active
@@ -967,7 +968,7 @@ static int bicg2_dev(mstm_hip_ctx *c, int ns, int niter, const double *eps, cons
     while (iter <= niter) {
         if (!act[0] && !act[1]) break;
         const int nst = std::min(K, niter - iter + 1);
-        if (lazy && nst == K) rc = bicg_graph_steps(c, ns, prec, act, X[0], K, step);
+        if (lazy && nst == K && c->graphs) rc = bicg_graph_steps(c, ns, prec, act, X[0], K, step);
         else
             for (int q = 0; q < nst && !rc; q++) rc = step();
         if (rc) return rc;
@@ -2097,6 +2098,13 @@ int mstm_hip_set_inner_solver(mstm_hip_ctx *c, int kind)
     if (!c) return MSTM_HIP_EARG;
     if (kind != 0 && kind != 1) return fail(c, MSTM_HIP_EARG, "set_inner_solver: kind %d (0 BiCG, 1 GMRES)", kind);
     c->inner = kind;
+    return 0;
+}
+
+int mstm_hip_set_graphs(mstm_hip_ctx *c, int on)
+{
+    if (!c) return MSTM_HIP_EARG;
+    c->graphs = on != 0;
     return 0;
 }
 

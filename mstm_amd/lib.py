@@ -56,6 +56,7 @@ _SIGNATURES = {
     "mstm_hip_fosmexpansion": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp, _dp]),
     "mstm_hip_set_inner_solver": (C.c_int, [_vp, C.c_int]),
     "mstm_hip_set_refinement_cycles": (C.c_int, [_vp, C.c_int]),
+    "mstm_hip_set_graphs": (C.c_int, [_vp, C.c_int]),
     "mstm_hip_direct_solver": (C.c_int, [_vp, _dp, _dp, C.c_int, C.c_int, _dp]),
     "mstm_hip_tmatrix_solution": (C.c_int, [_vp, C.c_double, C.c_int, C.c_double, C.c_int, _dp, C.c_int64, _dp, _dp,
                                             _dp, _ip, _ip, _dp, _ip]),

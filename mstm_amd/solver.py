@@ -365,6 +365,11 @@ class HipContext:
         plain FP64 solver from b (0: at once)."""
         self._chk(self.L.mstm_hip_set_refinement_cycles(self.h, int(cycles)), "set_refinement_cycles")
 
+    def set_graphs(self, on=True):
+        """Replay small-cluster BiCG iterations from HIP graphs (default) or
+        issue the same launches one by one (same results)."""
+        self._chk(self.L.mstm_hip_set_graphs(self.h, int(bool(on))), "set_graphs")
+
     def clear(self, clear_tables=True):
         self._chk(self.L.mstm_hip_clear(self.h, int(bool(clear_tables))), "clear")
 

@@ -161,6 +161,16 @@ def test_graph_replays_match_oracle(gpu_ctx):
     for _ in range(2):
         xg, itg, _ = gpu_ctx.cbicg(200, 1e-8, b, b)
         assert itg == ito and rel(xg, xo) < FP64_TOL
+    # the same launches issued one by one give the same bits
+    rg = gpu_ctx.fixedorsoln(eps=1e-6, niter=10000)
+    gpu_ctx.set_graphs(False)
+    try:
+        re_ = gpu_ctx.fixedorsoln(eps=1e-6, niter=10000)
+        xe, ite, _ = gpu_ctx.cbicg(200, 1e-8, b, b)
+    finally:
+        gpu_ctx.set_graphs(True)
+    assert re_["iterations"] == rg["iterations"] and np.array_equal(re_["amnp"], rg["amnp"])
+    assert ite == itg and np.array_equal(xe, xg)
 
 
 def test_fixedorsoln_oblique_and_from_above_matches_oracle(gpu_ctx):
