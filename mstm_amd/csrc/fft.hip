@@ -240,54 +240,32 @@ struct FftPlan {
 };
 
 // one radix-R butterfly of a Stockham stage (R = 2..5), twiddled inputs;
-// R is a compile-time constant so the work arrays stay in registers
+// R is a compile-time constant so the work arrays stay in registers.
+// tw[j] = exp(sign 2 pi i j / n): the stage twiddles exp(sign 2 pi i kk r /
+// (Ns R)) are tw[kk r n / (Ns R)] (Ns R divides n) and the butterfly's own
+// roots exp(sign 2 pi i kr / R) are tw[kr n / R].
 template <typename T, int R>
-__device__ __forceinline__ void stage_radix(const cplx<T> *__restrict__ A, cplx<T> *__restrict__ B, int p, int jj,
-                                            int n, int nr, int Ns, int sign)
+__device__ __forceinline__ void stage_radix(const cplx<T> *__restrict__ A, cplx<T> *__restrict__ B,
+                                            const cplx<T> *__restrict__ tw, int p, int jj, int n, int nr, int Ns)
 {
     const int kk = jj % Ns;
+    const int step = n / (Ns * R);
     cplx<T> v[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
         cplx<T> x = A[p * n + jj + r * nr];
-        if (kk != 0 && r != 0) {
-            T sn, cs;
-            if (sizeof(T) == 8) {
-                double sd, cd;
-                sincospi(2.0 * sign * (double)(kk * r) / (double)(Ns * R), &sd, &cd);
-                sn = (T)sd;
-                cs = (T)cd;
-            } else {
-                float sf, cf;
-                sincospif(2.0f * sign * (float)(kk * r) / (float)(Ns * R), &sf, &cf);
-                sn = (T)sf;
-                cs = (T)cf;
-            }
-            x = cmul(x, cmk(cs, sn));
-        }
+        if (kk != 0 && r != 0) x = cmul(x, tw[kk * r * step]);
         v[r] = x;
     }
+    cplx<T> root[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) root[q] = tw[q * nr];
     const int idxD = (jj / Ns) * Ns * R + kk;
 #pragma unroll
     for (int k = 0; k < R; k++) {
         cplx<T> s = cmk((T)0, (T)0);
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            T sn, cs;
-            const int kr = (k * r) % R;
-            if (sizeof(T) == 8) {
-                double sd, cd;
-                sincospi(2.0 * sign * (double)kr / (double)R, &sd, &cd);
-                sn = (T)sd;
-                cs = (T)cd;
-            } else {
-                float sf, cf;
-                sincospif(2.0f * sign * (float)kr / (float)R, &sf, &cf);
-                sn = (T)sf;
-                cs = (T)cf;
-            }
-            cfma(s, cmk(cs, sn), v[r]);
-        }
+        for (int r = 0; r < R; r++) cfma(s, root[(k * r) % R], v[r]);
         B[p * n + idxD + k * Ns] = s;
     }
 }
@@ -300,6 +278,7 @@ __global__ void fft_pass_kernel(const cplx<T> *__restrict__ in, cplx<T> *__restr
     cplx<T> *A = reinterpret_cast<cplx<T> *>(smem);
     const int n = plan.n;
     cplx<T> *B = A + P * n;
+    cplx<T> *tw = B + P * n;  // n roots of unity, computed once per workgroup
     const int d[3] = {din.x, din.y, din.z};
     const int o[3] = {dout.x, dout.y, dout.z};
     // non-axis dimensions (fast, slow)
@@ -312,6 +291,21 @@ __global__ void fft_pass_kernel(const cplx<T> *__restrict__ in, cplx<T> *__restr
     const int64_t sin_[3] = {1, d[0], (int64_t)d[0] * d[1]};
     const int64_t sout[3] = {1, o[0], (int64_t)o[0] * o[1]};
     const int tid = threadIdx.x, bs = blockDim.x;
+    for (int j = tid; j < n; j += bs) {
+        T sn, cs;
+        if (sizeof(T) == 8) {
+            double sd, cd;
+            sincospi(2.0 * sign * (double)j / (double)n, &sd, &cd);
+            sn = (T)sd;
+            cs = (T)cd;
+        } else {
+            float sf, cf;
+            sincospif(2.0f * sign * (float)j / (float)n, &sf, &cf);
+            sn = (T)sf;
+            cs = (T)cf;
+        }
+        tw[j] = cmk(cs, sn);
+    }
     // load
     for (int idx = tid; idx < P * n; idx += bs) {
         int p, j;
@@ -339,10 +333,10 @@ __global__ void fft_pass_kernel(const cplx<T> *__restrict__ in, cplx<T> *__restr
         for (int w = tid; w < P * nr; w += bs) {
             const int p = w / nr, jj = w % nr;
             switch (R) {
-            case 2: stage_radix<T, 2>(A, B, p, jj, n, nr, Ns, sign); break;
-            case 3: stage_radix<T, 3>(A, B, p, jj, n, nr, Ns, sign); break;
-            case 4: stage_radix<T, 4>(A, B, p, jj, n, nr, Ns, sign); break;
-            default: stage_radix<T, 5>(A, B, p, jj, n, nr, Ns, sign); break;
+            case 2: stage_radix<T, 2>(A, B, tw, p, jj, n, nr, Ns); break;
+            case 3: stage_radix<T, 3>(A, B, tw, p, jj, n, nr, Ns); break;
+            case 4: stage_radix<T, 4>(A, B, tw, p, jj, n, nr, Ns); break;
+            default: stage_radix<T, 5>(A, B, tw, p, jj, n, nr, Ns); break;
             }
         }
         __syncthreads();
@@ -400,12 +394,16 @@ static hipError_t fft_axis(const cplx<T> *in, cplx<T> *out, int batch, int3 din,
 {
     FftPlan pl = make_plan(n);
     if (pl.n < 0) return hipErrorInvalidValue;
-    int P = std::max(1, std::min(64, 1984 / n));  // 2 LDS buffers <= 64 KB
+    // P pencils per workgroup: about two elements per thread (small
+    // workgroups, many of them: a pass is short and latency-bound), two
+    // buffers of P pencils and the n roots of unity within 64 KB of LDS
+    const int P = std::max(1, std::min(512 / n, (65536 / (int)sizeof(cplx<T>) / n - 1) / 2));
+    const size_t sh = (2 * (size_t)P + 1) * n * sizeof(cplx<T>);
+    if (sh > 65536) return hipErrorInvalidValue;  // an axis of more than 1365 (FP64) or 2730 (FP32) points
     const int d[3] = {din.x, din.y, din.z};
     int a0 = axis == 0 ? 1 : 0, a1 = axis == 2 ? 1 : 2;
     int npen = d[a0] * d[a1];
     dim3 g((npen + P - 1) / P, batch);
-    size_t sh = 2 * (size_t)P * n * sizeof(cplx<T>);
     hipLaunchKernelGGL(fft_pass_kernel<T>, g, dim3(256), sh, st, in, out, din, dout, axis, pl, nin, nout, sign, P);
     return hipGetLastError();
 }
