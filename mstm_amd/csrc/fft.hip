@@ -415,41 +415,61 @@ static hipError_t fft_axis(const cplx<T> *in, cplx<T> *out, int batch, int3 din,
 // data precision is T.  One thread per (frequency, mode, block of CONTRACT_LC l).
 // ===========================================================================
 #define CONTRACT_LC 4  // output degrees l per thread: each source value read serves LC of them
+// NL (<= CONTRACT_LC) output degrees of one thread: a compile-time count, so
+// that every load of a step has no guard and all of them (and those of the
+// next step) are in flight together
+template <typename T, int NL>
+__device__ __forceinline__ void contract_degrees(const cplx<float> *__restrict__ K, const cplx<T> *__restrict__ a,
+                                                 cplx<T> *__restrict__ g, int M, int nblk, int k, int p, int l0)
+{
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    cplx<T> acc[2][NL];
+#pragma unroll
+    for (int l = 0; l < NL; l++) acc[0][l] = acc[1][l] = cmk((T)0, (T)0);
+    const int64_t ls = (int64_t)nblk * M;  // K stride between degrees l
+    const cplx<float> *kp = K + ((int64_t)p * nblk + l0) * ls + k;
+    const cplx<T> *ap0 = a + (int64_t)p * nblk * M + k, *ap1 = a + (int64_t)(2 + p) * nblk * M + k;
+#pragma unroll 2
+    for (int n = 0; n < nblk; n++) {
+        const cplx<T> a0 = ap0[(int64_t)n * M], a1 = ap1[(int64_t)n * M];
+#pragma unroll
+        for (int l = 0; l < NL; l++) {
+            const f2v kv = __builtin_nontemporal_load(reinterpret_cast<const f2v *>(kp + l * ls + (int64_t)n * M));
+            const cplx<T> kk = cmk((T)kv[0], (T)kv[1]);
+            cfma(acc[0][l], kk, a0);
+            cfma(acc[1][l], kk, a1);
+        }
+    }
+#pragma unroll
+    for (int l = 0; l < NL; l++) {
+        g[((int64_t)(0 * 2 + p) * nblk + l0 + l) * M + k] = acc[0][l];
+        g[((int64_t)(1 * 2 + p) * nblk + l0 + l) * M + k] = acc[1][l];
+    }
+}
+
 template <typename T>
 __global__ void contract_kernel(const cplx<float> *__restrict__ K, const cplx<T> *__restrict__ a,
                                 cplx<T> *__restrict__ g, int M, int nblk)
 {
     constexpr int LC = CONTRACT_LC;
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int p = blockIdx.y;
-    int l0 = blockIdx.z * LC;
+    static_assert(LC == 4, "the degree-count switch below covers 1..4");
+    // workgroup -> (frequency block x, mode p, degree block z): the Z degree
+    // blocks of one (x, p) read the same source fields, so they are dealt to
+    // one XCD (workgroups go to MI355X's 8 XCDs round robin) one after another
+    // and share its L2; the grid holds 8 * ceil(2 X / 8) * Z workgroups
+    const int X = (M + 127) / 128, Z = (nblk + LC - 1) / LC;
+    const int wg = blockIdx.x, slot = wg >> 3;
+    const int c = (wg & 7) + 8 * (slot / Z);  // (x, p) combination
+    if (c >= 2 * X) return;
+    const int k = (c >> 1) * blockDim.x + threadIdx.x;
+    const int p = c & 1;
+    const int l0 = (slot % Z) * LC;
     if (k >= M) return;
-    cplx<T> acc[2][LC];
-    for (int r = 0; r < 2; r++)
-        for (int l = 0; l < LC; l++) acc[r][l] = cmk((T)0, (T)0);
-    const int lend = imn(LC, nblk - l0);
-    for (int n = 0; n < nblk; n++) {
-        cplx<T> a0 = a[((int64_t)(0 * 2 + p) * nblk + n) * M + k];
-        cplx<T> a1 = a[((int64_t)(1 * 2 + p) * nblk + n) * M + k];
-#pragma unroll
-        for (int l = 0; l < LC; l++) {
-            if (l < lend) {
-                typedef float f2v __attribute__((ext_vector_type(2)));
-                f2v kv = __builtin_nontemporal_load(
-                    reinterpret_cast<const f2v *>(&K[(((int64_t)p * nblk + l0 + l) * nblk + n) * M + k]));
-                cplx<T> kk = cmk((T)kv[0], (T)kv[1]);
-                cfma(acc[0][l], kk, a0);
-                cfma(acc[1][l], kk, a1);
-            }
-        }
-    }
-    // unrolled with a guard so that acc stays in registers
-#pragma unroll
-    for (int l = 0; l < LC; l++) {
-        if (l < lend) {
-            g[((int64_t)(0 * 2 + p) * nblk + l0 + l) * M + k] = acc[0][l];
-            g[((int64_t)(1 * 2 + p) * nblk + l0 + l) * M + k] = acc[1][l];
-        }
+    switch (imn(LC, nblk - l0)) {  // uniform over the workgroup
+    case 4: contract_degrees<T, 4>(K, a, g, M, nblk, k, p, l0); break;
+    case 3: contract_degrees<T, 3>(K, a, g, M, nblk, k, p, l0); break;
+    case 2: contract_degrees<T, 2>(K, a, g, M, nblk, k, p, l0); break;
+    default: contract_degrees<T, 1>(K, a, g, M, nblk, k, p, l0); break;
     }
 }
 
@@ -1156,7 +1176,8 @@ template <typename T> static int matvec_t(FftOp *f, int nrhs, const double *ain,
     FCHK(fft_axis<T>(A, B, batch, d1, d2, 1, 2 * N.y, N.y, 2 * N.y, -1, st));
     FCHK(fft_axis<T>(B, A, batch, d2, d3, 0, 2 * N.x, N.x, 2 * N.x, -1, st));
     // per-frequency contraction with the node kernel spectrum
-    dim3 cg_((f->M + 127) / 128, 2, (nblk + CONTRACT_LC - 1) / CONTRACT_LC);
+    const int cX = (f->M + 127) / 128, cZ = (nblk + CONTRACT_LC - 1) / CONTRACT_LC;
+    dim3 cg_((unsigned)((2 * cX + 7) / 8 * 8 * cZ));
     if (f->cev_used + 2 > 512) collect_contract(f);
     while (f->cev.size() < f->cev_used + 2) {
         hipEvent_t e;
