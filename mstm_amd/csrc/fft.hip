@@ -34,7 +34,7 @@
 // opposite phase).  Source element (rhs r, mode p, coefficient e) is read at
 // src + src_off[q] + r*rs + p*ms + e*es; the result for slot q is written
 // contiguously to dst[q][r][p][NB(target)].
-// One 128-thread workgroup per slot serves its four columns (r, p); each
+// One workgroup per slot (128 or 256 threads) serves its four columns (r, p); each
 // stage is spread over the threads and kept in LDS (dynamic size, from the
 // orders):
 //   xp(n,k)  = e^{i k phi} a(k,n)                          n <= Ls
@@ -42,10 +42,9 @@
 //   z_m(t)   = sum_s A_m(t,s) y_m(s)                        (axial)
 //   W(t,k)   = e^{-i k phi} sum_m d^t_{m,k} z_m(t)          (back rotation)
 // ===========================================================================
-#define LT_THREADS 128
-
-template <typename T, int DIR>
-__global__ __launch_bounds__(LT_THREADS) void list_translate_kernel(const T *__restrict__ tab, int nslots, int LT,
+// NT threads per workgroup (128 or 256, by the size of the stages)
+template <typename T, int DIR, int NT>
+__global__ __launch_bounds__(NT) void list_translate_kernel(const T *__restrict__ tab, int nslots, int LT,
                                                                     int LS, const cplx<T> *__restrict__ src,
                                                                     const int64_t *__restrict__ src_off, int64_t rs,
                                                                     int64_t ms, int64_t es,
@@ -66,9 +65,9 @@ __global__ __launch_bounds__(LT_THREADS) void list_translate_kernel(const T *__r
         return xs[(c >> 1) * rs + (c & 1) * ms + (int64_t)amn(k, n, Ls) * es];
     };
     if (ident[q]) {  // zero translation of regular waves: truncated copy (mtransfer)
-        for (int e = tid; e < 4 * NBT; e += LT_THREADS) {
+        for (int e = tid; e < 4 * NBT; e += NT) {
             const int c = e / NBT, w = e - c * NBT;
-            int t = (int)sqrt((double)(w + 1));
+            int t = (int)sqrtf((float)(w + 1));
             while (t * (t + 2) < w + 1) t++;
             while ((t - 1) * (t + 1) >= w + 1) t--;
             const int k = w + 1 - t * (t + 1);
@@ -106,21 +105,21 @@ __global__ __launch_bounds__(LT_THREADS) void list_translate_kernel(const T *__r
         typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type V;
         const V *gsrc = reinterpret_cast<const V *>(p0);
         V *d = reinterpret_cast<V *>(tl);
-        for (int c = tid; c < SL / VW; c += LT_THREADS) d[c] = gsrc[(int64_t)c * MSTM_SLOTS];
+        for (int c = tid; c < SL / VW; c += NT) d[c] = gsrc[(int64_t)c * MSTM_SLOTS];
     }
     __syncthreads();
     auto E = [&](int e) -> T { return tl[e]; };
     const cplx<T> e1 = cmk(E(boff[NM]), DIR ? -E(boff[NM] + 1) : E(boff[NM] + 1));
-    for (int k = tid - nmax; k <= nmax; k += LT_THREADS) {
+    for (int k = tid - nmax; k <= nmax; k += NT) {
         cplx<T> r = cmk((T)1, (T)0), b = k < 0 ? cconj(e1) : e1;
         for (int u = 0; u < iab(k); u++) r = cmul(r, b);
         ph[k + nmax] = r;
     }
     __syncthreads();
     // phase-shifted source, m-flipped for the reverse direction
-    for (int e = tid; e < 4 * NBS; e += LT_THREADS) {
+    for (int e = tid; e < 4 * NBS; e += NT) {
         const int c = e / NBS, w = e - c * NBS;
-        int n = (int)sqrt((double)(w + 1));
+        int n = (int)sqrtf((float)(w + 1));
         while (n * (n + 2) < w + 1) n++;
         while ((n - 1) * (n + 1) >= w + 1) n--;
         const int k = w + 1 - n * (n + 1);
@@ -136,7 +135,7 @@ __global__ __launch_bounds__(LT_THREADS) void list_translate_kernel(const T *__r
     // rotation rows of block m start at sec_off(b); degree n at + sum_{q=m1}^{n-1} (2q+1) = n^2 - m1^2
     auto bofm = [](int m) { return m == 0 ? 0 : (m < 0 ? 2 * (-m) - 1 : 2 * m); };
     // forward rotation y_m(n), n = m1..Ls
-    for (int e = tid; e < 4 * NM * Ls; e += LT_THREADS) {
+    for (int e = tid; e < 4 * NM * Ls; e += NT) {
         const int c = e / (NM * Ls), rem = e - c * NM * Ls, mi = rem / Ls, n = rem - mi * Ls + 1;
         const int m = mi - nmin, m1 = imx(1, iab(m));
         if (n < m1) continue;
@@ -147,7 +146,7 @@ __global__ __launch_bounds__(LT_THREADS) void list_translate_kernel(const T *__r
     }
     __syncthreads();
     // axial z_m(t), t = m1..Lt (mode-specific matrix, transposed for DIR = 1)
-    for (int e = tid; e < 4 * NM * Lt; e += LT_THREADS) {
+    for (int e = tid; e < 4 * NM * Lt; e += NT) {
         const int c = e / (NM * Lt), rem = e - c * NM * Lt, mi = rem / Lt, t = rem - mi * Lt + 1;
         const int m = mi - nmin, m1 = imx(1, iab(m)), p = c & 1;
         if (t < m1) continue;
@@ -162,9 +161,9 @@ __global__ __launch_bounds__(LT_THREADS) void list_translate_kernel(const T *__r
     }
     __syncthreads();
     // back rotation with the output phase; m-flip back for the reverse direction
-    for (int e = tid; e < 4 * NBT; e += LT_THREADS) {
+    for (int e = tid; e < 4 * NBT; e += NT) {
         const int c = e / NBT, w = e - c * NBT;
-        int t = (int)sqrt((double)(w + 1));
+        int t = (int)sqrtf((float)(w + 1));
         while (t * (t + 2) < w + 1) t++;
         while ((t - 1) * (t + 1) >= w + 1) t--;
         const int k = w + 1 - t * (t + 1);
@@ -180,6 +179,27 @@ __global__ __launch_bounds__(LT_THREADS) void list_translate_kernel(const T *__r
         if (DIR && (k & 1)) v = cmk(-v.re, -v.im);
         out[(int64_t)c * NBT + amn(k, t, Lt)] = v;
     }
+}
+
+// threads of a translation's workgroup: 256 when its largest stage (the
+// rotations, 4 columns x (2 min(LT, LS) + 1) blocks x max(LT, LS) degrees)
+// has more than 256 items, else 128 (small orders leave the rest idle)
+static int list_translate_threads(int LT, int LS)
+{
+    return 4 * (2 * imn(LT, LS) + 1) * imx(LT, LS) > 256 ? 256 : 128;
+}
+template <typename T, int DIR>
+static hipError_t launch_list_translate(dim3 g, size_t lds, hipStream_t st, const T *tab, int nslots, int LT, int LS,
+                                        const cplx<T> *src, const int64_t *off, int64_t rs, int64_t ms, int64_t es,
+                                        const unsigned char *ident, cplx<T> *dst)
+{
+    if (list_translate_threads(LT, LS) == 256)
+        hipLaunchKernelGGL((list_translate_kernel<T, DIR, 256>), g, dim3(256), lds, st, tab, nslots, LT, LS, src, off,
+                           rs, ms, es, ident, dst);
+    else
+        hipLaunchKernelGGL((list_translate_kernel<T, DIR, 128>), g, dim3(128), lds, st, tab, nslots, LT, LS, src, off,
+                           rs, ms, es, ident, dst);
+    return hipGetLastError();
 }
 
 template <typename T> static size_t list_translate_lds(int LT, int LS)
@@ -1159,11 +1179,10 @@ template <typename T> static int matvec_t(FftOp *f, int nrhs, const double *ain,
                         f->g.d_an, st));
     const int64_t rs_sph = (int64_t)f->npad * 2 * NBP;
     // sphere -> node (J translation, :573-626)
-    dim3 lg(n), lb(LT_THREADS);
+    dim3 lg(n);
     const size_t ljb = list_translate_lds<T>(LG, Lp), lhb = list_translate_lds<T>(Lp, Lp);
-    hipLaunchKernelGGL((list_translate_kernel<T, 0>), lg, lb, ljb, st, jt, n, LG, Lp, xin, f->d_off_sph, rs_sph,
-                       (int64_t)NBP, (int64_t)1, f->d_ident_j, asph);
-    FCHK(hipGetLastError());
+    FCHK((launch_list_translate<T, 0>)(lg, ljb, st, jt, n, LG, Lp, xin, f->d_off_sph, rs_sph, (int64_t)NBP, (int64_t)1,
+                                     f->d_ident_j, asph));
     int64_t nn4 = (int64_t)4 * nblk * f->ncells;
     hipLaunchKernelGGL(node_reduce_kernel<T>, dim3((unsigned)((nn4 + 255) / 256)), dim3(256), 0, st, asph, f->d_cstart,
                        f->d_cmem, f->ncells, nblk, B);
@@ -1197,18 +1216,15 @@ template <typename T> static int matvec_t(FftOp *f, int nrhs, const double *ain,
     FCHK(fft_axis<T>(B, A, batch, d1, d0, 2, 2 * N.z, 2 * N.z, N.z, 1, st));
     // node -> sphere (reverse J translation) from gnode[r][p][l][cell] in A
     const int64_t nc = f->ncells;
-    hipLaunchKernelGGL((list_translate_kernel<T, 1>), lg, lb, ljb, st, jt, n, LG, Lp, A, f->d_off_node, 2 * nblk * nc,
-                       nblk * nc, nc, f->d_ident_j, far);
-    FCHK(hipGetLastError());
+    FCHK((launch_list_translate<T, 1>)(lg, ljb, st, jt, n, LG, Lp, A, f->d_off_node, 2 * nblk * nc, nblk * nc, nc,
+                                     f->d_ident_j, far));
     // near-field pairs, both directions (local_sphere_to_sphere_expansion)
     if (f->npairs) {
         dim3 pg(f->npairs);
-        hipLaunchKernelGGL((list_translate_kernel<T, 0>), pg, lb, lhb, st, ht, f->npairs, Lp, Lp, xin, f->d_off_pj,
-                           rs_sph, (int64_t)NBP, (int64_t)1, f->d_ident_h, nf);
-        FCHK(hipGetLastError());
-        hipLaunchKernelGGL((list_translate_kernel<T, 1>), pg, lb, lhb, st, ht, f->npairs, Lp, Lp, xin, f->d_off_pi,
-                           rs_sph, (int64_t)NBP, (int64_t)1, f->d_ident_h, nr);
-        FCHK(hipGetLastError());
+        FCHK((launch_list_translate<T, 0>)(pg, lhb, st, ht, f->npairs, Lp, Lp, xin, f->d_off_pj, rs_sph, (int64_t)NBP,
+                                         (int64_t)1, f->d_ident_h, nf));
+        FCHK((launch_list_translate<T, 1>)(pg, lhb, st, ht, f->npairs, Lp, Lp, xin, f->d_off_pi, rs_sph, (int64_t)NBP,
+                                         (int64_t)1, f->d_ident_h, nr));
     }
     int64_t ns4 = (int64_t)n * 4 * NBP;
     hipLaunchKernelGGL(sphere_reduce_kernel<T>, dim3((unsigned)((ns4 + 255) / 256)), dim3(256), 0, st, far, nf, nr,
