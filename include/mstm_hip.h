@@ -32,6 +32,7 @@ extern "C" {
 
 #define MSTM_HIP_FP64 0 /* plain: FP64 tables and arithmetic               */
 #define MSTM_HIP_FP32 1 /* mixed: FP32 tables and arithmetic               */
+#define MSTM_HIP_FP64_FP32 2 /* mstm_hip_setup_direct only: both tables      */
 
 #define MSTM_HIP_SOLVE_PLAIN 0   /* cbicg in FP64, iteration-identical         */
 #define MSTM_HIP_SOLVE_MIXED 1   /* FP32 BiCG inside FP64 iterative refinement */
@@ -72,7 +73,9 @@ int mstm_hip_get_mie(const mstm_hip_ctx *ctx, double *an_mie);
 /* Build the pair translation tables of the direct operator on the device
  * (the stored form of external_to_external_expansion,
  * mstm-modules-33.f90:6437-6462 with store_translation_matrix).  precision:
- * MSTM_HIP_FP64 or MSTM_HIP_FP32; both may coexist. */
+ * MSTM_HIP_FP64 or MSTM_HIP_FP32; both may coexist.  MSTM_HIP_FP64_FP32
+ * builds both, in one pass where the orders allow (each pair's FP64 values
+ * worked out once, stored at both precisions; the same tables as two calls). */
 int mstm_hip_setup_direct(mstm_hip_ctx *ctx, int precision);
 
 /* Node grid of the FFT operator: node_selection(fva, target_min, target_max,

@@ -584,14 +584,10 @@ int mstm_hip_get_mie(const mstm_hip_ctx *cc, double *an_mie)
     return 0;
 }
 
-int mstm_hip_setup_direct(mstm_hip_ctx *c, int prec)
+// the tables, padded input and partial sums of one precision's operator
+static int direct_alloc(mstm_hip_ctx *c, int prec)
 {
-    if (!c || !c->nsph) return fail(c, MSTM_HIP_ESTATE, "setup_direct: no spheres");
-    if (prec != MSTM_HIP_FP64 && prec != MSTM_HIP_FP32) return fail(c, MSTM_HIP_EARG, "setup_direct: precision");
-    HIPCHK(hipSetDevice(c->device));
     release_direct(c, prec);
-    int rc = ensure_vcc(c, c->Lmax);
-    if (rc) return rc;
     DirectOp &d = c->dop[prec];
     const int L = c->Lmax;
     const size_t esz = prec == MSTM_HIP_FP64 ? sizeof(double) : sizeof(float);
@@ -601,17 +597,44 @@ int mstm_hip_setup_direct(mstm_hip_ctx *c, int prec)
     HIPCHK(dalloc(&d.tab, d.tab_bytes));
     HIPCHK(dalloc(&d.xin, esz * 2 * 4 * 2 * (size_t)c->npad * nbp));
     HIPCHK(dalloc(&d.part, esz * 2 * (size_t)c->nblocks * c->tgroups * 4 * 2 * c->npad * nbp));
+    return 0;
+}
+
+int mstm_hip_setup_direct(mstm_hip_ctx *c, int prec)
+{
+    if (!c || !c->nsph) return fail(c, MSTM_HIP_ESTATE, "setup_direct: no spheres");
+    if (prec != MSTM_HIP_FP64 && prec != MSTM_HIP_FP32 && prec != MSTM_HIP_FP64_FP32)
+        return fail(c, MSTM_HIP_EARG, "setup_direct: precision");
+    HIPCHK(hipSetDevice(c->device));
+    const int L = c->Lmax;
+    if (prec == MSTM_HIP_FP64_FP32 && !direct_tables_both_ok(L)) {
+        int rc = mstm_hip_setup_direct(c, MSTM_HIP_FP64);
+        return rc ? rc : mstm_hip_setup_direct(c, MSTM_HIP_FP32);
+    }
+    int rc = ensure_vcc(c, L);
+    if (rc) return rc;
+    const int p0 = prec == MSTM_HIP_FP32 ? MSTM_HIP_FP32 : MSTM_HIP_FP64;
+    const int p1 = prec == MSTM_HIP_FP64 ? MSTM_HIP_FP64 : MSTM_HIP_FP32;
+    for (int p = p0; p <= p1; p++)
+        if ((rc = direct_alloc(c, p))) return rc;
     auto t0 = std::chrono::steady_clock::now();
-    if (prec == MSTM_HIP_FP64)
-        HIPCHK(launch_direct_tables<double>((double *)d.tab, c->d_tiles, c->ntiles, L, c->nsph, c->d_pos, c->rimed,
-                                            c->d_vcc, c->vcc_LM, c->st));
+    if (prec == MSTM_HIP_FP64_FP32)
+        HIPCHK(launch_direct_tables_both((double *)c->dop[0].tab, (float *)c->dop[1].tab, c->d_tiles, c->ntiles, L,
+                                         c->nsph, c->d_pos, c->rimed, c->d_vcc, c->vcc_LM, c->st));
+    else if (prec == MSTM_HIP_FP64)
+        HIPCHK(launch_direct_tables<double>((double *)c->dop[0].tab, c->d_tiles, c->ntiles, L, c->nsph, c->d_pos,
+                                            c->rimed, c->d_vcc, c->vcc_LM, c->st));
     else
-        HIPCHK(launch_direct_tables<float>((float *)d.tab, c->d_tiles, c->ntiles, L, c->nsph, c->d_pos, c->rimed,
-                                           c->d_vcc, c->vcc_LM, c->st));
+        HIPCHK(launch_direct_tables<float>((float *)c->dop[1].tab, c->d_tiles, c->ntiles, L, c->nsph, c->d_pos,
+                                           c->rimed, c->d_vcc, c->vcc_LM, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
-    c->stats.tables_ms[prec] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    c->stats.table_bytes[prec] = (double)d.tab_bytes;
-    d.ready = true;
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (int p = p0; p <= p1; p++) {
+        // one pass for both: its time is booked to the FP64 tables
+        c->stats.tables_ms[p] = p == p0 ? ms : 0.0;
+        c->stats.table_bytes[p] = (double)c->dop[p].tab_bytes;
+        c->dop[p].ready = true;
+    }
     return 0;
 }
 

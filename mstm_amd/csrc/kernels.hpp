@@ -14,6 +14,11 @@ hipError_t launch_direct_tables(T *tab, const int2 *tiles, int ntiles, int L, in
 template <typename T>
 hipError_t launch_list_tables(T *tab, const double *rtran, int nslots, int LT, int LS, int itype, double rimed,
                               const double *vcc, int LM, hipStream_t st);
+// the FP64 and FP32 direct tables in one pass: orders whose layout kind is
+// the same at both precisions (direct_tables_both_ok)
+bool direct_tables_both_ok(int L);
+hipError_t launch_direct_tables_both(double *tab64, float *tab32, const int2 *tiles, int ntiles, int L, int nsph,
+                                     const double *pos, double rimed, const double *vcc, int LM, hipStream_t st);
 
 // direct.hip
 // gate (optional, device int): the kernels of a product return at once

@@ -9,13 +9,36 @@
 #include "specfun.hpp"
 #include "kernels.hpp"
 
+// Where build_slot's values go: one table of store type T, or the FP64 and
+// FP32 tables of the same slot at once (the same FP64 values, rounded for
+// the second).  off(VW) is an element's offset in a layout of VW elements
+// per 16 bytes; the slot is interleaved at k.
+template <typename T> struct OneTable {
+    T *__restrict__ grp;
+    int k;
+    template <typename F> __device__ __forceinline__ void put(F off, double v) const
+    {
+        constexpr int VW = 16 / (int)sizeof(T);
+        grp[ilv(off(VW), k, VW)] = (T)v;
+    }
+};
+struct BothTables {
+    double *__restrict__ g64;
+    float *__restrict__ g32;
+    int k;
+    template <typename F> __device__ __forceinline__ void put(F off, double v) const
+    {
+        g64[ilv(off(2), k, 2)] = v;
+        g32[ilv(off(4), k, 4)] = (float)v;
+    }
+};
+
 // Table of one directed translation (target order LT <- source order LS)
-// along rtran = r_target - r_source, written interleaved for slot k.
-template <typename T>
-__device__ void build_slot(T *__restrict__ grp, int k, int LT, int LS, int itype, const double *rt, double rimed,
+// along rtran = r_target - r_source, written interleaved for its slot.
+template <typename Sink>
+__device__ void build_slot(const Sink &out, int LT, int LS, int itype, const double *rt, double rimed,
                            const double *__restrict__ vcc, int LM, bool sym = false)
 {
-    const int VW = 16 / (int)sizeof(T);
     const int nmax = imx(LT, LS), nmin = imn(LT, LS);
     double r = sqrt(rt[0] * rt[0] + rt[1] * rt[1] + rt[2] * rt[2]);
     double ct = rt[2] / r;
@@ -57,21 +80,21 @@ __device__ void build_slot(T *__restrict__ grp, int k, int LT, int LS, int itype
         auto put_row = [&](int m, const double *row, double sgnm) {
             int b = (m == 0) ? 0 : (m < 0 ? 2 * (-m) - 1 : 2 * m);
             int m1 = imx(1, iab(m));
-            int e0 = sec_off(b, LT, LS, VW);
-            for (int nn = m1; nn < n; nn++) e0 += 2 * nn + 1;
+            int r0 = 0;
+            for (int nn = m1; nn < n; nn++) r0 += 2 * nn + 1;
             for (int kk = -n; kk <= n; kk++) {
                 double v = (sgnm == 0.0) ? row[kk + n] : row[-kk + n] * sgnm * ((kk & 1) ? -1.0 : 1.0);
-                grp[ilv(e0 + kk + n, k, VW)] = (T)v;
+                out.put([&](int VW) { return sec_off(b, LT, LS, VW) + r0 + kk + n; }, v);
             }
         };
         // symmetric layout (LT = LS): half-sum / half-difference rows of block m >= 0
         auto put_sym = [&](int m, const double *row) {
-            int e0 = srow_off(m, n);
+            const int e0 = srow_off(m, n);
             for (int kk = 0; kk <= n; kk++) {
                 double sg = (kk & 1) ? -1.0 : 1.0;
                 double e = (kk == 0) ? row[n] : 0.5 * (row[kk + n] + sg * row[-kk + n]);
-                grp[ilv(e0 + kk, k, VW)] = (T)e;
-                if (m > 0 && kk > 0) grp[ilv(e0 + n + kk, k, VW)] = (T)(0.5 * (row[kk + n] - sg * row[-kk + n]));
+                out.put([&](int) { return e0 + kk; }, e);
+                if (m > 0 && kk > 0) out.put([&](int) { return e0 + n + kk; }, 0.5 * (row[kk + n] - sg * row[-kk + n]));
             }
         };
         if (sym) put_sym(0, rowp);
@@ -149,20 +172,23 @@ __device__ void build_slot(T *__restrict__ grp, int k, int LT, int LS, int itype
                 }
             }
         }
-        int e0 = sym ? sax_off(m, LT, VW) : sec_off(b, LT, LS, VW) + pad_to(rot_len(m, LT, LS), VW);
-        int ap = sym ? pad_to(sax_len(m, LT), VW) : pad_to(ax_len(m, LT, LS), VW);
+        // block start and mode stride in a layout of VW elements per 16 bytes
+        auto ax0 = [&](int VW) {
+            return sym ? sax_off(m, LT, VW) : sec_off(b, LT, LS, VW) + pad_to(rot_len(m, LT, LS), VW);
+        };
+        auto axp = [&](int VW) { return sym ? pad_to(sax_len(m, LT), VW) : pad_to(ax_len(m, LT, LS), VW); };
         for (int p = 0; p < (sym ? sax_modes(m) : 2); p++)
             for (int l = m1; l <= LS; l++)
                 for (int n = m1; n <= LT; n++) {
-                    int e = e0 + p * ap + 2 * ((n - m1) + (LT - m1 + 1) * (l - m1));
-                    grp[ilv(e, k, VW)] = (T)ACT(n, l)[p].re;
-                    grp[ilv(e + 1, k, VW)] = (T)ACT(n, l)[p].im;
+                    const int e = 2 * ((n - m1) + (LT - m1 + 1) * (l - m1));
+                    out.put([&](int VW) { return ax0(VW) + p * axp(VW) + e; }, ACT(n, l)[p].re);
+                    out.put([&](int VW) { return ax0(VW) + p * axp(VW) + e + 1; }, ACT(n, l)[p].im);
                 }
     }
 #undef ACT
-    int ep = sym ? sphi_off(LT, VW) : phi_off(LT, LS, VW);
-    grp[ilv(ep, k, VW)] = (T)cph;
-    grp[ilv(ep + 1, k, VW)] = (T)sph;
+    auto ep = [&](int VW) { return sym ? sphi_off(LT, VW) : phi_off(LT, LS, VW); };
+    out.put(ep, cph);
+    out.put([&](int VW) { return ep(VW) + 1; }, sph);
 }
 
 // Direct-path tables: slot q = (tile*S + step)*S + k (S = MSTM_SLOTS) holds
@@ -189,7 +215,35 @@ __global__ void direct_tables_kernel(T *__restrict__ tab, const int2 *__restrict
         return;
     }
     double rt[3] = {pos[3 * it] - pos[3 * is], pos[3 * it + 1] - pos[3 * is + 1], pos[3 * it + 2] - pos[3 * is + 2]};
-    build_slot<T>(grp, k, L, L, 3, rt, rimed, vcc, LM, direct_sym(L, (int)sizeof(T)));
+    build_slot(OneTable<T>{grp, k}, L, L, 3, rt, rimed, vcc, LM, direct_sym(L, (int)sizeof(T)));
+}
+
+// Both direct tables, FP64 and FP32, in one pass (orders with the same
+// layout kind at both precisions: symmetric up to 4, full from 6): each
+// slot's FP64 values are worked out once and stored twice.
+__global__ void direct_tables_both_kernel(double *__restrict__ tab64, float *__restrict__ tab32,
+                                          const int2 *__restrict__ tiles, int ntiles, int L, int nsph,
+                                          const double *__restrict__ pos, double rimed,
+                                          const double *__restrict__ vcc, int LM, int64_t se64, int64_t se32)
+{
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)ntiles * MSTM_SLOTS * MSTM_SLOTS) return;
+    int k = (int)(q % MSTM_SLOTS);
+    int64_t g = q / MSTM_SLOTS;
+    int step = (int)(g % MSTM_SLOTS);
+    int tile = (int)(g / MSTM_SLOTS);
+    int2 bt = tiles[tile];
+    int it = bt.x * MSTM_SLOTS + k;
+    int is = bt.y * MSTM_SLOTS + (k + step) % MSTM_SLOTS;
+    double *g64 = tab64 + g * se64 * MSTM_SLOTS;
+    float *g32 = tab32 + g * se32 * MSTM_SLOTS;
+    if (it >= nsph || is >= nsph || it == is) {
+        for (int e = 0; e < se64; e++) g64[ilv(e, k, 2)] = 0.0;
+        for (int e = 0; e < se32; e++) g32[ilv(e, k, 4)] = 0.0f;
+        return;
+    }
+    double rt[3] = {pos[3 * it] - pos[3 * is], pos[3 * it + 1] - pos[3 * is + 1], pos[3 * it + 2] - pos[3 * is + 2]};
+    build_slot(BothTables{g64, g32, k}, L, L, 3, rt, rimed, vcc, LM, direct_sym(L, 8));
 }
 
 template <typename T>
@@ -210,6 +264,20 @@ template hipError_t launch_direct_tables<float>(float *, const int2 *, int, int,
 template hipError_t launch_direct_tables<double>(double *, const int2 *, int, int, int, const double *, double,
                                                  const double *, int, hipStream_t);
 
+bool direct_tables_both_ok(int L) { return L >= 1 && L <= MSTM_TAB_L && direct_sym(L, 8) == direct_sym(L, 4); }
+
+hipError_t launch_direct_tables_both(double *tab64, float *tab32, const int2 *tiles, int ntiles, int L, int nsph,
+                                     const double *pos, double rimed, const double *vcc, int LM, hipStream_t st)
+{
+    if (!direct_tables_both_ok(L)) return hipErrorInvalidValue;
+    const int64_t nq = (int64_t)ntiles * MSTM_SLOTS * MSTM_SLOTS;
+    const int bs = 128;
+    hipLaunchKernelGGL(direct_tables_both_kernel, dim3((unsigned)((nq + bs - 1) / bs)), dim3(bs), 0, st, tab64, tab32,
+                       tiles, ntiles, L, nsph, pos, rimed, vcc, LM, (int64_t)direct_slot_len(L, 8),
+                       (int64_t)direct_slot_len(L, 4));
+    return hipGetLastError();
+}
+
 // General slot list (FFT path: sphere<->node J tables and near-field H
 // tables).  For slot q: target/source orders from (LT,LS), rtran given.
 template <typename T>
@@ -228,7 +296,7 @@ __global__ void list_tables_kernel(T *__restrict__ tab, const double *__restrict
         for (int e = 0; e < slot_elems; e++) grp[ilv(e, k, VW)] = (T)0;
         return;
     }
-    build_slot<T>(grp, k, LT, LS, itype, rt, rimed, vcc, LM);
+    build_slot(OneTable<T>{grp, k}, LT, LS, itype, rt, rimed, vcc, LM);
 }
 
 template <typename T>

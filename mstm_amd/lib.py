@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libmstm_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "mstm_hip.h")
 
 FP64, FP32 = 0, 1
+FP64_FP32 = 2  # setup_direct: both tables in one pass
 SOLVE_PLAIN, SOLVE_MIXED, SOLVE_LEARNED, SOLVE_FGMRES, SOLVE_LU = 0, 1, 2, 3, 4
 
 

@@ -11,7 +11,7 @@ import ctypes as C
 
 import numpy as np
 
-from .lib import FP32, FP64, SOLVE_FGMRES, SOLVE_LEARNED, SOLVE_LU, SOLVE_MIXED, SOLVE_PLAIN, MstmHipError, load_library  # noqa: F401
+from .lib import FP32, FP64, FP64_FP32, SOLVE_FGMRES, SOLVE_LEARNED, SOLVE_LU, SOLVE_MIXED, SOLVE_PLAIN, MstmHipError, load_library  # noqa: F401
 
 
 def _d(a):

@@ -360,3 +360,29 @@ def test_full_size_product_matches_oracle_on_a_subset(gpu_ctx):
     lhs = np.vdot(b[0], ha[0])
     rhs = np.vdot(hhb[0], a[0])
     assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+@pytest.mark.parametrize("x", [1.0, 2.5, 4.0])
+def test_one_pass_tables_equal_separate_builds(gpu_ctx, x):
+    """setup_direct(FP64_FP32) builds the FP64 and FP32 tables in one pass
+    (orders up to 4 and from 6; order 5, symmetric only in FP32, takes two
+    passes): the products of both precisions are bit for bit those of the
+    tables built one precision at a time."""
+    from mstm_amd import geometry as G
+    from mstm_amd.solver import FP32, FP64, FP64_FP32
+
+    c = G.rsa_in_sphere(40, x, 0.2, seed=70, ri=1.4 + 0.01j)
+    gpu_ctx.set_spheres(c.pos, c.radius, c.ri, 1.0, 1e-6)
+    ne = gpu_ctx.number_eqns
+    rng = np.random.default_rng(3)
+    a = rng.normal(size=(4, ne)) + 1j * rng.normal(size=(4, ne))
+    ct, mm = [False, True, True, False], [True] * 4
+    gpu_ctx.setup_direct(FP64)
+    gpu_ctx.setup_direct(FP32)
+    ref = [gpu_ctx.sphereinteraction(a, ct, mm, precision=p) for p in (FP64, FP32)]
+    gpu_ctx.setup_direct(FP64_FP32)
+    out = [gpu_ctx.sphereinteraction(a, ct, mm, precision=p) for p in (FP64, FP32)]
+    for r, o in zip(ref, out):
+        assert np.array_equal(r, o)
+    st = gpu_ctx.stats()
+    assert st["table_bytes_fp64"] > 0 and st["table_bytes_fp32"] > 0
