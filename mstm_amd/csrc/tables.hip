@@ -35,9 +35,12 @@ struct BothTables {
 
 // Table of one directed translation (target order LT <- source order LS)
 // along rtran = r_target - r_source, written interleaved for its slot.
-template <typename Sink>
-__device__ void build_slot(const Sink &out, int LT, int LS, int itype, const double *rt, double rimed,
-                           const double *__restrict__ vcc, int LM, bool sym = false)
+// LMX >= LT, LS bounds the work arrays; the direct tables pass their order
+// as LMX and as LT = LS, so that every loop has constant bounds and the
+// arrays can live in registers instead of scratch memory.
+template <int LMX, typename Sink>
+__device__ __forceinline__ void build_slot(const Sink &out, int LT, int LS, int itype, const double *rt, double rimed,
+                                           const double *__restrict__ vcc, int LM, bool sym = false)
 {
     const int nmax = imx(LT, LS), nmin = imn(LT, LS);
     double r = sqrt(rt[0] * rt[0] + rt[1] * rt[1] + rt[2] * rt[2]);
@@ -51,10 +54,10 @@ __device__ void build_slot(const Sink &out, int LT, int LS, int itype, const dou
     // ---- Wigner d rows, degree by degree --------------------------------
     double sinb = (fabs(ct) >= 1.0) ? 0.0 : sqrt(fabs((1.0 + ct) * (1.0 - ct)));
     double cos2h = 0.5 * (1.0 + ct), sin2h = 0.5 * (1.0 - ct);
-    double row_cur[2 * MSTM_TAB_L + 3], row_prev[2 * MSTM_TAB_L + 3];
-    double rowp[2 * MSTM_TAB_L + 1], rowc[2 * MSTM_TAB_L + 1];
-    const int mid = MSTM_TAB_L + 1;
-    for (int q = 0; q < 2 * MSTM_TAB_L + 3; q++) row_cur[q] = row_prev[q] = 0.0;
+    double row_cur[2 * LMX + 3], row_prev[2 * LMX + 3];
+    double rowp[2 * LMX + 1], rowc[2 * LMX + 1];
+    const int mid = LMX + 1;
+    for (int q = 0; q < 2 * LMX + 3; q++) row_cur[q] = row_prev[q] = 0.0;
     row_cur[mid] = 1.0;
     int sgn_n = 1;
     double sinpow = 1.0, binom = 1.0;  // binom = bcof(n,n) = sqrt((2n)!/(n!)^2)
@@ -124,18 +127,18 @@ __device__ void build_slot(const Sink &out, int LT, int LS, int itype, const dou
         }
     }
     // ---- axial coefficients ---------------------------------------------
-    cplx<double> hw[2 * MSTM_TAB_L + 1];
+    cplx<double> hw[2 * LMX + 1];
     const int nw = LT + LS;
     double z = r * rimed;
     if (itype == 3) {
         sph_hankel_up(nw, z, hw);
     } else {
-        double jw[2 * MSTM_TAB_L + 1];
+        double jw[2 * LMX + 1];
         sph_bessel_j(nw, z, jw);
         for (int w = 0; w <= nw; w++) hw[w] = cmk(jw[w], 0.0);
     }
-    cplx<double> act[(MSTM_TAB_L + 1) * (MSTM_TAB_L + 1)][2];
-#define ACT(n, l) act[(n) * (MSTM_TAB_L + 1) + (l)]
+    cplx<double> act[(LMX + 1) * (LMX + 1)][2];
+#define ACT(n, l) act[(n) * (LMX + 1) + (l)]
     for (int b = 0; b < 2 * nmin + 1; b++) {
         int m = mblock_m(b);
         if (sym && m < 0) continue;  // A_{-m} of one mode is A_m of the other
@@ -194,11 +197,14 @@ __device__ void build_slot(const Sink &out, int LT, int LS, int itype, const dou
 // Direct-path tables: slot q = (tile*S + step)*S + k (S = MSTM_SLOTS) holds
 // the pair target = blockI*S + k, source = blockJ*S + (k+step)%S (diagonal
 // tiles have blockJ = blockI); self pairs and padding spheres get zeros.
-template <typename T>
-__global__ void direct_tables_kernel(T *__restrict__ tab, const int2 *__restrict__ tiles, int ntiles, int L, int nsph,
+// LMX: the order when it is one of the compiled ones (1..6), else
+// MSTM_TAB_L with the order L taken at run time
+template <typename T, int LMX>
+__global__ void direct_tables_kernel(T *__restrict__ tab, const int2 *__restrict__ tiles, int ntiles, int L_, int nsph,
                                      const double *__restrict__ pos, double rimed, const double *__restrict__ vcc,
                                      int LM, int64_t slot_elems)
 {
+    const int L = LMX < MSTM_TAB_L ? LMX : L_;
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (int64_t)ntiles * MSTM_SLOTS * MSTM_SLOTS) return;
     int k = (int)(q % MSTM_SLOTS);
@@ -215,17 +221,19 @@ __global__ void direct_tables_kernel(T *__restrict__ tab, const int2 *__restrict
         return;
     }
     double rt[3] = {pos[3 * it] - pos[3 * is], pos[3 * it + 1] - pos[3 * is + 1], pos[3 * it + 2] - pos[3 * is + 2]};
-    build_slot(OneTable<T>{grp, k}, L, L, 3, rt, rimed, vcc, LM, direct_sym(L, (int)sizeof(T)));
+    build_slot<LMX>(OneTable<T>{grp, k}, L, L, 3, rt, rimed, vcc, LM, direct_sym(L, (int)sizeof(T)));
 }
 
 // Both direct tables, FP64 and FP32, in one pass (orders with the same
 // layout kind at both precisions: symmetric up to 4, full from 6): each
 // slot's FP64 values are worked out once and stored twice.
+template <int LMX>
 __global__ void direct_tables_both_kernel(double *__restrict__ tab64, float *__restrict__ tab32,
-                                          const int2 *__restrict__ tiles, int ntiles, int L, int nsph,
+                                          const int2 *__restrict__ tiles, int ntiles, int L_, int nsph,
                                           const double *__restrict__ pos, double rimed,
                                           const double *__restrict__ vcc, int LM, int64_t se64, int64_t se32)
 {
+    const int L = LMX < MSTM_TAB_L ? LMX : L_;
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (int64_t)ntiles * MSTM_SLOTS * MSTM_SLOTS) return;
     int k = (int)(q % MSTM_SLOTS);
@@ -243,7 +251,7 @@ __global__ void direct_tables_both_kernel(double *__restrict__ tab64, float *__r
         return;
     }
     double rt[3] = {pos[3 * it] - pos[3 * is], pos[3 * it + 1] - pos[3 * is + 1], pos[3 * it + 2] - pos[3 * is + 2]};
-    build_slot(BothTables{g64, g32, k}, L, L, 3, rt, rimed, vcc, LM, direct_sym(L, 8));
+    build_slot<LMX>(BothTables{g64, g32, k}, L, L, 3, rt, rimed, vcc, LM, direct_sym(L, 8));
 }
 
 template <typename T>
@@ -255,8 +263,18 @@ hipError_t launch_direct_tables(T *tab, const int2 *tiles, int ntiles, int L, in
     int64_t nq = (int64_t)ntiles * MSTM_SLOTS * MSTM_SLOTS;
     int bs = 128;
     int64_t nb = (nq + bs - 1) / bs;
-    hipLaunchKernelGGL(direct_tables_kernel<T>, dim3((unsigned)nb), dim3(bs), 0, st, tab, tiles, ntiles, L, nsph, pos,
-                       rimed, vcc, LM, se);
+    switch (L) {
+#define DT(LL)                                                                                                         \
+    case LL:                                                                                                           \
+        hipLaunchKernelGGL((direct_tables_kernel<T, LL>), dim3((unsigned)nb), dim3(bs), 0, st, tab, tiles, ntiles, L,   \
+                           nsph, pos, rimed, vcc, LM, se);                                                             \
+        break;
+        DT(1) DT(2) DT(3) DT(4) DT(5) DT(6)
+#undef DT
+    default:
+        hipLaunchKernelGGL((direct_tables_kernel<T, MSTM_TAB_L>), dim3((unsigned)nb), dim3(bs), 0, st, tab, tiles,
+                           ntiles, L, nsph, pos, rimed, vcc, LM, se);
+    }
     return hipGetLastError();
 }
 template hipError_t launch_direct_tables<float>(float *, const int2 *, int, int, int, const double *, double,
@@ -272,9 +290,20 @@ hipError_t launch_direct_tables_both(double *tab64, float *tab32, const int2 *ti
     if (!direct_tables_both_ok(L)) return hipErrorInvalidValue;
     const int64_t nq = (int64_t)ntiles * MSTM_SLOTS * MSTM_SLOTS;
     const int bs = 128;
-    hipLaunchKernelGGL(direct_tables_both_kernel, dim3((unsigned)((nq + bs - 1) / bs)), dim3(bs), 0, st, tab64, tab32,
-                       tiles, ntiles, L, nsph, pos, rimed, vcc, LM, (int64_t)direct_slot_len(L, 8),
-                       (int64_t)direct_slot_len(L, 4));
+    const dim3 g((unsigned)((nq + bs - 1) / bs));
+    const int64_t s64 = direct_slot_len(L, 8), s32 = direct_slot_len(L, 4);
+    switch (L) {
+#define DT(LL)                                                                                                         \
+    case LL:                                                                                                           \
+        hipLaunchKernelGGL((direct_tables_both_kernel<LL>), g, dim3(bs), 0, st, tab64, tab32, tiles, ntiles, L, nsph,   \
+                           pos, rimed, vcc, LM, s64, s32);                                                             \
+        break;
+        DT(1) DT(2) DT(3) DT(4) DT(6)
+#undef DT
+    default:
+        hipLaunchKernelGGL((direct_tables_both_kernel<MSTM_TAB_L>), g, dim3(bs), 0, st, tab64, tab32, tiles, ntiles, L,
+                           nsph, pos, rimed, vcc, LM, s64, s32);
+    }
     return hipGetLastError();
 }
 
@@ -296,7 +325,7 @@ __global__ void list_tables_kernel(T *__restrict__ tab, const double *__restrict
         for (int e = 0; e < slot_elems; e++) grp[ilv(e, k, VW)] = (T)0;
         return;
     }
-    build_slot(OneTable<T>{grp, k}, LT, LS, itype, rt, rimed, vcc, LM);
+    build_slot<MSTM_TAB_L>(OneTable<T>{grp, k}, LT, LS, itype, rt, rimed, vcc, LM);
 }
 
 template <typename T>
