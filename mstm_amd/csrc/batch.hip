@@ -146,6 +146,8 @@ __global__ __launch_bounds__(256) void expand_kernel(const double *__restrict__ 
 // Grouped complex GEMM, FP32 on MFMA: for every job, a 64-row x R tile of
 // Y = H X (op 0, inputs in0 -> out0) or Y = H^H X (op 1, in1 -> out1) of
 // cluster c, mode p.  Persistent: each workgroup takes jobs from a counter.
+// cact (optional): jobs of a cluster with cact[c] == 0 are skipped (inside
+// the inner solve: every column of that cluster has stopped).
 // 4 waves per workgroup, each a 32x32 quadrant; complex products as four
 // real MFMAs (Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br).
 // ---------------------------------------------------------------------------
@@ -154,7 +156,8 @@ __global__ __launch_bounds__(256) void gemm32_kernel(const GemmJob *__restrict__
                                                      const int64_t *__restrict__ choff,
                                                      const int64_t *__restrict__ cvoff, const Z32 *__restrict__ H,
                                                      const Z32 *__restrict__ in0, Z32 *__restrict__ out0,
-                                                     const Z32 *__restrict__ in1, Z32 *__restrict__ out1, int R)
+                                                     const Z32 *__restrict__ in1, Z32 *__restrict__ out1, int R,
+                                                     const int *__restrict__ cact)
 {
     __shared__ float As[2][64][33];
     __shared__ float Bs[2][32][65];
@@ -168,6 +171,7 @@ __global__ __launch_bounds__(256) void gemm32_kernel(const GemmJob *__restrict__
         __syncthreads();
         if (jb >= njobs) break;
         const GemmJob J = jobs[jb];
+        if (cact && !cact[J.c]) continue;  // every column of the cluster has stopped
         const int M = cM[J.c];
         const Z32 *Hc = H + choff[J.c] + (int64_t)J.mode * M * M;
         const int64_t vb = (cvoff[J.c] + (int64_t)J.mode * M) * R;
@@ -240,7 +244,8 @@ __global__ __launch_bounds__(256) void gemm32b_kernel(const GemmJob *__restrict_
                                                       const int64_t *__restrict__ choff,
                                                       const int64_t *__restrict__ cvoff, const Z32 *__restrict__ H,
                                                       const Z32 *__restrict__ in0, Z32 *__restrict__ out0,
-                                                      const Z32 *__restrict__ in1, Z32 *__restrict__ out1, int R)
+                                                      const Z32 *__restrict__ in1, Z32 *__restrict__ out1, int R,
+                                                      const int *__restrict__ cact)
 {
     typedef float float4v __attribute__((ext_vector_type(4)));
     __shared__ __align__(16) float As[2][2][64][36];
@@ -255,6 +260,7 @@ __global__ __launch_bounds__(256) void gemm32b_kernel(const GemmJob *__restrict_
         __syncthreads();
         if (jb >= njobs) break;
         const GemmJob J = jobs[jb];
+        if (cact && !cact[J.c]) continue;  // every column of the cluster has stopped
         const int M = cM[J.c];
         const Z32 *Hc = H + choff[J.c] + (int64_t)J.mode * M * M;
         const int64_t vb = (cvoff[J.c] + (int64_t)J.mode * M) * R;
@@ -702,6 +708,16 @@ __global__ void bt_outer_kernel(int ncl, int R, const int *__restrict__ ch0, con
     if (w) atomicAdd(nwant, 1);
 }
 
+// cact[c] = 1 while any column of cluster c is active in the inner solve
+__global__ void cluster_active_kernel(int ncl, const int *__restrict__ active, int *__restrict__ cact)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncl) return;
+    int a = 0;
+    for (int j = 0; j < BR; j++) a |= active[c * BR + j];
+    cact[c] = a != 0;
+}
+
 __global__ void count_active_kernel(int n, const int *__restrict__ active, int *__restrict__ out)
 {
     __shared__ int s;
@@ -820,6 +836,7 @@ struct mstm_hip_batch {
     Z64 *partials = nullptr;
     // column state [ncl * BR]
     int *d_active = nullptr, *d_want = nullptr, *d_iters = nullptr, *d_itot = nullptr, *d_cnt = nullptr;
+    int *d_cact = nullptr;  // per cluster: any column active in the inner solve
     Z64 *d_csk = nullptr, *d_cak = nullptr, *d_cbk = nullptr;
     double *d_enorm = nullptr, *d_epsin = nullptr, *d_err = nullptr, *d_oerr = nullptr;
     // graph of GRAPH_K inner iterations
@@ -879,7 +896,7 @@ static void batch_release(mstm_hip_batch *b)
     bfree(b->r32), bfree(b->d32), bfree(b->q32), bfree(b->w32), bfree(b->p32), bfree(b->y1), bfree(b->y2),
         bfree(b->tw);
     bfree(b->partials);
-    bfree(b->d_active), bfree(b->d_want), bfree(b->d_iters), bfree(b->d_itot), bfree(b->d_cnt);
+    bfree(b->d_active), bfree(b->d_want), bfree(b->d_iters), bfree(b->d_itot), bfree(b->d_cnt), bfree(b->d_cact);
     bfree(b->d_csk), bfree(b->d_cak), bfree(b->d_cbk), bfree(b->d_enorm), bfree(b->d_epsin), bfree(b->d_err),
         bfree(b->d_oerr);
     b->built = false;
@@ -889,15 +906,15 @@ static inline int grid_of(int64_t n, int bs) { return (int)((n + bs - 1) / bs); 
 
 // one grouped FP32 GEMM: y1 = H p (op 0) and y2 = H^H tw (op 1)
 static hipError_t gemm32(mstm_hip_batch *b, const Z32 *in0, Z32 *out0, const Z32 *in1, Z32 *out1, int grid,
-                         const GemmJob *jobs)
+                         const GemmJob *jobs, const int *cact = nullptr)
 {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, b->st, b->d_counter);
     if (b->kvar == 0)
         hipLaunchKernelGGL(gemm32_kernel, dim3(std::min(grid, b->njobs32)), dim3(256), 0, b->st, jobs, b->njobs32,
-                           b->d_counter, b->d_cM, b->d_choff, b->d_cvoff, b->d_H32, in0, out0, in1, out1, b->R);
+                           b->d_counter, b->d_cM, b->d_choff, b->d_cvoff, b->d_H32, in0, out0, in1, out1, b->R, cact);
     else
         hipLaunchKernelGGL(gemm32b_kernel, dim3(std::min(grid, b->njobs32)), dim3(256), 0, b->st, jobs, b->njobs32,
-                           b->d_counter, b->d_cM, b->d_choff, b->d_cvoff, b->d_H32, in0, out0, in1, out1, b->R);
+                           b->d_counter, b->d_cM, b->d_choff, b->d_cvoff, b->d_H32, in0, out0, in1, out1, b->R, cact);
     return hipGetLastError();
 }
 
@@ -916,7 +933,10 @@ static hipError_t inner_iteration(mstm_hip_batch *b)
     const int64_t ne = (int64_t)b->ng * R;
     hipLaunchKernelGGL(tmulh32_kernel, dim3(grid_of(ne, 256)), dim3(256), 0, b->st, b->ng, b->d_ri, b->d_an, R,
                        b->w32, b->tw);
-    hipError_t e = gemm32(b, b->p32, b->y1, b->tw, b->y2, b->grid32, b->d_jobs32[b->lpt]);
+    // the products of clusters whose columns have all stopped are skipped
+    hipLaunchKernelGGL(cluster_active_kernel, dim3(grid_of(ncl, 256)), dim3(256), 0, b->st, ncl, b->d_active,
+                       b->d_cact);
+    hipError_t e = gemm32(b, b->p32, b->y1, b->tw, b->y2, b->grid32, b->d_jobs32[b->lpt], b->d_cact);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(bt_post_kernel, dim3(b->nchunks), dim3(256), 0, b->st, b->d_chunks, b->d_ri, b->d_an, R,
                        b->d_active, b->p32, b->w32, b->y1, b->y2, b->partials);
@@ -1274,6 +1294,7 @@ int mstm_hip_batch_set(mstm_hip_batch *b, int ncluster, const int *counts, const
     BCHK(balloc(&b->partials, sizeof(Z64) * 2 * (size_t)b->nchunks * b->R));
     const size_t ns = (size_t)ncluster * BR;
     BCHK(balloc(&b->d_active, sizeof(int) * ns));
+    BCHK(balloc(&b->d_cact, sizeof(int) * (size_t)ncluster));
     BCHK(balloc(&b->d_want, sizeof(int) * ns));
     BCHK(balloc(&b->d_iters, sizeof(int) * ns));
     BCHK(balloc(&b->d_itot, sizeof(int) * ns));
