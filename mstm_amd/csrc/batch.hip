@@ -146,8 +146,10 @@ __global__ __launch_bounds__(256) void expand_kernel(const double *__restrict__ 
 // Grouped complex GEMM, FP32 on MFMA: for every job, a 64-row x R tile of
 // Y = H X (op 0, inputs in0 -> out0) or Y = H^H X (op 1, in1 -> out1) of
 // cluster c, mode p.  Persistent: each workgroup takes jobs from a counter.
-// cact (optional): jobs of a cluster with cact[c] == 0 are skipped (inside
-// the inner solve: every column of that cluster has stopped).
+// cact (optional, inside the inner solve): cact[2c + h] = 0 when every
+// column of half h (columns 32h .. 32h + 31) of cluster c has stopped; the
+// waves of that half skip their products, and a job with both halves
+// stopped is skipped whole.
 // 4 waves per workgroup, each a 32x32 quadrant; complex products as four
 // real MFMAs (Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br).
 // ---------------------------------------------------------------------------
@@ -171,7 +173,8 @@ __global__ __launch_bounds__(256) void gemm32_kernel(const GemmJob *__restrict__
         __syncthreads();
         if (jb >= njobs) break;
         const GemmJob J = jobs[jb];
-        if (cact && !cact[J.c]) continue;  // every column of the cluster has stopped
+        if (cact && !(cact[2 * J.c] | cact[2 * J.c + 1])) continue;  // every column has stopped
+        const bool on = !cact || cact[2 * J.c + wc];                   // this wave's 32 columns
         const int M = cM[J.c];
         const Z32 *Hc = H + choff[J.c] + (int64_t)J.mode * M * M;
         const int64_t vb = (cvoff[J.c] + (int64_t)J.mode * M) * R;
@@ -213,15 +216,17 @@ __global__ __launch_bounds__(256) void gemm32_kernel(const GemmJob *__restrict__
                 Bs[1][kk][cc] = v.im;
             }
             __syncthreads();
+            if (on) {
 #pragma unroll
-            for (int s = 0; s < 16; s++) {
-                const int ka = 2 * s + (lane >> 5);
-                const float ar = As[0][wr * 32 + (lane & 31)][ka], ai = As[1][wr * 32 + (lane & 31)][ka];
-                const float br = Bs[0][ka][wc * 32 + (lane & 31)], bi = Bs[1][ka][wc * 32 + (lane & 31)];
-                cr = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, br, cr, 0, 0, 0);
-                cr = __builtin_amdgcn_mfma_f32_32x32x2f32(-ai, bi, cr, 0, 0, 0);
-                ci = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, bi, ci, 0, 0, 0);
-                ci = __builtin_amdgcn_mfma_f32_32x32x2f32(ai, br, ci, 0, 0, 0);
+                for (int s = 0; s < 16; s++) {
+                    const int ka = 2 * s + (lane >> 5);
+                    const float ar = As[0][wr * 32 + (lane & 31)][ka], ai = As[1][wr * 32 + (lane & 31)][ka];
+                    const float br = Bs[0][ka][wc * 32 + (lane & 31)], bi = Bs[1][ka][wc * 32 + (lane & 31)];
+                    cr = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, br, cr, 0, 0, 0);
+                    cr = __builtin_amdgcn_mfma_f32_32x32x2f32(-ai, bi, cr, 0, 0, 0);
+                    ci = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, bi, ci, 0, 0, 0);
+                    ci = __builtin_amdgcn_mfma_f32_32x32x2f32(ai, br, ci, 0, 0, 0);
+                }
             }
             __syncthreads();
         }
@@ -229,7 +234,7 @@ __global__ __launch_bounds__(256) void gemm32_kernel(const GemmJob *__restrict__
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             const int row = row0 + wr * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            if (row < M && col < R) Y[(int64_t)row * R + col] = cmk(cr[r], ci[r]);
+            if (on && row < M && col < R) Y[(int64_t)row * R + col] = cmk(cr[r], ci[r]);
         }
     }
 }
@@ -260,7 +265,8 @@ __global__ __launch_bounds__(256) void gemm32b_kernel(const GemmJob *__restrict_
         __syncthreads();
         if (jb >= njobs) break;
         const GemmJob J = jobs[jb];
-        if (cact && !cact[J.c]) continue;  // every column of the cluster has stopped
+        if (cact && !(cact[2 * J.c] | cact[2 * J.c + 1])) continue;  // every column has stopped
+        const bool on = !cact || cact[2 * J.c + wc];                   // this wave's 32 columns
         const int M = cM[J.c];
         const Z32 *Hc = H + choff[J.c] + (int64_t)J.mode * M * M;
         const int64_t vb = (cvoff[J.c] + (int64_t)J.mode * M) * R;
@@ -325,18 +331,20 @@ __global__ __launch_bounds__(256) void gemm32b_kernel(const GemmJob *__restrict_
         for (int ch = 0; ch < nch; ch++) {
             const int buf = ch & 1;
             if (ch + 1 < nch) fetch((ch + 1) * 32);
+            if (on) {
 #pragma unroll
-            for (int sb = 0; sb < 4; sb++) {
-                const float4v a_r = *reinterpret_cast<const float4v *>(&As[buf][0][arow][h * 16 + 4 * sb]);
-                const float4v a_i = *reinterpret_cast<const float4v *>(&As[buf][1][arow][h * 16 + 4 * sb]);
-                const float4v b_r = *reinterpret_cast<const float4v *>(&Bs[buf][0][bcol][h * 16 + 4 * sb]);
-                const float4v b_i = *reinterpret_cast<const float4v *>(&Bs[buf][1][bcol][h * 16 + 4 * sb]);
+                for (int sb = 0; sb < 4; sb++) {
+                    const float4v a_r = *reinterpret_cast<const float4v *>(&As[buf][0][arow][h * 16 + 4 * sb]);
+                    const float4v a_i = *reinterpret_cast<const float4v *>(&As[buf][1][arow][h * 16 + 4 * sb]);
+                    const float4v b_r = *reinterpret_cast<const float4v *>(&Bs[buf][0][bcol][h * 16 + 4 * sb]);
+                    const float4v b_i = *reinterpret_cast<const float4v *>(&Bs[buf][1][bcol][h * 16 + 4 * sb]);
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    cr = __builtin_amdgcn_mfma_f32_32x32x2f32(a_r[u], b_r[u], cr, 0, 0, 0);
-                    cr = __builtin_amdgcn_mfma_f32_32x32x2f32(-a_i[u], b_i[u], cr, 0, 0, 0);
-                    ci = __builtin_amdgcn_mfma_f32_32x32x2f32(a_r[u], b_i[u], ci, 0, 0, 0);
-                    ci = __builtin_amdgcn_mfma_f32_32x32x2f32(a_i[u], b_r[u], ci, 0, 0, 0);
+                    for (int u = 0; u < 4; u++) {
+                        cr = __builtin_amdgcn_mfma_f32_32x32x2f32(a_r[u], b_r[u], cr, 0, 0, 0);
+                        cr = __builtin_amdgcn_mfma_f32_32x32x2f32(-a_i[u], b_i[u], cr, 0, 0, 0);
+                        ci = __builtin_amdgcn_mfma_f32_32x32x2f32(a_r[u], b_i[u], ci, 0, 0, 0);
+                        ci = __builtin_amdgcn_mfma_f32_32x32x2f32(a_i[u], b_r[u], ci, 0, 0, 0);
+                    }
                 }
             }
             if (ch + 1 < nch) stash(buf ^ 1);
@@ -346,7 +354,7 @@ __global__ __launch_bounds__(256) void gemm32b_kernel(const GemmJob *__restrict_
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             const int row = row0 + wr * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            if (row < M && col < R) Y[(int64_t)row * R + col] = cmk(cr[r], ci[r]);
+            if (on && row < M && col < R) Y[(int64_t)row * R + col] = cmk(cr[r], ci[r]);
         }
     }
 }
@@ -708,14 +716,16 @@ __global__ void bt_outer_kernel(int ncl, int R, const int *__restrict__ ch0, con
     if (w) atomicAdd(nwant, 1);
 }
 
-// cact[c] = 1 while any column of cluster c is active in the inner solve
+// cact[2c + h] = 1 while any column of half h of cluster c (columns
+// 32h .. 32h + 31) is active in the inner solve
 __global__ void cluster_active_kernel(int ncl, const int *__restrict__ active, int *__restrict__ cact)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncl) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * ncl) return;
+    const int c = t >> 1, h = t & 1;
     int a = 0;
-    for (int j = 0; j < BR; j++) a |= active[c * BR + j];
-    cact[c] = a != 0;
+    for (int j = 32 * h; j < 32 * h + 32; j++) a |= active[c * BR + j];
+    cact[t] = a != 0;
 }
 
 __global__ void count_active_kernel(int n, const int *__restrict__ active, int *__restrict__ out)
@@ -836,7 +846,7 @@ struct mstm_hip_batch {
     Z64 *partials = nullptr;
     // column state [ncl * BR]
     int *d_active = nullptr, *d_want = nullptr, *d_iters = nullptr, *d_itot = nullptr, *d_cnt = nullptr;
-    int *d_cact = nullptr;  // per cluster: any column active in the inner solve
+    int *d_cact = nullptr;  // per cluster and half of its columns: any column active in the inner solve
     Z64 *d_csk = nullptr, *d_cak = nullptr, *d_cbk = nullptr;
     double *d_enorm = nullptr, *d_epsin = nullptr, *d_err = nullptr, *d_oerr = nullptr;
     // graph of GRAPH_K inner iterations
@@ -934,7 +944,7 @@ static hipError_t inner_iteration(mstm_hip_batch *b)
     hipLaunchKernelGGL(tmulh32_kernel, dim3(grid_of(ne, 256)), dim3(256), 0, b->st, b->ng, b->d_ri, b->d_an, R,
                        b->w32, b->tw);
     // the products of clusters whose columns have all stopped are skipped
-    hipLaunchKernelGGL(cluster_active_kernel, dim3(grid_of(ncl, 256)), dim3(256), 0, b->st, ncl, b->d_active,
+    hipLaunchKernelGGL(cluster_active_kernel, dim3(grid_of(2 * ncl, 256)), dim3(256), 0, b->st, ncl, b->d_active,
                        b->d_cact);
     hipError_t e = gemm32(b, b->p32, b->y1, b->tw, b->y2, b->grid32, b->d_jobs32[b->lpt], b->d_cact);
     if (e != hipSuccess) return e;
@@ -1294,7 +1304,7 @@ int mstm_hip_batch_set(mstm_hip_batch *b, int ncluster, const int *counts, const
     BCHK(balloc(&b->partials, sizeof(Z64) * 2 * (size_t)b->nchunks * b->R));
     const size_t ns = (size_t)ncluster * BR;
     BCHK(balloc(&b->d_active, sizeof(int) * ns));
-    BCHK(balloc(&b->d_cact, sizeof(int) * (size_t)ncluster));
+    BCHK(balloc(&b->d_cact, sizeof(int) * 2 * (size_t)ncluster));
     BCHK(balloc(&b->d_want, sizeof(int) * ns));
     BCHK(balloc(&b->d_iters, sizeof(int) * ns));
     BCHK(balloc(&b->d_itot, sizeof(int) * ns));
