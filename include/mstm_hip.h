@@ -174,7 +174,8 @@ int mstm_hip_set_refinement_cycles(mstm_hip_ctx *ctx, int cycles);
 /* BiCG iterations of small clusters (up to 500 spheres, direct operator) are
  * replayed four at a time from a HIP graph captured once per set of buffers
  * (no counterpart in MSTM).  on = 0 issues the same launches one by one, with
- * identical results: for tools that do not follow graph launches.  Default 1. */
+ * identical results: for tools that do not follow graph launches.  Default 1;
+ * a context whose stream cannot be captured switches to 0 by itself. */
 int mstm_hip_set_graphs(mstm_hip_ctx *ctx, int on);
 
 /* ---------------------------------------------------------------------------

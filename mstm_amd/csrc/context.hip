@@ -857,27 +857,31 @@ static int bicg_graph_steps(mstm_hip_ctx *c, int ns, int prec, const bool act[2]
         const long mv0 = c->stats.matvecs[prec];
         hipGraph_t gr = nullptr;
         int rc = 0;
+        bool graph_ok = false;  // capture and instantiation both worked
         hipError_t e = hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
             for (int q = 0; q < K && !rc; q++) rc = step();
             const hipError_t e2 = hipStreamEndCapture(c->st, &gr);
-            if (!rc && e2 != hipSuccess)
-                rc = fail(c, MSTM_HIP_EHIP, "BiCG graph capture: %s", hipGetErrorString(e2));
-        } else {
-            rc = fail(c, MSTM_HIP_EHIP, "BiCG graph capture: %s", hipGetErrorString(e));
+            if (!rc && e2 == hipSuccess) {
+                graph_ok = hipGraphInstantiate(&g.exec, gr, nullptr, nullptr, 0) == hipSuccess;
+                if (!graph_ok) g.exec = nullptr;
+            }
         }
         c->timing = timing;
         g.matvecs = (int)(c->stats.matvecs[prec] - mv0);
         c->stats.matvecs[prec] = mv0;
-        if (!rc) {
-            e = hipGraphInstantiate(&g.exec, gr, nullptr, nullptr, 0);
-            if (e != hipSuccess) {
-                g.exec = nullptr;
-                rc = fail(c, MSTM_HIP_EHIP, "BiCG graph instantiate: %s", hipGetErrorString(e));
-            }
-        }
         if (gr) (void)hipGraphDestroy(gr);
-        if (rc) return rc;
+        if (rc) return rc;  // a launch of the step itself failed
+        if (!graph_ok) {
+            // the stream could not be captured here (another capture, a tool
+            // that intercepts graphs): this context issues the same launches
+            // one by one from now on
+            (void)hipGetLastError();
+            c->graphs = false;
+            for (int q = 0; q < K; q++)
+                if ((rc = step())) return rc;
+            return 0;
+        }
         g.key = key;
     }
     HIPCHK(hipGraphLaunch(g.exec, c->st));
