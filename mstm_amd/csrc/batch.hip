@@ -147,9 +147,9 @@ __global__ __launch_bounds__(256) void expand_kernel(const double *__restrict__ 
 // Y = H X (op 0, inputs in0 -> out0) or Y = H^H X (op 1, in1 -> out1) of
 // cluster c, mode p.  Persistent: each workgroup takes jobs from a counter.
 // cact (optional, inside the inner solve): cact[2c + h] = 0 when every
-// column of half h (columns 32h .. 32h + 31) of cluster c has stopped; the
-// waves of that half skip their products, and a job with both halves
-// stopped is skipped whole.
+// column of half h (columns 32h .. 32h + 31) of cluster c has stopped; a job
+// with both halves stopped is skipped whole, and in the double-buffered form
+// the waves of a stopped half skip their products.
 // 4 waves per workgroup, each a 32x32 quadrant; complex products as four
 // real MFMAs (Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br).
 // ---------------------------------------------------------------------------
@@ -174,7 +174,6 @@ __global__ __launch_bounds__(256) void gemm32_kernel(const GemmJob *__restrict__
         if (jb >= njobs) break;
         const GemmJob J = jobs[jb];
         if (cact && !(cact[2 * J.c] | cact[2 * J.c + 1])) continue;  // every column has stopped
-        const bool on = !cact || cact[2 * J.c + wc];                   // this wave's 32 columns
         const int M = cM[J.c];
         const Z32 *Hc = H + choff[J.c] + (int64_t)J.mode * M * M;
         const int64_t vb = (cvoff[J.c] + (int64_t)J.mode * M) * R;
@@ -216,17 +215,15 @@ __global__ __launch_bounds__(256) void gemm32_kernel(const GemmJob *__restrict__
                 Bs[1][kk][cc] = v.im;
             }
             __syncthreads();
-            if (on) {
 #pragma unroll
-                for (int s = 0; s < 16; s++) {
-                    const int ka = 2 * s + (lane >> 5);
-                    const float ar = As[0][wr * 32 + (lane & 31)][ka], ai = As[1][wr * 32 + (lane & 31)][ka];
-                    const float br = Bs[0][ka][wc * 32 + (lane & 31)], bi = Bs[1][ka][wc * 32 + (lane & 31)];
-                    cr = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, br, cr, 0, 0, 0);
-                    cr = __builtin_amdgcn_mfma_f32_32x32x2f32(-ai, bi, cr, 0, 0, 0);
-                    ci = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, bi, ci, 0, 0, 0);
-                    ci = __builtin_amdgcn_mfma_f32_32x32x2f32(ai, br, ci, 0, 0, 0);
-                }
+            for (int s = 0; s < 16; s++) {
+                const int ka = 2 * s + (lane >> 5);
+                const float ar = As[0][wr * 32 + (lane & 31)][ka], ai = As[1][wr * 32 + (lane & 31)][ka];
+                const float br = Bs[0][ka][wc * 32 + (lane & 31)], bi = Bs[1][ka][wc * 32 + (lane & 31)];
+                cr = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, br, cr, 0, 0, 0);
+                cr = __builtin_amdgcn_mfma_f32_32x32x2f32(-ai, bi, cr, 0, 0, 0);
+                ci = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, bi, ci, 0, 0, 0);
+                ci = __builtin_amdgcn_mfma_f32_32x32x2f32(ai, br, ci, 0, 0, 0);
             }
             __syncthreads();
         }
@@ -234,7 +231,7 @@ __global__ __launch_bounds__(256) void gemm32_kernel(const GemmJob *__restrict__
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             const int row = row0 + wr * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            if (on && row < M && col < R) Y[(int64_t)row * R + col] = cmk(cr[r], ci[r]);
+            if (row < M && col < R) Y[(int64_t)row * R + col] = cmk(cr[r], ci[r]);
         }
     }
 }
