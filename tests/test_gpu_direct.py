@@ -362,12 +362,13 @@ def test_full_size_product_matches_oracle_on_a_subset(gpu_ctx):
     assert abs(lhs - rhs) < 1e-10 * abs(lhs)
 
 
-@pytest.mark.parametrize("x", [1.0, 2.5, 4.0])
+@pytest.mark.parametrize("x", [1.0, 2.0, 2.5, 4.0])
 def test_one_pass_tables_equal_separate_builds(gpu_ctx, x):
     """setup_direct(FP64_FP32) builds the FP64 and FP32 tables in one pass
     (orders up to 4 and from 6; order 5, symmetric only in FP32, takes two
     passes): the products of both precisions are bit for bit those of the
-    tables built one precision at a time."""
+    tables built one precision at a time.  x = 1, 2, 2.5, 4 give orders 4
+    (symmetric), 5 (two passes), 6 (compiled order) and 7 (run-time order)."""
     from mstm_amd import geometry as G
     from mstm_amd.solver import FP32, FP64, FP64_FP32
 
