@@ -36,3 +36,27 @@ def test_create_is_clean_with_or_without_gpu():
         assert L.mstm_hip_destroy(h) == 0
     else:
         assert not h.value
+
+
+def test_header_constants_match_python():
+    """The precision and solver codes of include/mstm_hip.h are the ones the
+    Python binding passes."""
+    import re
+
+    txt = open(lib.HEADER).read()
+    defs = {m.group(1): int(m.group(2)) for m in re.finditer(r"#define MSTM_HIP_(\w+)\s+(-?\d+)", txt)}
+    for name in ("FP64", "FP32", "FP64_FP32", "SOLVE_PLAIN", "SOLVE_MIXED", "SOLVE_LEARNED", "SOLVE_FGMRES",
+                 "SOLVE_LU"):
+        assert defs[name] == getattr(lib, name), name
+
+
+def test_bench_runs_on_one_gpu_only():
+    """bench.py --gpus N > 1 stops with a message instead of running."""
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120, cwd=root)
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr)
