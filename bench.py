@@ -247,7 +247,7 @@ def bench_fft(args):
                     traffic=None, algorithmic_bytes=alg_c, avg_launch_ms=round(t_c, 4), launches=int(nc),
                     flops_per_launch=2 * 2 * nblk * nblk * M * 8)
     conc = None
-    if len(dirs) > 1:
+    if len(dirs) > 1 and not args.no_concurrent:
         conc = orientations_concurrent(cl, dirs, args.steps, q_mix)
     t_plain, q_plain, it_plain = solve_all(SOLVE_PLAIN, True)
     ctx.setup_direct(FP64)
@@ -465,6 +465,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tight", action="store_true", help="skip the tight FP64 reference solve")
+    ap.add_argument("--no-concurrent", action="store_true",
+                    help="fractal: skip the orientations-at-once run (for kernel tracing: its launches overlap)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="tmatrix: issue the small-cluster BiCG launches one by one instead of replaying HIP graphs "
                          "(for kernel tracing; same results)")
